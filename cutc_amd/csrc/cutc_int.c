@@ -1,0 +1,715 @@
+/*
+ * C host layer of libcutcint (gfx950): the reference's cutc_int path behind
+ * its C ABI, driving the HIP kernels (kernels/ directory) through launch.h.
+ *
+ * Reference host code replaced here:
+ *   cutc_int               src/cutc_int.c:15-243
+ *   get_int2_grad1_u12_ao  src/build_int2_grad1_u12_ao.cu:9-347
+ *   get_int_2e_ao          src/build_int_2e_ao.cu:11-145
+ *   exported launch wrappers src/jast_bh.cu:268-303, src/long_range_integ.cu:52-66,
+ *     src/short_range_integ_herm.cu:51-63, src/short_range_integ_nonherm.cu:69-81,
+ *     src/add_trans_inplace.cu:24-34
+ *
+ * Differences that are deliberate: 64-bit sizes; a memory plan instead of
+ * the reference's chunk heuristic; every r1 chunk starts its contraction
+ * fresh (the reference keeps beta = 1, src/build_int2_grad1_u12_ao.cu:166);
+ * errors are returned instead of exit(0); kernels run back to back on one
+ * stream with timing events read once at the end.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "cutc_int.h"
+#include "kernels/launch.h"
+
+void cutc_set_last_error(int code);
+
+/* split-precision stage 1 (split.hip) */
+int cutc_stage1_split(int precision, int slices, const double *Apk, int64_t lda, int64_t P, const double *G,
+                      int64_t ldg, int64_t n1c, int64_t K, const int *pairs, int n_ao, double *X, int64_t n1,
+                      int64_t i0, void *work, size_t work_bytes, hipStream_t st, double *ms_split, int *n_launch);
+size_t cutc_split_work_bytes(int precision, int slices, int64_t P, int64_t n1c, int64_t K);
+int cutc_stage2_split(int precision, int slices, const double *X, int64_t xld, const double *B, int64_t ldb,
+                      int64_t P, int64_t K, double alpha, const int *pairs, int n_ao, double *M, int mode,
+                      void *work, size_t work_bytes, hipStream_t st);
+size_t cutc_split2_work_bytes(int precision, int slices, int64_t P, int64_t K);
+
+#define NCAT 7
+enum { CAT_JAST = 0, CAT_INTEG, CAT_GEMM1, CAT_GEMM2, CAT_SYMM, CAT_SPLIT, CAT_TOTAL };
+
+typedef struct {
+    hipStream_t st;
+    int timing;
+    int nev;
+    int cap;
+    hipEvent_t *ev; /* pairs (start, stop) */
+    int *cat;
+    int launches[NCAT];
+} tracer_t;
+
+static void tr_init(tracer_t *t, hipStream_t st, int timing)
+{
+    memset(t, 0, sizeof *t);
+    t->st = st;
+    t->timing = timing;
+}
+
+static int tr_begin(tracer_t *t, int cat)
+{
+    t->launches[cat]++;
+    if (!t->timing)
+        return -1;
+    if (t->nev == t->cap) {
+        int nc = t->cap ? 2 * t->cap : 64;
+        hipEvent_t *ne = realloc(t->ev, sizeof(hipEvent_t) * 2 * nc);
+        int *nk = realloc(t->cat, sizeof(int) * nc);
+        if (!ne || !nk) {
+            free(ne ? ne : t->ev);
+            free(nk ? nk : t->cat);
+            t->ev = NULL;
+            t->cat = NULL;
+            t->timing = 0;
+            return -1;
+        }
+        t->ev = ne;
+        t->cat = nk;
+        t->cap = nc;
+    }
+    int i = t->nev;
+    if (hipEventCreate(&t->ev[2 * i]) != hipSuccess)
+        return -1;
+    if (hipEventCreate(&t->ev[2 * i + 1]) != hipSuccess) {
+        hipEventDestroy(t->ev[2 * i]);
+        return -1;
+    }
+    t->cat[i] = cat;
+    hipEventRecord(t->ev[2 * i], t->st);
+    t->nev++;
+    return i;
+}
+
+static void tr_end(tracer_t *t, int i)
+{
+    if (i >= 0)
+        hipEventRecord(t->ev[2 * i + 1], t->st);
+}
+
+/* After the stream is synchronised: per-category milliseconds. */
+static void tr_collect(tracer_t *t, double *ms)
+{
+    for (int c = 0; c < NCAT; c++)
+        ms[c] = 0.0;
+    for (int i = 0; i < t->nev; i++) {
+        float f = 0.0f;
+        if (hipEventElapsedTime(&f, t->ev[2 * i], t->ev[2 * i + 1]) == hipSuccess)
+            ms[t->cat[i]] += f;
+    }
+}
+
+static void tr_free(tracer_t *t)
+{
+    for (int i = 0; i < t->nev; i++) {
+        hipEventDestroy(t->ev[2 * i]);
+        hipEventDestroy(t->ev[2 * i + 1]);
+    }
+    free(t->ev);
+    free(t->cat);
+    t->ev = NULL;
+    t->cat = NULL;
+    t->nev = t->cap = 0;
+}
+
+static int hip_rc(hipError_t e)
+{
+    if (e == hipSuccess)
+        return CUTC_OK;
+    if (e == hipErrorOutOfMemory)
+        return CUTC_ERR_NOMEM;
+    return CUTC_ERR_HIP;
+}
+
+#define TRY(x)                                                                                                       \
+    do {                                                                                                             \
+        rc = (x);                                                                                                    \
+        if (rc != CUTC_OK)                                                                                           \
+            goto done;                                                                                               \
+    } while (0)
+#define TRYH(x) TRY(hip_rc(x))
+
+void cutc_default_options(cutc_options *opt)
+{
+    memset(opt, 0, sizeof *opt);
+    opt->precision = CUTC_PREC_FP64;
+    opt->slices = 4;
+}
+
+static int valid_precision(const cutc_options *o)
+{
+    if (o->precision == CUTC_PREC_FP64)
+        return 1;
+    if ((o->precision == CUTC_PREC_BF16 || o->precision == CUTC_PREC_FP16) && (o->slices == 3 || o->slices == 4))
+        return 1;
+    return 0;
+}
+
+/* Boys-Handy exponents must be non-negative; a^b is evaluated by repeated
+ * products, so large exponents are refused as meaningless (> 64). */
+static int check_exponents(const int *m, const int *n, const int *o, int64_t cnt)
+{
+    for (int64_t i = 0; i < cnt; i++)
+        if (m[i] < 0 || n[i] < 0 || o[i] < 0 || m[i] > 64 || n[i] > 64 || o[i] > 64)
+            return CUTC_ERR_ARG;
+    return CUTC_OK;
+}
+
+static int check_exponents_device(const int *dm, const int *dn, const int *dno, int64_t cnt)
+{
+    if (cnt <= 0)
+        return CUTC_OK;
+    int *h = malloc(sizeof(int) * 3 * cnt);
+    if (!h)
+        return CUTC_ERR_NOMEM;
+    int rc = CUTC_OK;
+    if (hipMemcpy(h, dm, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h + cnt, dn, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h + 2 * cnt, dno, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = CUTC_ERR_HIP;
+    if (rc == CUTC_OK)
+        rc = check_exponents(h, h + cnt, h + 2 * cnt, cnt);
+    free(h);
+    return rc;
+}
+
+static size_t device_budget(const cutc_options *o)
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess)
+        return 0;
+    const size_t margin = (size_t)1 << 30;
+    size_t b = fr > margin ? fr - margin : 0;
+    if (o && o->mem_limit && o->mem_limit < b)
+        b = o->mem_limit;
+    return b;
+}
+
+static void dfree(void *p)
+{
+    if (p)
+        hipFree(p);
+}
+
+/* ------------------------------------------------------------ stage 1 */
+
+static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
+                  int size_bh, const double *r1, const double *r2, const double *wr2, const double *rn,
+                  const double *aos2, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh, double *X,
+                  tracer_t *tr, cutc_stats *stats)
+{
+    const hipStream_t st = tr->st;
+    const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    int *pairs = NULL;
+    double *Apk = NULL, *G = NULL, *f2tab = NULL;
+    void *work = NULL;
+    size_t work_bytes = 0;
+    int rc = CUTC_OK, ev;
+
+    if (n1 == 0 || n_ao == 0)
+        return CUTC_OK;
+    if (n2 == 0) {
+        ev = tr_begin(tr, CAT_GEMM1);
+        rc = cutc_launch_fill(X, 4 * n1 * nao2, 0.0, st);
+        tr_end(tr, ev);
+        return rc;
+    }
+    TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
+    TRYH(hipMalloc((void **)&Apk, sizeof(double) * P * ldk));
+    TRYH(hipMalloc((void **)&G, sizeof(double) * 4 * n1c * ldk));
+    if (n_nuc > 32)
+        TRYH(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * ldk));
+    if (opt->precision != CUTC_PREC_FP64) {
+        work_bytes = cutc_split_work_bytes(opt->precision, opt->slices, P, n1c, ldk);
+        TRYH(hipMalloc(&work, work_bytes));
+    }
+    if (stats)
+        stats->workspace_bytes = sizeof(int) * 2 * P + sizeof(double) * (P * ldk + 4 * n1c * ldk) +
+                                 (f2tab ? sizeof(double) * n_nuc * ldk : 0) + work_bytes;
+
+    TRY(cutc_launch_pair_table(n_ao, pairs, st));
+    ev = tr_begin(tr, CAT_INTEG);
+    rc = cutc_launch_pair_weights(wr2, aos2, n2, 0, n2, ldk, n_ao, Apk, ldk, st);
+    tr_end(tr, ev);
+    TRY(rc);
+
+    int nchunks = 0;
+    for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
+        const int64_t n1e = (n1 - i0 < n1c) ? n1 - i0 : n1c;
+        nchunks++;
+        ev = tr_begin(tr, CAT_JAST);
+        rc = cutc_launch_jastrow(r1, r2, rn, c_bh, m_bh, n_bh, o_bh, n_nuc, size_bh, i0, n1e, 0, n2, ldk, G, ldk,
+                                 n1e * ldk, f2tab, st);
+        tr_end(tr, ev);
+        TRY(rc);
+        if (opt->precision == CUTC_PREC_FP64) {
+            ev = tr_begin(tr, CAT_GEMM1);
+            rc = cutc_gemm1_f64(Apk, ldk, G, ldk, P, n1e, ldk, pairs, n_ao, X, n1, i0, 0, st);
+            tr_end(tr, ev);
+            TRY(rc);
+        } else {
+            double ms = 0.0;
+            int nl = 0;
+            ev = tr_begin(tr, CAT_GEMM1);
+            rc = cutc_stage1_split(opt->precision, opt->slices, Apk, ldk, P, G, ldk, n1e, ldk, pairs, n_ao, X, n1,
+                                   i0, work, work_bytes, st, &ms, &nl);
+            tr_end(tr, ev);
+            TRY(rc);
+        }
+        if (stats) {
+            /* executed contraction work, FP64-equivalent: 2 x pairs x (4 n1e) x n2 */
+            stats->flops_exec += 2.0 * (double)P * 4.0 * (double)n1e * (double)n2;
+        }
+    }
+    if (stats) {
+        stats->n1_chunk = n1c;
+        stats->n1_chunks = nchunks;
+    }
+    TRYH(hipStreamSynchronize(st));
+done:
+    if (rc != CUTC_OK)
+        hipStreamSynchronize(st);
+    dfree(pairs);
+    dfree(Apk);
+    dfree(G);
+    dfree(f2tab);
+    dfree(work);
+    return rc;
+}
+
+/* ------------------------------------------------------------ stage 2 */
+
+static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int n_ao, const double *wr1,
+                  const double *aos1, const double *X, double *out, tracer_t *tr, cutc_stats *stats)
+{
+    const hipStream_t st = tr->st;
+    const int64_t P = pl->n_pairs, n1c = pl->n1_chunk_2e;
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    int *pairs = NULL;
+    double *Hp = NULL, *NHp = NULL;
+    void *work = NULL;
+    size_t work_bytes = 0;
+    int rc = CUTC_OK, ev;
+
+    if (n_ao == 0)
+        return CUTC_OK;
+    if (n1 == 0) {
+        ev = tr_begin(tr, CAT_GEMM2);
+        rc = cutc_launch_fill(out, nao2 * nao2, 0.0, st);
+        tr_end(tr, ev);
+        if (rc == CUTC_OK)
+            rc = hip_rc(hipStreamSynchronize(st));
+        return rc;
+    }
+    const int64_t rows = (n1c + 15) / 16 * 16;
+    const int64_t plane = rows * P;
+    TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
+    TRYH(hipMalloc((void **)&Hp, sizeof(double) * plane));
+    TRYH(hipMalloc((void **)&NHp, sizeof(double) * 3 * plane));
+    if (opt->precision != CUTC_PREC_FP64) {
+        work_bytes = cutc_split2_work_bytes(opt->precision, opt->slices, P, rows);
+        if (work_bytes)
+            TRYH(hipMalloc(&work, work_bytes));
+    }
+    TRY(cutc_launch_pair_table(n_ao, pairs, st));
+
+    for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
+        const int64_t n1e = (n1 - i0 < n1c) ? n1 - i0 : n1c;
+        const int64_t pad0 = n1e * P, padn = ((n1e + 15) / 16 * 16 - n1e) * P;
+        ev = tr_begin(tr, CAT_INTEG);
+        rc = cutc_launch_r1_integrands(wr1, aos1, n1, i0, n1e, n_ao, pairs, Hp, NHp, P, plane, st);
+        for (int m = 0; m < 4 && rc == CUTC_OK && padn > 0; m++)
+            rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
+        tr_end(tr, ev);
+        TRY(rc);
+        ev = tr_begin(tr, CAT_GEMM2);
+        if (opt->precision == CUTC_PREC_FP64) {
+            rc = cutc_gemm2_f64(X + (3 * n1 + i0) * nao2, nao2, Hp, P, P, n1e, 1.0, pairs, n_ao, out, i0 == 0 ? 0 : 1,
+                                st);
+            for (int m = 0; m < 3 && rc == CUTC_OK; m++)
+                rc = cutc_gemm2_f64(X + (m * n1 + i0) * nao2, nao2, NHp + m * plane, P, P, n1e, -0.5, pairs, n_ao,
+                                    out, 2, st);
+        } else {
+            rc = cutc_stage2_split(opt->precision, opt->slices, X + (3 * n1 + i0) * nao2, nao2, Hp, P, P, n1e, 1.0,
+                                   pairs, n_ao, out, i0 == 0 ? 0 : 1, work, work_bytes, st);
+            for (int m = 0; m < 3 && rc == CUTC_OK; m++)
+                rc = cutc_stage2_split(opt->precision, opt->slices, X + (m * n1 + i0) * nao2, nao2, NHp + m * plane,
+                                       P, P, n1e, -0.5, pairs, n_ao, out, 2, work, work_bytes, st);
+        }
+        tr_end(tr, ev);
+        TRY(rc);
+        if (stats)
+            stats->flops_exec += 2.0 * (double)P * (double)P * 4.0 * (double)n1e;
+    }
+    ev = tr_begin(tr, CAT_SYMM);
+    rc = cutc_launch_trans_inplace(out, nao2, st);
+    tr_end(tr, ev);
+    TRY(rc);
+    TRYH(hipStreamSynchronize(st));
+done:
+    if (rc != CUTC_OK)
+        hipStreamSynchronize(st);
+    dfree(pairs);
+    dfree(Hp);
+    dfree(NHp);
+    dfree(work);
+    return rc;
+}
+
+/* --------------------------------------------------------- entry points */
+
+static int check_sizes(int n1, int n2, int n_ao, int n_nuc, int size_bh)
+{
+    if (n1 < 0 || n2 < 0 || n_ao < 0 || n_nuc < 0 || size_bh < 0)
+        return CUTC_ERR_ARG;
+    return CUTC_OK;
+}
+
+static void fill_stats(cutc_stats *stats, tracer_t *tr)
+{
+    if (!stats)
+        return;
+    double ms[NCAT];
+    tr_collect(tr, ms);
+    stats->ms_jastrow = ms[CAT_JAST];
+    stats->ms_integrands = ms[CAT_INTEG];
+    stats->ms_gemm1 = ms[CAT_GEMM1];
+    stats->ms_gemm2 = ms[CAT_GEMM2];
+    stats->ms_symm = ms[CAT_SYMM];
+    stats->ms_split = ms[CAT_SPLIT];
+    stats->ms_total = ms[CAT_TOTAL];
+    stats->n_gemm1 = tr->launches[CAT_GEMM1];
+    stats->n_gemm2 = tr->launches[CAT_GEMM2];
+    stats->n_jastrow = tr->launches[CAT_JAST];
+}
+
+static int device_path(const cutc_options *opt, int stages, int n1, int n2, int n_ao, int n_nuc, int size_bh,
+                       const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
+                       const double *aos1, const double *aos2, const double *c_bh, const int *m_bh, const int *n_bh,
+                       const int *o_bh, double *X, double *out, hipStream_t st, cutc_stats *stats)
+{
+    int rc = check_sizes(n1, n2, n_ao, n_nuc, size_bh);
+    if (rc)
+        return rc;
+    if (!valid_precision(opt))
+        return CUTC_ERR_ARG;
+    if (n_ao > 0 && n1 > 0 && !X)
+        return CUTC_ERR_ARG;
+    if ((stages & 2) && n_ao > 0 && !out)
+        return CUTC_ERR_ARG;
+    if ((stages & 1) && n1 > 0 && n_ao > 0 && n2 > 0 &&
+        (!r1 || !r2 || !wr2 || !aos2 || (n_nuc > 0 && !rn) ||
+         ((int64_t)n_nuc * size_bh > 0 && (!c_bh || !m_bh || !n_bh || !o_bh))))
+        return CUTC_ERR_ARG;
+    if ((stages & 2) && n1 > 0 && n_ao > 0 && (!wr1 || !aos1))
+        return CUTC_ERR_ARG;
+    if ((stages & 1) && n1 > 0 && n2 > 0 && n_ao > 0) {
+        rc = check_exponents_device(m_bh, n_bh, o_bh, (int64_t)n_nuc * size_bh);
+        if (rc)
+            return rc;
+    }
+    cutc_plan pl;
+    rc = cutc_make_plan(n1, n2, n_ao, n_nuc, opt->precision, opt->slices, device_budget(opt), opt->n1_chunk, &pl);
+    if (rc)
+        return rc;
+    if (stats)
+        memset(stats, 0, sizeof *stats);
+    tracer_t tr;
+    tr_init(&tr, st, stats != NULL || opt->verbose);
+    int evt = tr_begin(&tr, CAT_TOTAL);
+    if (stages & 1) {
+        rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, size_bh, r1, r2, wr2, rn, aos2, c_bh, m_bh, n_bh, o_bh, X, &tr,
+                    stats);
+    }
+    if (rc == CUTC_OK && (stages & 2))
+        rc = stage2(opt, &pl, n1, n_ao, wr1, aos1, X, out, &tr, stats);
+    tr_end(&tr, evt);
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipStreamSynchronize(st));
+    else
+        hipStreamSynchronize(st);
+    if (rc == CUTC_OK) {
+        fill_stats(stats, &tr);
+        if (opt->verbose) {
+            double ms[NCAT];
+            tr_collect(&tr, ms);
+            if (stages & 1) {
+                printf("Ellapsed time for DGEMM to build int2_grad1_u12_ao = %.3f sec\n", ms[CAT_GEMM1] / 1000.0);
+                printf("Ellapsed time for int_long_range kernel = %.3f sec\n", ms[CAT_INTEG] / 1000.0);
+                printf("Ellapsed time for cutc_int_bh kernel = %.3f sec\n", ms[CAT_JAST] / 1000.0);
+            }
+            if (stages & 2) {
+                printf("Ellapsed time for DGEMM to build int_2e_ao = %.3f sec\n", ms[CAT_GEMM2] / 1000.0);
+                printf("Ellapsed time for addT = %.3f sec\n", ms[CAT_SYMM] / 1000.0);
+            }
+            fflush(stdout);
+        }
+    }
+    tr_free(&tr);
+    return rc;
+}
+
+int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
+                    const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
+                    const double *aos_data1, const double *aos_data2, const double *c_bh, const int *m_bh,
+                    const int *n_bh, const int *o_bh, double *int2_grad1_u12_ao, double *int_2e_ao, void *stream,
+                    cutc_stats *stats)
+{
+    cutc_options def;
+    if (!opt) {
+        cutc_default_options(&def);
+        opt = &def;
+    }
+    int rc = device_path(opt, 3, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats);
+    cutc_set_last_error(rc);
+    return rc;
+}
+
+void get_int2_grad1_u12_ao(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int n_grid1, int n_grid2, int n_ao, int n_nuc,
+                           int size_bh, double *r1, double *r2, double *wr2, double *rn, double *aos_data2,
+                           double *c_bh, int *m_bh, int *n_bh, int *o_bh, double *int2_grad1_u12_ao)
+{
+    (void)dimGrid;
+    (void)dimBlock;
+    cutc_options opt;
+    cutc_default_options(&opt);
+    int rc = device_path(&opt, 1, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, NULL, r2, wr2, rn, NULL, aos_data2,
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL);
+    cutc_set_last_error(rc);
+    if (rc)
+        fprintf(stderr, "get_int2_grad1_u12_ao: %s\n", cutc_error_string(rc));
+}
+
+void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double *int2_grad1_u12_ao,
+                   double *tc_int_2e_ao)
+{
+    cutc_options opt;
+    cutc_default_options(&opt);
+    int rc = device_path(&opt, 2, n_grid1, 0, n_ao, 0, 0, NULL, wr1, NULL, NULL, NULL, aos_data1, NULL, NULL, NULL,
+                         NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL);
+    cutc_set_last_error(rc);
+    if (rc)
+        fprintf(stderr, "get_int_2e_ao: %s\n", cutc_error_string(rc));
+}
+
+/* ---------------------------------------------------- host-array path */
+
+static int geometry_ok(int nx, int ny, int nz, int bx, int by, int bz)
+{
+    int dev = 0;
+    hipDeviceProp_t pr;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&pr, dev) != hipSuccess)
+        return CUTC_ERR_HIP;
+    if (nx < 0 || ny < 0 || nz < 0 || bx < 0 || by < 0 || bz < 0)
+        return CUTC_ERR_GEOMETRY;
+    if ((long long)bx * by * bz > pr.maxThreadsPerBlock) {
+        printf("Error: Too many threads per block!\n");
+        return CUTC_ERR_GEOMETRY;
+    }
+    if (nx > pr.maxGridSize[0] || ny > pr.maxGridSize[1] || nz > pr.maxGridSize[2]) {
+        printf("Error: Grid dimensions exceed device capabilities!\n");
+        return CUTC_ERR_GEOMETRY;
+    }
+    return CUTC_OK;
+}
+
+static int upload(void **d, const void *h, size_t bytes)
+{
+    *d = NULL;
+    if (bytes == 0)
+        return CUTC_OK;
+    if (!h)
+        return CUTC_ERR_ARG;
+    int rc = hip_rc(hipMalloc(d, bytes));
+    if (rc)
+        return rc;
+    return hip_rc(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+}
+
+static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nuc, int size_bh, const double *h_r1,
+                     const double *h_wr1, const double *h_r2, const double *h_wr2, const double *h_rn,
+                     const double *h_aos1, const double *h_aos2, const double *h_c, const int *h_m, const int *h_n,
+                     const int *h_o, double *h_X, double *h_out, cutc_stats *stats)
+{
+    int rc = check_sizes(n1, n2, n_ao, n_nuc, size_bh);
+    if (rc)
+        return rc;
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    const int64_t nt = (int64_t)n_nuc * size_bh;
+    if ((n1 > 0 && nao2 > 0 && !h_X) || (nao2 > 0 && !h_out))
+        return CUTC_ERR_ARG;
+    if (nt > 0 && (!h_m || !h_n || !h_o))
+        return CUTC_ERR_ARG;
+    if (nt > 0 && (rc = check_exponents(h_m, h_n, h_o, nt)) != CUTC_OK)
+        return rc;
+    double *r1 = NULL, *wr1 = NULL, *r2 = NULL, *wr2 = NULL, *rn = NULL, *aos1 = NULL, *aos2 = NULL, *c = NULL;
+    int *m = NULL, *n = NULL, *o = NULL;
+    double *X = NULL, *out = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL, e2 = NULL, e3 = NULL;
+    TRYH(hipEventCreate(&e0));
+    TRYH(hipEventCreate(&e1));
+    TRYH(hipEventCreate(&e2));
+    TRYH(hipEventCreate(&e3));
+    TRYH(hipEventRecord(e0, NULL));
+    TRY(upload((void **)&r1, h_r1, sizeof(double) * 3 * n1));
+    TRY(upload((void **)&wr1, h_wr1, sizeof(double) * n1));
+    TRY(upload((void **)&r2, h_r2, sizeof(double) * 3 * n2));
+    TRY(upload((void **)&wr2, h_wr2, sizeof(double) * n2));
+    TRY(upload((void **)&rn, h_rn, sizeof(double) * 3 * n_nuc));
+    TRY(upload((void **)&aos1, h_aos1, sizeof(double) * 4 * (size_t)n1 * n_ao));
+    TRY(upload((void **)&aos2, h_aos2, sizeof(double) * 4 * (size_t)n2 * n_ao));
+    TRY(upload((void **)&c, h_c, sizeof(double) * nt));
+    TRY(upload((void **)&m, h_m, sizeof(int) * nt));
+    TRY(upload((void **)&n, h_n, sizeof(int) * nt));
+    TRY(upload((void **)&o, h_o, sizeof(int) * nt));
+    if (n1 > 0 && nao2 > 0)
+        TRYH(hipMalloc((void **)&X, sizeof(double) * 4 * nao2 * n1));
+    if (nao2 > 0)
+        TRYH(hipMalloc((void **)&out, sizeof(double) * nao2 * nao2));
+    TRYH(hipEventRecord(e1, NULL));
+    TRY(device_path(opt, 3, n1, n2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos1, aos2, c, m, n, o, X, out, NULL,
+                    stats));
+    TRYH(hipEventRecord(e2, NULL));
+    if (X)
+        TRYH(hipMemcpy(h_X, X, sizeof(double) * 3 * nao2 * n1, hipMemcpyDeviceToHost));
+    if (out)
+        TRYH(hipMemcpy(h_out, out, sizeof(double) * nao2 * nao2, hipMemcpyDeviceToHost));
+    TRYH(hipEventRecord(e3, NULL));
+    TRYH(hipEventSynchronize(e3));
+    if (opt->verbose) {
+        float t_up = 0, t_comp = 0, t_down = 0;
+        hipEventElapsedTime(&t_up, e0, e1);
+        hipEventElapsedTime(&t_comp, e1, e2);
+        hipEventElapsedTime(&t_down, e2, e3);
+        printf("Ellapsed time for Device <-> Host transf = %.3f sec\n", (t_up + t_down) / 1000.0);
+        printf("Ellapsed (total) time on GPU for cutc_int = %.3f sec\n", (t_up + t_comp + t_down) / 1000.0);
+        fflush(stdout);
+    }
+done:
+    dfree(r1);
+    dfree(wr1);
+    dfree(r2);
+    dfree(wr2);
+    dfree(rn);
+    dfree(aos1);
+    dfree(aos2);
+    dfree(c);
+    dfree(m);
+    dfree(n);
+    dfree(o);
+    dfree(X);
+    dfree(out);
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    if (e2)
+        hipEventDestroy(e2);
+    if (e3)
+        hipEventDestroy(e3);
+    return rc;
+}
+
+int cutc_int(int nxBlocks, int nyBlocks, int nzBlocks, int blockxSize, int blockySize, int blockzSize, int n_grid1,
+             int n_grid2, int n_ao, int n_nuc, int size_bh, double *h_r1, double *h_wr1, double *h_r2, double *h_wr2,
+             double *h_rn, double *h_aos_data1, double *h_aos_data2, double *h_c_bh, int *h_m_bh, int *h_n_bh,
+             int *h_o_bh, double *h_int2_grad1_u12_ao, double *h_int_2e_ao)
+{
+    printf(" Computing TC-Integrals With CuTC\n");
+    int rc = geometry_ok(nxBlocks, nyBlocks, nzBlocks, blockxSize, blockySize, blockzSize);
+    if (rc == CUTC_OK) {
+        cutc_options opt;
+        cutc_default_options(&opt);
+        opt.verbose = 1;
+        rc = host_path(&opt, n_grid1, n_grid2, n_ao, n_nuc, size_bh, h_r1, h_wr1, h_r2, h_wr2, h_rn, h_aos_data1,
+                       h_aos_data2, h_c_bh, h_m_bh, h_n_bh, h_o_bh, h_int2_grad1_u12_ao, h_int_2e_ao, NULL);
+    }
+    cutc_set_last_error(rc);
+    return rc;
+}
+
+int cutc_int_ex(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
+                const double *h_r1, const double *h_wr1, const double *h_r2, const double *h_wr2, const double *h_rn,
+                const double *h_aos_data1, const double *h_aos_data2, const double *h_c_bh, const int *h_m_bh,
+                const int *h_n_bh, const int *h_o_bh, double *h_int2_grad1_u12_ao, double *h_int_2e_ao,
+                cutc_stats *stats)
+{
+    cutc_options def;
+    if (!opt) {
+        cutc_default_options(&def);
+        opt = &def;
+    }
+    int rc = valid_precision(opt) ? CUTC_OK : CUTC_ERR_ARG;
+    if (rc == CUTC_OK)
+        rc = host_path(opt, n_grid1, n_grid2, n_ao, n_nuc, size_bh, h_r1, h_wr1, h_r2, h_wr2, h_rn, h_aos_data1,
+                       h_aos_data2, h_c_bh, h_m_bh, h_n_bh, h_o_bh, h_int2_grad1_u12_ao, h_int_2e_ao, stats);
+    cutc_set_last_error(rc);
+    return rc;
+}
+
+/* ------------------------------------------ exported launch wrappers */
+
+static void wrapper_status(int rc, const char *name)
+{
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipDeviceSynchronize());
+    cutc_set_last_error(rc);
+    if (rc)
+        fprintf(stderr, "%s: %s\n", name, cutc_error_string(rc));
+}
+
+void cutc_int_bh(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int ii0, int n_grid1_eff, int n_grid1_tot, int jj0,
+                 int n_grid2_eff, int n_grid2_tot, int n_nuc, int size_bh, double *r1, double *r2, double *rn,
+                 double *c_bh, int *m_bh, int *n_bh, int *o_bh, double *grad1_u12)
+{
+    (void)dimGrid;
+    (void)dimBlock;
+    double *f2tab = NULL;
+    int rc = CUTC_OK;
+    if (n_nuc > 32 && n_grid2_eff > 0)
+        rc = hip_rc(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * (size_t)n_grid2_eff));
+    if (rc == CUTC_OK)
+        rc = cutc_launch_jastrow(r1, r2, rn, c_bh, m_bh, n_bh, o_bh, n_nuc, size_bh, ii0, n_grid1_eff, jj0,
+                                 n_grid2_eff, n_grid2_eff, grad1_u12, n_grid2_tot,
+                                 (int64_t)n_grid1_tot * n_grid2_tot, f2tab, NULL);
+    wrapper_status(rc, "cutc_int_bh");
+    dfree(f2tab);
+}
+
+void int_long_range(int jj0, int n_grid2_eff, int n_grid2_tot, int n_grid2, int n_ao, double *wr2, double *aos_data2,
+                    double *int_fct_long_range)
+{
+    wrapper_status(cutc_launch_long_range_ref(jj0, n_grid2_eff, n_grid2_tot, n_grid2, n_ao, wr2, aos_data2,
+                                              int_fct_long_range, NULL),
+                   "int_long_range");
+}
+
+void int_short_range_herm(int n_grid1, int n_ao, double *wr1, double *aos_data1, double *int_fct_short_range_herm)
+{
+    wrapper_status(cutc_launch_herm_ref(n_grid1, n_ao, wr1, aos_data1, int_fct_short_range_herm, NULL),
+                   "int_short_range_herm");
+}
+
+void int_short_range_nonherm(int n_grid1, int n_ao, double *wr1, double *aos_data1,
+                             double *int_fct_short_range_nonherm)
+{
+    wrapper_status(cutc_launch_nonherm_ref(n_grid1, n_ao, wr1, aos_data1, int_fct_short_range_nonherm, NULL),
+                   "int_short_range_nonherm");
+}
+
+void trans_inplace(double *data, int size)
+{
+    wrapper_status(cutc_launch_trans_inplace(data, size, NULL), "trans_inplace");
+}

@@ -1,0 +1,290 @@
+"""Host-side mirror of the reference's cutc_int interface over libcutcint.so.
+
+``cutc_int`` takes the same arguments with the same meaning as the reference's
+Fortran binding (``src/cutc_module.f90:10-37``, with the 6-integer launch
+geometry of the C definition ``src/cutc_int.c:15-20``) and returns the same
+two arrays; the device-pointer stage functions mirror
+``get_int2_grad1_u12_ao`` / ``get_int_2e_ao`` (``include/cutc_int.h:8-14``).
+Errors raise :class:`CutcError` carrying the library's negative status code.
+
+Everything here calls the HIP library; there is no CPU fallback.  torch is
+used only to own device memory and to hand over a stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+
+CUTC_OK = 0
+CUTC_ERR_GEOMETRY = -1
+CUTC_ERR_ARG = -2
+CUTC_ERR_NOMEM = -3
+CUTC_ERR_HIP = -4
+CUTC_ERR_UNSUPPORTED = -5
+
+PREC_FP64 = 0
+PREC_BF16 = 1
+PREC_FP16 = 2
+PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16}
+
+
+class CutcError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__(f"{where}: status {code} ({_error_string(code)})")
+
+
+class Dim3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint), ("y", ctypes.c_uint), ("z", ctypes.c_uint)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("slices", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("n1_chunk", ctypes.c_int), ("mem_limit", ctypes.c_size_t)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_jastrow", ctypes.c_double),
+                ("ms_integrands", ctypes.c_double), ("ms_gemm1", ctypes.c_double), ("ms_gemm2", ctypes.c_double),
+                ("ms_symm", ctypes.c_double), ("ms_split", ctypes.c_double), ("n_gemm1", ctypes.c_int),
+                ("n_gemm2", ctypes.c_int), ("n_jastrow", ctypes.c_int), ("n1_chunk", ctypes.c_int64),
+                ("n1_chunks", ctypes.c_int), ("flops_exec", ctypes.c_double), ("workspace_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Plan(ctypes.Structure):
+    _fields_ = [("n1_chunk", ctypes.c_int64), ("ldk", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
+                ("n1_chunk_2e", ctypes.c_int64), ("bytes_stage1", ctypes.c_size_t),
+                ("bytes_stage2", ctypes.c_size_t)]
+
+
+# every C symbol the public header declares
+HEADER_SYMBOLS = [
+    "cutc_int", "get_int2_grad1_u12_ao", "get_int_2e_ao", "checkCudaErrors", "checkCublasErrors", "cutc_int_bh",
+    "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
+    "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
+]
+
+_vp = ctypes.c_void_p
+_ci = ctypes.c_int
+_L = None
+
+
+def lib():
+    """The loaded libcutcint.so (raises if it is not built)."""
+    global _L
+    if _L is None:
+        L = ctypes.CDLL(_lib.lib_path("libcutcint.so"))
+        L.cutc_int.restype = _ci
+        L.cutc_int.argtypes = [_ci] * 11 + [_vp] * 13
+        L.cutc_int_ex.restype = _ci
+        L.cutc_int_ex.argtypes = [ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [ctypes.POINTER(Stats)]
+        L.cutc_int_device.restype = _ci
+        L.cutc_int_device.argtypes = ([ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [_vp]
+                                      + [ctypes.POINTER(Stats)])
+        L.get_int2_grad1_u12_ao.restype = None
+        L.get_int2_grad1_u12_ao.argtypes = [Dim3, Dim3] + [_ci] * 5 + [_vp] * 10
+        L.get_int_2e_ao.restype = None
+        L.get_int_2e_ao.argtypes = [_ci, _ci] + [_vp] * 4
+        L.cutc_int_bh.restype = None
+        L.cutc_int_bh.argtypes = [Dim3, Dim3] + [_ci] * 8 + [_vp] * 8
+        L.int_long_range.restype = None
+        L.int_long_range.argtypes = [_ci] * 5 + [_vp] * 3
+        L.int_short_range_herm.restype = None
+        L.int_short_range_herm.argtypes = [_ci, _ci, _vp, _vp, _vp]
+        L.int_short_range_nonherm.restype = None
+        L.int_short_range_nonherm.argtypes = [_ci, _ci, _vp, _vp, _vp]
+        L.trans_inplace.restype = None
+        L.trans_inplace.argtypes = [_vp, _ci]
+        L.cutc_get_last_error.restype = _ci
+        L.cutc_error_string.restype = ctypes.c_char_p
+        L.cutc_error_string.argtypes = [_ci]
+        L.cutc_default_options.restype = None
+        L.cutc_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.cutc_make_plan.restype = _ci
+        L.cutc_make_plan.argtypes = [ctypes.c_int64, ctypes.c_int64, _ci, _ci, _ci, _ci, ctypes.c_size_t,
+                                     ctypes.c_int64, ctypes.POINTER(Plan)]
+        _L = L
+    return _L
+
+
+def _error_string(code):
+    try:
+        return lib().cutc_error_string(code).decode()
+    except Exception:  # library not loadable: keep the message usable
+        return "unknown"
+
+
+def _check(rc, where):
+    if rc != CUTC_OK:
+        raise CutcError(rc, where)
+
+
+def options(precision="fp64", slices=4, verbose=False, n1_chunk=0, mem_limit=0) -> Options:
+    o = Options()
+    lib().cutc_default_options(ctypes.byref(o))
+    o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    o.slices = slices
+    o.verbose = int(verbose)
+    o.n1_chunk = n1_chunk
+    o.mem_limit = mem_limit
+    return o
+
+
+def make_plan(n1, n2, n_ao, n_nuc, budget, precision="fp64", slices=4, n1_chunk=0) -> Plan:
+    p = Plan()
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    _check(lib().cutc_make_plan(n1, n2, n_ao, n_nuc, prec, slices, budget, n1_chunk, ctypes.byref(p)),
+           "cutc_make_plan")
+    return p
+
+
+def _np_ptr(a, dtype):
+    assert a.dtype == dtype and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _host_inputs(sys):
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    i = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    return dict(r1=f(sys.r1), wr1=f(sys.wr1), r2=f(sys.r2), wr2=f(sys.wr2), rn=f(sys.rn), aos1=f(sys.aos1),
+                aos2=f(sys.aos2), c=f(sys.c_bh), m=i(sys.m_bh), n=i(sys.n_bh), o=i(sys.o_bh))
+
+
+def cutc_int(sys, geometry=(100, 1, 1, 32, 1, 1)):
+    """Reference entry point on host arrays.  Returns (X, int_2e_ao):
+    X numpy (3, n1, n_ao, n_ao) = Fortran int2_grad1_u12_ao(n_ao, n_ao, n1, 3),
+    int_2e_ao numpy (n_ao,)*4 = Fortran int_2e_ao(n_ao, n_ao, n_ao, n_ao)."""
+    h = _host_inputs(sys)
+    X = np.zeros((3, sys.n1, sys.n_ao, sys.n_ao))
+    I = np.zeros((sys.n_ao,) * 4)
+    rc = lib().cutc_int(*geometry, sys.n1, sys.n2, sys.n_ao, sys.n_nuc, sys.size_bh,
+                        *(_np_ptr(h[k], np.float64) for k in ("r1", "wr1", "r2", "wr2", "rn", "aos1", "aos2", "c")),
+                        *(_np_ptr(h[k], np.int32) for k in ("m", "n", "o")),
+                        _np_ptr(X, np.float64), _np_ptr(I, np.float64))
+    _check(rc, "cutc_int")
+    return X, I
+
+
+def cutc_int_ex(sys, opt: Options | None = None):
+    """cutc_int with options; returns (X[0:3], int_2e_ao, stats dict)."""
+    h = _host_inputs(sys)
+    X = np.zeros((3, sys.n1, sys.n_ao, sys.n_ao))
+    I = np.zeros((sys.n_ao,) * 4)
+    st = Stats()
+    rc = lib().cutc_int_ex(ctypes.byref(opt or options()), sys.n1, sys.n2, sys.n_ao, sys.n_nuc, sys.size_bh,
+                           *(_np_ptr(h[k], np.float64) for k in ("r1", "wr1", "r2", "wr2", "rn", "aos1", "aos2", "c")),
+                           *(_np_ptr(h[k], np.int32) for k in ("m", "n", "o")),
+                           _np_ptr(X, np.float64), _np_ptr(I, np.float64), ctypes.byref(st))
+    _check(rc, "cutc_int_ex")
+    return X, I, st.as_dict()
+
+
+# ---------------------------------------------------------------- device side
+
+def _tp(t):
+    """Device pointer of a contiguous torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclasses.dataclass
+class DeviceSystem:
+    """Inputs resident in HBM (torch tensors on cuda), same layouts as the host arrays."""
+    r1: object
+    wr1: object
+    r2: object
+    wr2: object
+    rn: object
+    aos1: object
+    aos2: object
+    c_bh: object
+    m_bh: object
+    n_bh: object
+    o_bh: object
+    n1: int
+    n2: int
+    n_ao: int
+    n_nuc: int
+    size_bh: int
+
+    @classmethod
+    def upload(cls, sys, device="cuda"):
+        import torch
+        h = _host_inputs(sys)
+        d = {k: torch.from_numpy(v).to(device) for k, v in h.items()}
+        return cls(d["r1"], d["wr1"], d["r2"], d["wr2"], d["rn"], d["aos1"], d["aos2"], d["c"], d["m"], d["n"],
+                   d["o"], sys.n1, sys.n2, sys.n_ao, sys.n_nuc, sys.size_bh)
+
+
+def alloc_outputs(ds, device="cuda"):
+    """(X, int_2e_ao) device buffers: X (4, n1, n_ao, n_ao), int_2e_ao (n_ao,)*4."""
+    import torch
+    X = torch.empty((4, ds.n1, ds.n_ao, ds.n_ao), dtype=torch.float64, device=device)
+    I = torch.empty((ds.n_ao,) * 4, dtype=torch.float64, device=device)
+    return X, I
+
+
+def cutc_int_device(ds, X, I, opt: Options | None = None, stream=None):
+    """Whole path on resident inputs, on ``stream`` (a torch stream or None
+    for torch's current stream).  Returns the library's stats dict."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    st = Stats()
+    rc = lib().cutc_int_device(ctypes.byref(opt or options()), ds.n1, ds.n2, ds.n_ao, ds.n_nuc, ds.size_bh,
+                               _tp(ds.r1), _tp(ds.wr1), _tp(ds.r2), _tp(ds.wr2), _tp(ds.rn), _tp(ds.aos1),
+                               _tp(ds.aos2), _tp(ds.c_bh), _tp(ds.m_bh), _tp(ds.n_bh), _tp(ds.o_bh), _tp(X), _tp(I),
+                               ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+    _check(rc, "cutc_int_device")
+    return st.as_dict()
+
+
+def get_int2_grad1_u12_ao(ds, X, grid=(1, 1, 1), block=(32, 1, 1)):
+    """Stage 1 into the device buffer X (4, n1, n_ao, n_ao)."""
+    lib().get_int2_grad1_u12_ao(Dim3(*grid), Dim3(*block), ds.n1, ds.n2, ds.n_ao, ds.n_nuc, ds.size_bh, _tp(ds.r1),
+                                _tp(ds.r2), _tp(ds.wr2), _tp(ds.rn), _tp(ds.aos2), _tp(ds.c_bh), _tp(ds.m_bh),
+                                _tp(ds.n_bh), _tp(ds.o_bh), _tp(X))
+    _check(lib().cutc_get_last_error(), "get_int2_grad1_u12_ao")
+
+
+def get_int_2e_ao(ds, X, I):
+    """Stage 2: int_2e_ao (n_ao,)*4 from X (4, n1, n_ao, n_ao)."""
+    lib().get_int_2e_ao(ds.n1, ds.n_ao, _tp(ds.wr1), _tp(ds.aos1), _tp(X), _tp(I))
+    _check(lib().cutc_get_last_error(), "get_int_2e_ao")
+
+
+def cutc_int_bh(ds, G, ii0, n1_eff, n1_tot, jj0, n2_eff, n2_tot):
+    """Reference-layout Jastrow block: G[m * n1_tot * n2_tot + i1 * n2_tot + i2]."""
+    lib().cutc_int_bh(Dim3(1, 1, 1), Dim3(32, 1, 1), ii0, n1_eff, n1_tot, jj0, n2_eff, n2_tot, ds.n_nuc,
+                      ds.size_bh, _tp(ds.r1), _tp(ds.r2), _tp(ds.rn), _tp(ds.c_bh), _tp(ds.m_bh), _tp(ds.n_bh),
+                      _tp(ds.o_bh), _tp(G))
+    _check(lib().cutc_get_last_error(), "cutc_int_bh")
+
+
+def int_long_range(ds, out, jj0, n2_eff, n2_tot):
+    lib().int_long_range(jj0, n2_eff, n2_tot, ds.n2, ds.n_ao, _tp(ds.wr2), _tp(ds.aos2), _tp(out))
+    _check(lib().cutc_get_last_error(), "int_long_range")
+
+
+def int_short_range_herm(ds, out):
+    lib().int_short_range_herm(ds.n1, ds.n_ao, _tp(ds.wr1), _tp(ds.aos1), _tp(out))
+    _check(lib().cutc_get_last_error(), "int_short_range_herm")
+
+
+def int_short_range_nonherm(ds, out):
+    lib().int_short_range_nonherm(ds.n1, ds.n_ao, _tp(ds.wr1), _tp(ds.aos1), _tp(out))
+    _check(lib().cutc_get_last_error(), "int_short_range_nonherm")
+
+
+def trans_inplace(M):
+    lib().trans_inplace(_tp(M), M.shape[0])
+    _check(lib().cutc_get_last_error(), "trans_inplace")

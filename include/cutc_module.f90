@@ -1,0 +1,49 @@
+! Fortran binding of libcutcint's cutc_int (bind(C), name "cutc_int").
+!
+! Array shapes follow the reference binding src/cutc_module.f90:23-35.  The
+! argument list follows the C definition src/cutc_int.c:15-20, which takes six
+! launch-geometry integers (grid x, y, z and block x, y, z); the reference
+! module declares only two (src/cutc_module.f90:10-18), which shifts every
+! later argument.  The function returns the C status (0, or a negative code).
+module cutc_module
+
+  use, intrinsic :: iso_c_binding
+
+  implicit none
+
+  interface
+
+    integer(c_int) function cutc_int(nxBlocks, nyBlocks, nzBlocks,          &
+                                     blockxSize, blockySize, blockzSize,    &
+                                     n_grid1, n_grid2, n_ao, n_nuc, size_bh, &
+                                     r1, wr1, r2, wr2, rn,                   &
+                                     aos_data1, aos_data2,                   &
+                                     c_bh, m_bh, n_bh, o_bh,                 &
+                                     int2_grad1_u12_ao, int_2e_ao) bind(C, name = "cutc_int")
+
+      import c_int, c_double
+      integer(c_int), intent(in), value :: nxBlocks, nyBlocks, nzBlocks
+      integer(c_int), intent(in), value :: blockxSize, blockySize, blockzSize
+      integer(c_int), intent(in), value :: n_grid1, n_grid2
+      integer(c_int), intent(in), value :: n_ao
+      integer(c_int), intent(in), value :: n_nuc
+      integer(c_int), intent(in), value :: size_bh
+      real(c_double), intent(in)        :: r1(3,n_grid1)
+      real(c_double), intent(in)        :: wr1(n_grid1)
+      real(c_double), intent(in)        :: r2(3,n_grid2)
+      real(c_double), intent(in)        :: wr2(n_grid2)
+      real(c_double), intent(in)        :: rn(3,n_nuc)
+      real(c_double), intent(in)        :: aos_data1(n_grid1,n_ao,4)
+      real(c_double), intent(in)        :: aos_data2(n_grid2,n_ao,4)
+      real(c_double), intent(in)        :: c_bh(size_bh,n_nuc)
+      integer(c_int), intent(in)        :: m_bh(size_bh,n_nuc)
+      integer(c_int), intent(in)        :: n_bh(size_bh,n_nuc)
+      integer(c_int), intent(in)        :: o_bh(size_bh,n_nuc)
+      real(c_double), intent(out)       :: int2_grad1_u12_ao(n_ao,n_ao,n_grid1,3)
+      real(c_double), intent(out)       :: int_2e_ao(n_ao,n_ao,n_ao,n_ao)
+
+    end function cutc_int
+
+  end interface
+
+end module cutc_module

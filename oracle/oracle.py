@@ -1,0 +1,182 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker.  The product library
+never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64 = ctypes.c_int64
+_L = None
+
+
+def build():
+    r = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout + r.stderr)
+
+
+def _find_openblas():
+    import numpy
+    d = os.path.join(os.path.dirname(os.path.dirname(numpy.__file__)), "numpy.libs")
+    c = sorted(glob.glob(os.path.join(d, "libscipy_openblas64_*.so")))
+    return c[0] if c else None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.oracle_use_blas.restype = ctypes.c_int
+        L.oracle_use_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.oracle_blas_loaded.restype = ctypes.c_int
+        L.oracle_threads.restype = ctypes.c_int
+        L.oracle_powd_int.restype = ctypes.c_double
+        L.oracle_powd_int.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.oracle_dgemm.restype = None
+        L.oracle_dgemm.argtypes = [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _dp, _i64,
+                                   _dp, _i64, ctypes.c_double, _dp, _i64]
+        L.oracle_jast_bh.restype = None
+        L.oracle_jast_bh.argtypes = [_i64, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _ip,
+                                     _ip, _ip, _dp]
+        L.oracle_long_range.restype = None
+        L.oracle_long_range.argtypes = [_i64, _i64, _i64, _i64, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_herm.restype = None
+        L.oracle_herm.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_nonherm.restype = None
+        L.oracle_nonherm.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_trans_inplace.restype = None
+        L.oracle_trans_inplace.argtypes = [_dp, _i64]
+        L.oracle_int2_grad1_u12_ao.restype = ctypes.c_int
+        L.oracle_int2_grad1_u12_ao.argtypes = [_i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                               _dp, _dp, _dp, _ip, _ip, _ip, _dp, _i64, _i64]
+        L.oracle_int_2e_ao.restype = ctypes.c_int
+        L.oracle_int_2e_ao.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp, _dp]
+        ob = _find_openblas()
+        if ob and os.environ.get("CUTC_ORACLE_BUILTIN_DGEMM", "0") != "1":
+            L.oracle_use_blas(ob.encode(), b"scipy_dgemm_64_")
+        _L = L
+    return _L
+
+
+def blas_name():
+    return "openblas" if lib().oracle_blas_loaded() else "builtin"
+
+
+def threads():
+    return lib().oracle_threads()
+
+
+def _p(a, t=_dp):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def powd_int(a, b):
+    return lib().oracle_powd_int(float(a), int(b))
+
+
+def dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    """Column-major BLAS dgemm on flat float64 arrays (C updated in place)."""
+    lib().oracle_dgemm(ta.encode(), tb.encode(), m, n, k, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+
+
+def jast_bh(sys, ii0=0, n1=None, jj0=0, n2=None):
+    """G with shape (4, n1, n2): grad_1 u (x, y, z) and -|grad_1 u|^2 / 2."""
+    n1 = sys.n1 - ii0 if n1 is None else n1
+    n2 = sys.n2 - jj0 if n2 is None else n2
+    G = np.zeros((4, n1, n2))
+    lib().oracle_jast_bh(ii0, n1, jj0, n2, sys.n_nuc, sys.size_bh, _p(_f64(sys.r1)), _p(_f64(sys.r2)),
+                         _p(_f64(sys.rn)), _p(_f64(sys.c_bh)), _p(_i32(sys.m_bh), _ip), _p(_i32(sys.n_bh), _ip),
+                         _p(_i32(sys.o_bh), _ip), _p(G))
+    return G
+
+
+def long_range(sys):
+    """A with numpy shape (n_ao, n_ao, n2): A[i, j, r2] = w2 chi_i chi_j
+    (Fortran A(r2, j + n_ao i))."""
+    A = np.zeros((sys.n_ao, sys.n_ao, sys.n2))
+    lib().oracle_long_range(0, sys.n2, sys.n2, sys.n2, sys.n_ao, _p(_f64(sys.wr2)), _p(_f64(sys.aos2)), _p(A))
+    return A
+
+
+def herm(sys):
+    """H with numpy shape (n_ao, n_ao, n1): H[i, j, r1] = w1 chi_i chi_j."""
+    H = np.zeros((sys.n_ao, sys.n_ao, sys.n1))
+    lib().oracle_herm(sys.n1, sys.n_ao, _p(_f64(sys.wr1)), _p(_f64(sys.aos1)), _p(H))
+    return H
+
+
+def nonherm(sys):
+    """NH with numpy shape (n_ao, n_ao, 3, n1):
+    NH[i, j, m, r1] = w1 (chi_j d_m chi_i - chi_i d_m chi_j)."""
+    NH = np.zeros((sys.n_ao, sys.n_ao, 3, sys.n1))
+    lib().oracle_nonherm(sys.n1, sys.n_ao, _p(_f64(sys.wr1)), _p(_f64(sys.aos1)), _p(NH))
+    return NH
+
+
+def trans_inplace(M):
+    """M + M^T of a square matrix (returns a new array)."""
+    M = np.array(M, dtype=np.float64, order="C", copy=True)
+    lib().oracle_trans_inplace(_p(M), M.shape[0])
+    return M
+
+
+def default_n2_chunk(sys, max_bytes=1 << 31):
+    """r2 chunk keeping the long-range block (n2c x n_ao^2 doubles) under max_bytes."""
+    return int(max(1, min(sys.n2, max_bytes // (8 * max(1, sys.n_ao) ** 2))))
+
+
+def int2_grad1_u12_ao(sys, n1c=0, n2c=None):
+    """X with numpy shape (4, n1, n_ao, n_ao): X[m, r1, l, k] is Fortran
+    X(k, l, r1, m).  n2c None: chunk r2 so that the long-range block stays
+    under 2 GiB."""
+    if n2c is None:
+        n2c = default_n2_chunk(sys)
+    X = np.zeros((4, sys.n1, sys.n_ao, sys.n_ao))
+    rc = lib().oracle_int2_grad1_u12_ao(
+        sys.n1, sys.n2, sys.n_ao, sys.n_nuc, sys.size_bh, _p(_f64(sys.r1)), _p(_f64(sys.r2)), _p(_f64(sys.wr2)),
+        _p(_f64(sys.rn)), _p(_f64(sys.aos2)), _p(_f64(sys.c_bh)), _p(_i32(sys.m_bh), _ip), _p(_i32(sys.n_bh), _ip),
+        _p(_i32(sys.o_bh), _ip), _p(X), n1c, n2c)
+    if rc != 0:
+        raise MemoryError(f"oracle_int2_grad1_u12_ao returned {rc}")
+    return X
+
+
+def int_2e_ao(sys, X):
+    """int_2e_ao with numpy shape (n_ao,)*4: out[b, a, l, k] is Fortran
+    int_2e_ao(k, l, a, b)."""
+    X = _f64(X)
+    out = np.zeros((sys.n_ao,) * 4)
+    rc = lib().oracle_int_2e_ao(sys.n1, sys.n_ao, _p(_f64(sys.wr1)), _p(_f64(sys.aos1)), _p(X), _p(out))
+    if rc != 0:
+        raise MemoryError(f"oracle_int_2e_ao returned {rc}")
+    return out
+
+
+def cutc_int(sys, n1c=0, n2c=None):
+    """(X[0:3], int_2e_ao) as cutc_int returns them."""
+    X = int2_grad1_u12_ao(sys, n1c, n2c)
+    return X[:3].copy(), int_2e_ao(sys, X)

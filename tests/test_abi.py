@@ -1,0 +1,115 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+public header declares (no compute calls without a GPU), and its memory plan
+behaves (pure host function)."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from cutc_amd import _lib, cutc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cutc_int.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*([A-Za-z_][A-Za-z_0-9]*)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_header_symbol():
+    path = _lib.lib_path("libcutcint.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = header_functions()
+    assert len(declared) >= 16
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+    assert set(cutc.HEADER_SYMBOLS) <= set(declared)
+
+
+def test_library_loads_and_binds():
+    L = cutc.lib()
+    for name in cutc.HEADER_SYMBOLS:
+        assert hasattr(L, name)
+    assert L.cutc_error_string(-1).decode().startswith("launch geometry")
+    o = cutc.options()
+    assert o.precision == 0 and o.slices == 4
+
+
+def test_reference_entry_points_keep_their_signatures():
+    """Argument lists of the reference symbols (src/cutc_int.c:15-20,
+    include/cutc_int.h:4-15) as declared in include/cutc_int.h."""
+    src = re.sub(r"\s+", " ", open(HEADER).read())
+    assert ("int cutc_int(int nxBlocks, int nyBlocks, int nzBlocks, int blockxSize, int blockySize, int blockzSize, "
+            "int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh, double *h_r1, double *h_wr1, double *h_r2, "
+            "double *h_wr2, double *h_rn, double *h_aos_data1, double *h_aos_data2, double *h_c_bh, int *h_m_bh, "
+            "int *h_n_bh, int *h_o_bh, double *h_int2_grad1_u12_ao, double *h_int_2e_ao);") in src
+    assert ("void get_int2_grad1_u12_ao(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int n_grid1, int n_grid2, int n_ao, "
+            "int n_nuc, int size_bh, double *r1, double *r2, double *wr2, double *rn, double *aos_data2, "
+            "double *c_bh, int *m_bh, int *n_bh, int *o_bh, double *int2_grad1_u12_ao);") in src
+    assert ("void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double *int2_grad1_u12_ao, "
+            "double *tc_int_2e_ao);") in src
+    assert ctypes.sizeof(cutc.Dim3) == 12
+
+
+GB = 1 << 30
+
+
+def test_plan_fits_budget_and_covers_r1():
+    # the 20-nucleus workload on one GPU: n_ao = 200, n1 = 60k, n2 = 200k
+    for budget in (64 * GB, 120 * GB, 200 * GB):
+        p = cutc.make_plan(60000, 200000, 200, 20, budget)
+        assert p.n_pairs == 20100
+        assert p.ldk % 64 == 0 and p.ldk >= 200000
+        assert 1 <= p.n1_chunk <= 60000
+        assert p.bytes_stage1 <= budget and p.bytes_stage2 <= budget
+        assert p.n1_chunk % 64 == 0 or p.n1_chunk == 60000
+        # 64-bit sizes: the Apk workspace alone is 32 GB
+        assert p.bytes_stage1 > 20100 * 200000 * 8
+
+
+def test_plan_small_and_edge_cases():
+    p = cutc.make_plan(4000, 16000, 24, 3, 8 * GB)
+    assert p.n1_chunk == 4000 and p.n1_chunk_2e == 4000
+    p = cutc.make_plan(0, 0, 0, 0, GB)
+    assert p.n1_chunk == 0
+    p = cutc.make_plan(1, 1, 1, 1, GB)
+    assert p.n1_chunk == 1 and p.n_pairs == 1
+    with pytest.raises(cutc.CutcError) as e:
+        cutc.make_plan(60000, 200000, 200, 20, 1 * GB)
+    assert e.value.code == cutc.CUTC_ERR_NOMEM
+    with pytest.raises(cutc.CutcError) as e:
+        cutc.make_plan(-1, 10, 10, 1, GB)
+    assert e.value.code == cutc.CUTC_ERR_ARG
+    # a requested chunk is honoured when it fits
+    p = cutc.make_plan(10000, 5000, 30, 4, 8 * GB, n1_chunk=1000)
+    assert p.n1_chunk <= 1000
+
+
+def test_plan_large_streamed_shape_is_64bit():
+    # 40 nuclei, n_ao = 300, n1 = 80k, n2 = 300k: workspaces beyond 2^31 elements
+    p = cutc.make_plan(80000, 300000, 300, 40, 200 * GB)
+    assert p.n_pairs == 45150
+    assert p.bytes_stage1 > 2 ** 31 * 8
+    assert p.bytes_stage1 <= 200 * GB
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/llvm/bin/amdflang"), reason="amdflang not present")
+def test_fortran_binding_compiles_and_links(tmp_path):
+    """include/cutc_module.f90 (the binding with the C definition's 6 launch
+    integers) compiles, and a Fortran caller links against libcutcint.so."""
+    fc = "/opt/rocm/llvm/bin/amdflang"
+    mod = os.path.join(ROOT, "include", "cutc_module.f90")
+    demo = os.path.join(ROOT, "tests", "fortran", "cutc_demo.f90")
+    libdir = os.path.dirname(_lib.lib_path("libcutcint.so"))
+    exe = tmp_path / "demo"
+    r = subprocess.run([fc, "-J", str(tmp_path), mod, demo, "-o", str(exe), f"-L{libdir}", "-lcutcint",
+                        f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert exe.exists()
