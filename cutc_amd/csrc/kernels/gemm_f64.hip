@@ -159,8 +159,8 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
         __syncthreads();
     }
 
-    // Epilogue.  v_mfma_f64_16x16x4f64 result layout: lane l, element r holds
-    // row 4 (l / 16) + r, column l % 16 of the 16 x 16 block.
+    // Epilogue.  v_mfma_f64_16x16x4f64 result layout (measured on gfx950):
+    // lane l, element r holds row 4 r + l / 16, column l % 16 of the block.
     const int64_t n2 = (int64_t)g.n_ao * g.n_ao;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
                 continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t row = row0 + wm + 16 * i + 4 * lg + r;
+                const int64_t row = row0 + wm + 16 * i + 4 * r + lg;
                 if (row >= g.M)
                     continue;
                 const double v = g.alpha * acc[i][j][r];

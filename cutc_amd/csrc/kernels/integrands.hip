@@ -15,6 +15,10 @@
 #include "common.h"
 #include "launch.h"
 
+// Products are rounded one by one (no FMA contraction), as in the reference's
+// C expressions, so NH is exactly antisymmetric with an exactly zero diagonal.
+#pragma clang fp contract(off)
+
 __global__ void pair_table_kernel(int n_ao, int *pairs)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
