@@ -165,21 +165,43 @@ static int check_exponents(const int *m, const int *n, const int *o, int64_t cnt
     return CUTC_OK;
 }
 
-static int check_exponents_device(const int *dm, const int *dn, const int *dno, int64_t cnt)
+/* Copy the (small) Boys-Handy tables to the host, check the exponents and
+ * expand them into the Jastrow monomial program. */
+static int program_from_device(int n_nuc, int size_bh, const double *dc, const int *dm, const int *dn,
+                               const int *dno, cutc_jprog *prog)
 {
-    if (cnt <= 0)
-        return CUTC_OK;
-    int *h = malloc(sizeof(int) * 3 * cnt);
-    if (!h)
-        return CUTC_ERR_NOMEM;
-    int rc = CUTC_OK;
-    if (hipMemcpy(h, dm, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(h + cnt, dn, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(h + 2 * cnt, dno, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess)
+    const int64_t cnt = (int64_t)n_nuc * size_bh;
+    memset(prog, 0, sizeof *prog);
+    double *c = malloc(sizeof(double) * (cnt + 1));
+    int *h = malloc(sizeof(int) * (3 * cnt + 1));
+    int rc = (c && h) ? CUTC_OK : CUTC_ERR_NOMEM;
+    if (rc == CUTC_OK && cnt > 0 &&
+        (hipMemcpy(c, dc, sizeof(double) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(h, dm, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(h + cnt, dn, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(h + 2 * cnt, dno, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess))
         rc = CUTC_ERR_HIP;
     if (rc == CUTC_OK)
         rc = check_exponents(h, h + cnt, h + 2 * cnt, cnt);
+    if (rc == CUTC_OK)
+        rc = cutc_jast_program(n_nuc, size_bh, c, h, h + cnt, h + 2 * cnt, prog);
+    free(c);
     free(h);
+    return rc;
+}
+
+/* Device copies of a program's monomials and classes. */
+static int program_upload(const cutc_jprog *prog, cutc_jmono **dmono, int **dcls)
+{
+    *dmono = NULL;
+    *dcls = NULL;
+    int rc = hip_rc(hipMalloc((void **)dmono, sizeof(cutc_jmono) * (prog->n_mono + 1)));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipMalloc((void **)dcls, sizeof(int) * (2 * prog->n_cls + 2)));
+    if (rc == CUTC_OK && prog->n_mono > 0)
+        rc = hip_rc(hipMemcpy(*dmono, prog->mono, sizeof(cutc_jmono) * prog->n_mono, hipMemcpyHostToDevice));
+    if (rc == CUTC_OK && prog->n_cls > 0)
+        rc = hip_rc(hipMemcpy(*dcls, prog->cls, sizeof(int) * 2 * prog->n_cls, hipMemcpyHostToDevice));
     return rc;
 }
 
@@ -204,15 +226,15 @@ static void dfree(void *p)
 /* ------------------------------------------------------------ stage 1 */
 
 static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
-                  int size_bh, const double *r1, const double *r2, const double *wr2, const double *rn,
-                  const double *aos2, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh, double *X,
-                  tracer_t *tr, cutc_stats *stats)
+                  const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
+                  const cutc_jprog *prog, double *X, tracer_t *tr, cutc_stats *stats)
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
     const int64_t nao2 = (int64_t)n_ao * n_ao;
-    int *pairs = NULL;
-    double *Apk = NULL, *G = NULL, *f2tab = NULL;
+    int *pairs = NULL, *dcls = NULL;
+    cutc_jmono *dmono = NULL;
+    double *Apk = NULL, *G = NULL, *f2tab = NULL, *tab = NULL;
     void *work = NULL;
     size_t work_bytes = 0;
     int rc = CUTC_OK, ev;
@@ -230,6 +252,8 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     TRYH(hipMalloc((void **)&G, sizeof(double) * 4 * n1c * ldk));
     if (n_nuc > 32)
         TRYH(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * ldk));
+    TRYH(hipMalloc((void **)&tab, sizeof(double) * (prog->ts * n1c + 1)));
+    TRY(program_upload(prog, &dmono, &dcls));
     if (opt->precision != CUTC_PREC_FP64) {
         work_bytes = cutc_split_work_bytes(opt->precision, opt->slices, P, n1c, ldk);
         TRYH(hipMalloc(&work, work_bytes));
@@ -249,8 +273,10 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         const int64_t n1e = (n1 - i0 < n1c) ? n1 - i0 : n1c;
         nchunks++;
         ev = tr_begin(tr, CAT_JAST);
-        rc = cutc_launch_jastrow(r1, r2, rn, c_bh, m_bh, n_bh, o_bh, n_nuc, size_bh, i0, n1e, 0, n2, ldk, G, ldk,
-                                 n1e * ldk, f2tab, st);
+        rc = cutc_launch_jast_prep(r1, i0, n1e, rn, n_nuc, dmono, prog->n_mono, tab, prog->ts, st);
+        if (rc == CUTC_OK)
+            rc = cutc_launch_jast_pairs(r1, r2, rn, n_nuc, tab, prog->ts, dcls, prog->n_cls, prog->kmax, i0, n1e, 0,
+                                        n2, ldk, G, ldk, n1e * ldk, f2tab, st);
         tr_end(tr, ev);
         TRY(rc);
         if (opt->precision == CUTC_PREC_FP64) {
@@ -284,6 +310,9 @@ done:
     dfree(Apk);
     dfree(G);
     dfree(f2tab);
+    dfree(tab);
+    dfree(dmono);
+    dfree(dcls);
     dfree(work);
     return rc;
 }
@@ -414,23 +443,26 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         return CUTC_ERR_ARG;
     if ((stages & 2) && n1 > 0 && n_ao > 0 && (!wr1 || !aos1))
         return CUTC_ERR_ARG;
+    cutc_jprog prog;
+    memset(&prog, 0, sizeof prog);
     if ((stages & 1) && n1 > 0 && n2 > 0 && n_ao > 0) {
-        rc = check_exponents_device(m_bh, n_bh, o_bh, (int64_t)n_nuc * size_bh);
+        rc = program_from_device(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, &prog);
         if (rc)
             return rc;
     }
     cutc_plan pl;
     rc = cutc_make_plan(n1, n2, n_ao, n_nuc, opt->precision, opt->slices, device_budget(opt), opt->n1_chunk, &pl);
-    if (rc)
+    if (rc) {
+        cutc_jast_program_free(&prog);
         return rc;
+    }
     if (stats)
         memset(stats, 0, sizeof *stats);
     tracer_t tr;
     tr_init(&tr, st, stats != NULL || opt->verbose);
     int evt = tr_begin(&tr, CAT_TOTAL);
     if (stages & 1) {
-        rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, size_bh, r1, r2, wr2, rn, aos2, c_bh, m_bh, n_bh, o_bh, X, &tr,
-                    stats);
+        rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
     }
     if (rc == CUTC_OK && (stages & 2))
         rc = stage2(opt, &pl, n1, n_ao, wr1, aos1, X, out, &tr, stats);
@@ -457,6 +489,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         }
     }
     tr_free(&tr);
+    cutc_jast_program_free(&prog);
     return rc;
 }
 
@@ -676,16 +709,35 @@ void cutc_int_bh(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int ii0, int n_grid1_eff
 {
     (void)dimGrid;
     (void)dimBlock;
-    double *f2tab = NULL;
+    double *f2tab = NULL, *tab = NULL;
+    cutc_jmono *dmono = NULL;
+    int *dcls = NULL;
+    cutc_jprog prog;
     int rc = CUTC_OK;
-    if (n_nuc > 32 && n_grid2_eff > 0)
+    if (n_grid1_eff < 0 || n_grid2_eff < 0 || n_nuc < 0 || size_bh < 0)
+        rc = CUTC_ERR_ARG;
+    if (rc == CUTC_OK)
+        rc = program_from_device(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, &prog);
+    else
+        memset(&prog, 0, sizeof prog);
+    if (rc == CUTC_OK && n_nuc > 32 && n_grid2_eff > 0)
         rc = hip_rc(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * (size_t)n_grid2_eff));
     if (rc == CUTC_OK)
-        rc = cutc_launch_jastrow(r1, r2, rn, c_bh, m_bh, n_bh, o_bh, n_nuc, size_bh, ii0, n_grid1_eff, jj0,
-                                 n_grid2_eff, n_grid2_eff, grad1_u12, n_grid2_tot,
-                                 (int64_t)n_grid1_tot * n_grid2_tot, f2tab, NULL);
+        rc = hip_rc(hipMalloc((void **)&tab, sizeof(double) * (prog.ts * (int64_t)n_grid1_eff + 1)));
+    if (rc == CUTC_OK)
+        rc = program_upload(&prog, &dmono, &dcls);
+    if (rc == CUTC_OK)
+        rc = cutc_launch_jast_prep(r1, ii0, n_grid1_eff, rn, n_nuc, dmono, prog.n_mono, tab, prog.ts, NULL);
+    if (rc == CUTC_OK)
+        rc = cutc_launch_jast_pairs(r1, r2, rn, n_nuc, tab, prog.ts, dcls, prog.n_cls, prog.kmax, ii0, n_grid1_eff,
+                                    jj0, n_grid2_eff, n_grid2_eff, grad1_u12, n_grid2_tot,
+                                    (int64_t)n_grid1_tot * n_grid2_tot, f2tab, NULL);
     wrapper_status(rc, "cutc_int_bh");
     dfree(f2tab);
+    dfree(tab);
+    dfree(dmono);
+    dfree(dcls);
+    cutc_jast_program_free(&prog);
 }
 
 void int_long_range(int jj0, int n_grid2_eff, int n_grid2_tot, int n_grid2, int n_ao, double *wr2, double *aos_data2,

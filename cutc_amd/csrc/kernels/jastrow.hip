@@ -1,156 +1,72 @@
-// Boys-Handy Jastrow pair kernel for gfx950: G(r1, r2) = grad_1 u and
+// Boys-Handy Jastrow pair kernels for gfx950: G(r1, r2) = grad_1 u and
 // -1/2 |grad_1 u|^2 over an r1 chunk x all r2 points (FP64 throughout).
 //
-// Reference: src/jast_bh.cu:31-264 (kernel) and :268-303 (wrapper cutc_int_bh).
-// Same formula, thresholds (1e-15 on squared distances, |c| < 1e-10 skip) and
-// term algebra; the work is laid out for CDNA4 instead:
-//   - lanes run along r2, so every store is a coalesced 512-byte row segment
-//     (the reference puts r1 on threadIdx.x and stores with stride n2);
-//   - a block owns 256 r2 points x R1B r1 points.  The e-n factors of its r1
-//     points (f1A and grad f1A) are computed once into LDS, and each lane keeps
-//     f2A of its own r2 point in registers across the R1B rows, so a pair costs
-//     one sqrt/div for r12 plus the term algebra;
-//   - accumulation is in registers, one store per output element (the
-//     reference read-modify-writes global memory once per term).
+// Reference: src/jast_bh.cu:31-264 (kernel) and :268-303 (wrapper).  Same
+// formula, thresholds (1e-15 on squared distances, |c| < 1e-10 skip) and
+// special cases; the work is reorganised for CDNA4:
+//   - the term table is expanded once on the host into monomials
+//     coef f1^i f2^j g^k (jastrow_prog.c).  Everything that depends on r1
+//     alone (f1A, grad f1A and their powers, the nucleus sums of the r1-only
+//     monomials) is computed once per r1 point by jast_prep_kernel into a
+//     table row, so a pair costs one sqrt/div for r12, the powers of g, and
+//     for each (class, nucleus) one f2^j times a 4-wide FMA;
+//   - the table row is uniform across a block and is read with scalar loads;
+//   - lanes run along r2: every store is a coalesced row segment, and each
+//     lane keeps f2A of its r2 point in registers across R1B rows;
+//   - accumulation in registers, one store per output element.
 #include "common.h"
 #include "launch.h"
 
 #define JAST_THREADS 256
 
-struct JastArgs {
-    const double *r1;   // (3, *) xyz rows, first row used = ii0
-    const double *r2;   // (3, *) xyz rows, first row used = jj0
-    const double *rn;   // (3, n_nuc)
-    const double *c_bh; // (size_bh, n_nuc)
-    const int *m_bh, *n_bh, *o_bh;
-    const double *f2tab; // (n_nuc, ld_f2) e2-n factors, generic path only
-    double *G;           // G[m * plane + i1 * ldg + i2]
-    int64_t ii0, jj0;
-    int64_t n1, n2;      // valid r1 rows, valid r2 columns
-    int64_t n2_fill;     // columns written (>= n2; the extra ones get 0)
-    int64_t ldg, plane;
-    int64_t ld_f2;
-    int n_nuc, size_bh;
-};
-
-// Sum over nuclei and terms for one pair, given the e-n factors.
-__device__ static inline void jast_terms(const JastArgs &a, int A, double f1, double f1x, double f1y, double f1z,
-                                         double f2, double g, double gx, double gy, double gz, double &ux,
-                                         double &uy, double &uz)
+// Row i1 of the per-r1 table: zero it, then add every monomial.
+__global__ __launch_bounds__(256) void jast_prep_kernel(const double *__restrict__ r1, int64_t ii0, int64_t n1,
+                                                        const double *__restrict__ rn, int n_nuc,
+                                                        const cutc_jmono *__restrict__ mono, int n_mono,
+                                                        double *__restrict__ tab, int64_t ts)
 {
-    const int base = a.size_bh * A;
-    for (int p = 0; p < a.size_bh; p++) {
-        const int kk = base + p;
-        const double c = a.c_bh[kk];
-        if (fabs(c) < 1e-10)
-            continue;
-        const int m = a.m_bh[kk], n = a.n_bh[kk], o = a.o_bh[kk];
-        double f1_mm1 = 0.0, f1_m = 1.0, f2_m = 1.0;
-        double f1_nm1 = 0.0, f1_n = 1.0, f2_n = 1.0;
-        if (m > 0) {
-            f1_mm1 = cutc_powi(f1, m - 1);
-            f1_m = f1_mm1 * f1;
-            f2_m = cutc_powi(f2, m);
+    const int64_t i1 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i1 >= n1)
+        return;
+    double *row = tab + i1 * ts;
+    for (int64_t t = 0; t < ts; t++)
+        row[t] = 0.0;
+    const double *p = r1 + 3 * (ii0 + i1);
+    const double x = p[0], y = p[1], z = p[2];
+    int Aprev = -1;
+    double f = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int t = 0; t < n_mono; t++) {
+        const cutc_jmono mo = mono[t];
+        if (mo.A != Aprev) {
+            const double *R = rn + 3 * mo.A;
+            cutc_rbar_grad(x - R[0], y - R[1], z - R[2], f, gx, gy, gz);
+            Aprev = mo.A;
         }
-        if (n > 0) {
-            f1_nm1 = cutc_powi(f1, n - 1);
-            f1_n = f1_nm1 * f1;
-            f2_n = cutc_powi(f2, n);
-        }
-        double t1 = 0.0, t2 = 0.0;
-        if (m > 0)
-            t1 += (double)m * f1_mm1 * f2_n;
-        if (n > 0)
-            t1 += (double)n * f1_nm1 * f2_m;
-        if (o > 0) {
-            const double g_om1 = cutc_powi(g, o - 1);
-            const double g_o = g_om1 * g;
-            t2 = c * (double)o * g_om1 * (f1_m * f2_n + f1_n * f2_m);
-            t1 = c * t1 * g_o;
+        double fi = 1.0;
+        for (int e = 0; e < mo.i; e++)
+            fi *= f;
+        const double v = mo.coef * fi;
+        if (mo.is_r) {
+            row[mo.slot + 3] += v;
         } else {
-            t1 *= c;
+            row[mo.slot] += v * gx;
+            row[mo.slot + 1] += v * gy;
+            row[mo.slot + 2] += v * gz;
         }
-        ux += t1 * f1x + t2 * gx;
-        uy += t1 * f1y + t2 * gy;
-        uz += t1 * f1z + t2 * gz;
     }
 }
 
-// NMAX > 0: f2A held in registers (n_nuc <= NMAX).  NMAX == 0: f2A read from
-// the precomputed table a.f2tab.
-template <int NMAX, int R1B>
-__global__ __launch_bounds__(JAST_THREADS) void jast_pair_kernel(JastArgs a)
+extern "C" int cutc_launch_jast_prep(const double *r1, int64_t ii0, int64_t n1, const double *rn, int n_nuc,
+                                     const cutc_jmono *mono, int n_mono, double *tab, int64_t ts, hipStream_t st)
 {
-    extern __shared__ double s_r1[]; // [R1B][n_nuc][4]  (f1, d/dx, d/dy, d/dz)
-    const int64_t i2 = (int64_t)blockIdx.x * JAST_THREADS + threadIdx.x;
-    const bool live = i2 < a.n2;
-    const bool fill = i2 < a.n2_fill;
-
-    double p2x = 0.0, p2y = 0.0, p2z = 0.0;
-    if (live) {
-        const double *p = a.r2 + 3 * (a.jj0 + i2);
-        p2x = p[0];
-        p2y = p[1];
-        p2z = p[2];
-    }
-    double f2r[NMAX > 0 ? NMAX : 1];
-    if constexpr (NMAX > 0) {
-#pragma unroll
-        for (int A = 0; A < NMAX; A++) {
-            f2r[A] = 0.0;
-            if (A < a.n_nuc)
-                f2r[A] = cutc_rbar(p2x - a.rn[3 * A], p2y - a.rn[3 * A + 1], p2z - a.rn[3 * A + 2]);
-        }
-    }
-
-    for (int64_t i1_0 = (int64_t)blockIdx.y * R1B; i1_0 < a.n1; i1_0 += (int64_t)gridDim.y * R1B) {
-        const int nr = (a.n1 - i1_0 < R1B) ? (int)(a.n1 - i1_0) : R1B;
-        __syncthreads();
-        for (int t = threadIdx.x; t < nr * a.n_nuc; t += JAST_THREADS) {
-            const int r = t / a.n_nuc, A = t - r * a.n_nuc;
-            const double *p = a.r1 + 3 * (a.ii0 + i1_0 + r);
-            double f, gx, gy, gz;
-            cutc_rbar_grad(p[0] - a.rn[3 * A], p[1] - a.rn[3 * A + 1], p[2] - a.rn[3 * A + 2], f, gx, gy, gz);
-            double *d = s_r1 + 4 * t;
-            d[0] = f;
-            d[1] = gx;
-            d[2] = gy;
-            d[3] = gz;
-        }
-        __syncthreads();
-        if (!fill)
-            continue;
-        for (int r = 0; r < nr; r++) {
-            const int64_t i1 = i1_0 + r;
-            double ux = 0.0, uy = 0.0, uz = 0.0;
-            if (live) {
-                const double *p = a.r1 + 3 * (a.ii0 + i1);
-                double g, gx, gy, gz;
-                cutc_rbar_grad(p[0] - p2x, p[1] - p2y, p[2] - p2z, g, gx, gy, gz);
-                const double *s = s_r1 + 4 * (int64_t)r * a.n_nuc;
-                if constexpr (NMAX > 0) {
-#pragma unroll
-                    for (int A = 0; A < NMAX; A++) {
-                        if (A < a.n_nuc)
-                            jast_terms(a, A, s[4 * A], s[4 * A + 1], s[4 * A + 2], s[4 * A + 3], f2r[A], g, gx, gy,
-                                       gz, ux, uy, uz);
-                    }
-                } else {
-                    for (int A = 0; A < a.n_nuc; A++)
-                        jast_terms(a, A, s[4 * A], s[4 * A + 1], s[4 * A + 2], s[4 * A + 3],
-                                   a.f2tab[(int64_t)A * a.ld_f2 + i2], g, gx, gy, gz, ux, uy, uz);
-                }
-            }
-            double *o = a.G + i1 * a.ldg + i2;
-            o[0] = ux;
-            o[a.plane] = uy;
-            o[2 * a.plane] = uz;
-            o[3 * a.plane] = -0.5 * (ux * ux + uy * uy + uz * uz);
-        }
-    }
+    if (n1 <= 0)
+        return 0;
+    hipLaunchKernelGGL(jast_prep_kernel, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, r1, ii0, n1, rn,
+                       n_nuc, mono, n_mono, tab, ts);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// e2-n factors f2A(r2) for the generic (many-nuclei) path: f2tab[A * ld + i2].
+// e2-n factors f2A(r2) for the many-nuclei path: f2tab[A * ld + i2].
 __global__ void jast_f2_kernel(const double *r2, int64_t jj0, int64_t n2, const double *rn, int n_nuc, double *f2tab,
                                int64_t ld)
 {
@@ -162,68 +78,142 @@ __global__ void jast_f2_kernel(const double *r2, int64_t jj0, int64_t n2, const 
         f2tab[(int64_t)A * ld + i2] = cutc_rbar(p[0] - rn[3 * A], p[1] - rn[3 * A + 1], p[2] - rn[3 * A + 2]);
 }
 
-template <int NMAX, int R1B>
-static int launch_pairs(const JastArgs &a, hipStream_t st)
+__device__ static inline double ipow_u(double x, int e)
 {
-    const size_t lds = sizeof(double) * 4 * (size_t)R1B * a.n_nuc;
-    if (lds > 64 * 1024)
-        return CUTC_ERR_ARG;
-    int64_t gx = (a.n2_fill + JAST_THREADS - 1) / JAST_THREADS;
-    int64_t gy = (a.n1 + R1B - 1) / R1B;
-    if (gx == 0 || gy == 0)
-        return 0;
+    double r = 1.0;
+    for (int t = 0; t < e; t++)
+        r *= x;
+    return r;
+}
+
+// NMAX > 0: f2A of the lane's r2 point held in registers (n_nuc <= NMAX).
+// NMAX == 0: f2A read from f2tab.
+template <int NMAX, int R1B>
+__global__ __launch_bounds__(JAST_THREADS) void jast_pair_kernel(
+    const double *__restrict__ r1, const double *__restrict__ r2, const double *__restrict__ rn, int n_nuc,
+    const double *__restrict__ tab, int64_t ts, const int *__restrict__ cls, int n_cls, int kmax,
+    const double *__restrict__ f2tab, int64_t ld_f2, double *__restrict__ G, int64_t ii0, int64_t n1, int64_t jj0,
+    int64_t n2, int64_t n2_fill, int64_t ldg, int64_t plane)
+{
+    const int64_t i2 = (int64_t)blockIdx.x * JAST_THREADS + threadIdx.x;
+    if (i2 >= n2_fill)
+        return;
+    const bool live = i2 < n2;
+    double p2x = 0.0, p2y = 0.0, p2z = 0.0;
+    if (live) {
+        const double *p = r2 + 3 * (jj0 + i2);
+        p2x = p[0];
+        p2y = p[1];
+        p2z = p[2];
+    }
+    double f2r[NMAX > 0 ? NMAX : 1];
+    if constexpr (NMAX > 0) {
+#pragma unroll
+        for (int A = 0; A < NMAX; A++) {
+            f2r[A] = 0.0;
+            if (A < n_nuc)
+                f2r[A] = cutc_rbar(p2x - rn[3 * A], p2y - rn[3 * A + 1], p2z - rn[3 * A + 2]);
+        }
+    }
+    const int base = 4 * (kmax + 1);
+    for (int64_t i1_0 = (int64_t)blockIdx.y * R1B; i1_0 < n1; i1_0 += (int64_t)gridDim.y * R1B) {
+        const int nr = (n1 - i1_0 < R1B) ? (int)(n1 - i1_0) : R1B;
+        for (int r = 0; r < nr; r++) {
+            const int64_t i1 = i1_0 + r;
+            double ux = 0.0, uy = 0.0, uz = 0.0;
+            if (live) {
+                const double *p = r1 + 3 * (ii0 + i1);
+                double g, gx, gy, gz;
+                cutc_rbar_grad(p[0] - p2x, p[1] - p2y, p[2] - p2z, g, gx, gy, gz);
+                const double *row = tab + i1 * ts;
+                double s = 0.0, gp = 1.0;
+                for (int k = 0; k <= kmax; k++) {
+                    ux += gp * row[4 * k];
+                    uy += gp * row[4 * k + 1];
+                    uz += gp * row[4 * k + 2];
+                    s += gp * row[4 * k + 3];
+                    gp *= g;
+                }
+                for (int c = 0; c < n_cls; c++) {
+                    const int j = cls[2 * c], k = cls[2 * c + 1];
+                    const double gk = ipow_u(g, k);
+                    const double *q = row + base + 4 * (int64_t)c * n_nuc;
+                    if constexpr (NMAX > 0) {
+#pragma unroll
+                        for (int A = 0; A < NMAX; A++) {
+                            if (A < n_nuc) {
+                                const double w = ipow_u(f2r[A], j) * gk;
+                                ux += w * q[4 * A];
+                                uy += w * q[4 * A + 1];
+                                uz += w * q[4 * A + 2];
+                                s += w * q[4 * A + 3];
+                            }
+                        }
+                    } else {
+                        for (int A = 0; A < n_nuc; A++) {
+                            const double w = ipow_u(f2tab[(int64_t)A * ld_f2 + i2], j) * gk;
+                            ux += w * q[4 * A];
+                            uy += w * q[4 * A + 1];
+                            uz += w * q[4 * A + 2];
+                            s += w * q[4 * A + 3];
+                        }
+                    }
+                }
+                ux += s * gx;
+                uy += s * gy;
+                uz += s * gz;
+            }
+            double *o = G + i1 * ldg + i2;
+            o[0] = ux;
+            o[plane] = uy;
+            o[2 * plane] = uz;
+            o[3 * plane] = -0.5 * (ux * ux + uy * uy + uz * uz);
+        }
+    }
+}
+
+template <int NMAX, int R1B>
+static int launch_pairs(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab,
+                        int64_t ts, const int *cls, int n_cls, int kmax, const double *f2tab, int64_t ld_f2,
+                        double *G, int64_t ii0, int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, int64_t ldg,
+                        int64_t plane, hipStream_t st)
+{
+    int64_t gx = (n2_fill + JAST_THREADS - 1) / JAST_THREADS;
+    int64_t gy = (n1 + R1B - 1) / R1B;
     if (gx > 2147483647)
         return CUTC_ERR_ARG;
     if (gy > 65535)
         gy = 65535;
-    hipLaunchKernelGGL((jast_pair_kernel<NMAX, R1B>), dim3((unsigned)gx, (unsigned)gy), dim3(JAST_THREADS), lds, st,
-                       a);
+    hipLaunchKernelGGL((jast_pair_kernel<NMAX, R1B>), dim3((unsigned)gx, (unsigned)gy), dim3(JAST_THREADS), 0, st,
+                       r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, f2tab, ld_f2, G, ii0, n1, jj0, n2, n2_fill, ldg,
+                       plane);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// Compute G for r1 rows [ii0, ii0 + n1) x r2 rows [jj0, jj0 + n2), written as
-// G[m * plane + i1 * ldg + i2]; columns [n2, n2_fill) are set to 0.
-// f2tab (n_nuc x n2_fill doubles) is scratch for n_nuc > 32 and may be NULL
-// otherwise.
-extern "C" int cutc_launch_jastrow(const double *r1, const double *r2, const double *rn, const double *c_bh,
-                                   const int *m_bh, const int *n_bh, const int *o_bh, int n_nuc, int size_bh,
-                                   int64_t ii0, int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, double *G,
-                                   int64_t ldg, int64_t plane, double *f2tab, hipStream_t st)
+extern "C" int cutc_launch_jast_pairs(const double *r1, const double *r2, const double *rn, int n_nuc,
+                                      const double *tab, int64_t ts, const int *cls, int n_cls, int kmax, int64_t ii0,
+                                      int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, double *G, int64_t ldg,
+                                      int64_t plane, double *f2tab, hipStream_t st)
 {
-    JastArgs a;
-    a.r1 = r1;
-    a.r2 = r2;
-    a.rn = rn;
-    a.c_bh = c_bh;
-    a.m_bh = m_bh;
-    a.n_bh = n_bh;
-    a.o_bh = o_bh;
-    a.f2tab = f2tab;
-    a.G = G;
-    a.ii0 = ii0;
-    a.jj0 = jj0;
-    a.n1 = n1;
-    a.n2 = n2;
-    a.n2_fill = n2_fill;
-    a.ldg = ldg;
-    a.plane = plane;
-    a.ld_f2 = n2_fill;
-    a.n_nuc = n_nuc;
-    a.size_bh = size_bh;
     if (n1 <= 0 || n2_fill <= 0)
         return 0;
+#define LP(NM, RB)                                                                                                   \
+    launch_pairs<NM, RB>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, f2tab, n2_fill, G, ii0, n1, jj0, n2, n2_fill,  \
+                         ldg, plane, st)
     if (n_nuc <= 8)
-        return launch_pairs<8, 16>(a, st);
+        return LP(8, 32);
     if (n_nuc <= 16)
-        return launch_pairs<16, 16>(a, st);
+        return LP(16, 32);
     if (n_nuc <= 32)
-        return launch_pairs<32, 16>(a, st);
+        return LP(32, 32);
+#undef LP
     if (!f2tab)
         return CUTC_ERR_ARG;
     if (n2 > 0) {
         int64_t nb = (n2 + 255) / 256;
         hipLaunchKernelGGL(jast_f2_kernel, dim3((unsigned)nb), dim3(256), 0, st, r2, jj0, n2, rn, n_nuc, f2tab,
-                           a.ld_f2);
+                           n2_fill);
     }
-    return launch_pairs<0, 4>(a, st);
+    return launch_pairs<0, 32>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, f2tab, n2_fill, G, ii0, n1, jj0, n2,
+                               n2_fill, ldg, plane, st);
 }

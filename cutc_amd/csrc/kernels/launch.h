@@ -12,11 +12,37 @@
 extern "C" {
 #endif
 
-/* Jastrow pair block (jastrow.hip). */
-int cutc_launch_jastrow(const double *r1, const double *r2, const double *rn, const double *c_bh, const int *m_bh,
-                        const int *n_bh, const int *o_bh, int n_nuc, int size_bh, int64_t ii0, int64_t n1,
-                        int64_t jj0, int64_t n2, int64_t n2_fill, double *G, int64_t ldg, int64_t plane,
-                        double *f2tab, hipStream_t st);
+/* Jastrow monomial program (jastrow_prog.c, host) and kernels (jastrow.hip).
+ * A monomial coef f1^i (x grad f1 unless is_r) is added to table slot `slot`
+ * of its r1 row (slot .. slot+2 for grad-f1 terms, slot+3 for is_r terms). */
+typedef struct cutc_jmono {
+    int A, slot, i, is_r;
+    double coef;
+} cutc_jmono;
+
+typedef struct cutc_jprog {
+    cutc_jmono *mono; /* sorted by nucleus */
+    int *cls;         /* (j, k) per class */
+    int n_mono, n_cls, kmax;
+    int64_t ts;       /* doubles per r1 row of the table */
+} cutc_jprog;
+
+int cutc_jast_program(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh,
+                      cutc_jprog *prog);
+void cutc_jast_program_free(cutc_jprog *prog);
+
+/* Per-r1 table rows for r1 points [ii0, ii0 + n1) into tab (n1 x ts). */
+int cutc_launch_jast_prep(const double *r1, int64_t ii0, int64_t n1, const double *rn, int n_nuc,
+                          const cutc_jmono *mono, int n_mono, double *tab, int64_t ts, hipStream_t st);
+/* G for r1 rows [ii0, ii0 + n1) x r2 rows [jj0, jj0 + n2), written as
+ * G[m * plane + i1 * ldg + i2]; columns [n2, n2_fill) get 0.  tab from
+ * cutc_launch_jast_prep for the same r1 rows; cls: device copy of the
+ * program's classes.  f2tab (n_nuc x n2_fill) is scratch used when
+ * n_nuc > 32, NULL otherwise. */
+int cutc_launch_jast_pairs(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab,
+                           int64_t ts, const int *cls, int n_cls, int kmax, int64_t ii0, int64_t n1, int64_t jj0,
+                           int64_t n2, int64_t n2_fill, double *G, int64_t ldg, int64_t plane, double *f2tab,
+                           hipStream_t st);
 
 /* Integrands (integrands.hip).  pairs: (k, l) per packed index, k <= l. */
 int cutc_launch_pair_table(int n_ao, int *pairs, hipStream_t st);
