@@ -1,0 +1,87 @@
+/*
+ * Constants of the modular (CRT) split-precision contraction (host code).
+ *
+ * Scheme (Ozaki, Uchino & Imamura, "Ozaki Scheme II: a GEMM-oriented
+ * emulation of floating-point matrix multiplication using an integer modular
+ * technique"): each operand row is scaled by a power of two and rounded to
+ * an integer of at most `bits` bits; the exact integer product
+ * V = sum_k a_k b_k is obtained from its residues modulo n pairwise-coprime
+ * moduli m_i <= 256, each residue product being an int8 x int8 -> int32 GEMM;
+ * V is rebuilt by the Chinese remainder theorem and scaled back to FP64.
+ * Correct reconstruction needs |V| < M/2 with M = prod m_i; `bits` is chosen
+ * so that K 2^(2 bits) <= M/4.
+ *
+ * CRT: with M_i = M / m_i and y_i = M_i^-1 mod m_i,
+ *   V == S = sum_i u_i M_i  (mod M),  u_i = (c_i y_i) mod m_i in [0, m_i),
+ * and V = S - M round(S / M).  S < n M < 2^128 is carried as three
+ * 40-bit integer parts (hi 2^80 + mid 2^40 + lo) so that every partial sum is
+ * exact in FP64.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "cutc_int.h"
+#include "kernels/launch.h"
+
+static const int k_moduli[CUTC_CRT_MAX] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                           229, 227, 223, 217, 211, 199, 197, 193};
+
+typedef unsigned __int128 u128;
+
+static int inv_mod(int a, int m) /* a^-1 mod m, gcd(a, m) = 1 */
+{
+    int t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+    while (nr) {
+        int q = r / nr, tmp;
+        tmp = t - q * nt;
+        t = nt;
+        nt = tmp;
+        tmp = r - q * nr;
+        r = nr;
+        nr = tmp;
+    }
+    return r == 1 ? (t < 0 ? t + m : t) : -1;
+}
+
+static void split3(u128 v, double out[3])
+{
+    const u128 mask = ((u128)1 << 40) - 1;
+    out[2] = (double)(uint64_t)(v & mask);
+    out[1] = (double)(uint64_t)((v >> 40) & mask);
+    out[0] = (double)(uint64_t)(v >> 80);
+}
+
+int cutc_crt_setup(int64_t K, int n, cutc_crt *c)
+{
+    memset(c, 0, sizeof *c);
+    if (n < 2 || n > CUTC_CRT_MAX || K < 1)
+        return CUTC_ERR_ARG;
+    u128 M = 1;
+    double log2M = 0.0;
+    for (int i = 0; i < n; i++) {
+        M *= (u128)k_moduli[i];
+        log2M += log2((double)k_moduli[i]);
+    }
+    int bits = (int)floor((log2M - 2.0 - log2((double)K)) / 2.0);
+    if (bits > 50)
+        bits = 50;
+    if (bits < 8)
+        return CUTC_ERR_ARG;
+    c->n = n;
+    c->bits = bits;
+    for (int i = 0; i < n; i++) {
+        const int m = k_moduli[i];
+        const u128 Mi = M / (u128)m;
+        c->m[i] = m;
+        c->inv_m[i] = 1.0 / (double)m;
+        c->y[i] = inv_mod((int)(Mi % (u128)m), m);
+        if (c->y[i] < 0)
+            return CUTC_ERR_ARG;
+        split3(Mi, c->Mi[i]);
+    }
+    split3(M, c->M);
+    c->Minv = 1.0 / ((double)c->M[0] * 1208925819614629174706176.0 + (double)c->M[1] * 1099511627776.0 +
+                     (double)c->M[2]);
+    return CUTC_OK;
+}

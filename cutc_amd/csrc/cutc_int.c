@@ -27,16 +27,6 @@
 
 void cutc_set_last_error(int code);
 
-/* split-precision stage 1 (split.hip) */
-int cutc_stage1_split(int precision, int slices, const double *Apk, int64_t lda, int64_t P, const double *G,
-                      int64_t ldg, int64_t n1c, int64_t K, const int *pairs, int n_ao, double *X, int64_t n1,
-                      int64_t i0, void *work, size_t work_bytes, hipStream_t st, double *ms_split, int *n_launch);
-size_t cutc_split_work_bytes(int precision, int slices, int64_t P, int64_t n1c, int64_t K);
-int cutc_stage2_split(int precision, int slices, const double *X, int64_t xld, const double *B, int64_t ldb,
-                      int64_t P, int64_t K, double alpha, const int *pairs, int n_ao, double *M, int mode,
-                      void *work, size_t work_bytes, hipStream_t st);
-size_t cutc_split2_work_bytes(int precision, int slices, int64_t P, int64_t K);
-
 #define NCAT 7
 enum { CAT_JAST = 0, CAT_INTEG, CAT_GEMM1, CAT_GEMM2, CAT_SYMM, CAT_SPLIT, CAT_TOTAL };
 
@@ -150,7 +140,7 @@ static int valid_precision(const cutc_options *o)
 {
     if (o->precision == CUTC_PREC_FP64)
         return 1;
-    if ((o->precision == CUTC_PREC_BF16 || o->precision == CUTC_PREC_FP16) && (o->slices == 3 || o->slices == 4))
+    if (o->precision == CUTC_PREC_INT8 && o->slices >= 6 && o->slices <= CUTC_CRT_MAX)
         return 1;
     return 0;
 }
@@ -231,22 +221,12 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
-    const int64_t nao2 = (int64_t)n_ao * n_ao;
     int *pairs = NULL, *dcls = NULL;
     cutc_jmono *dmono = NULL;
     double *Apk = NULL, *G = NULL, *f2tab = NULL, *tab = NULL;
-    void *work = NULL;
-    size_t work_bytes = 0;
     int rc = CUTC_OK, ev;
+    (void)opt;
 
-    if (n1 == 0 || n_ao == 0)
-        return CUTC_OK;
-    if (n2 == 0) {
-        ev = tr_begin(tr, CAT_GEMM1);
-        rc = cutc_launch_fill(X, 4 * n1 * nao2, 0.0, st);
-        tr_end(tr, ev);
-        return rc;
-    }
     TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
     TRYH(hipMalloc((void **)&Apk, sizeof(double) * P * ldk));
     TRYH(hipMalloc((void **)&G, sizeof(double) * 4 * n1c * ldk));
@@ -254,13 +234,9 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         TRYH(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * ldk));
     TRYH(hipMalloc((void **)&tab, sizeof(double) * (prog->ts * n1c + 1)));
     TRY(program_upload(prog, &dmono, &dcls));
-    if (opt->precision != CUTC_PREC_FP64) {
-        work_bytes = cutc_split_work_bytes(opt->precision, opt->slices, P, n1c, ldk);
-        TRYH(hipMalloc(&work, work_bytes));
-    }
     if (stats)
         stats->workspace_bytes = sizeof(int) * 2 * P + sizeof(double) * (P * ldk + 4 * n1c * ldk) +
-                                 (f2tab ? sizeof(double) * n_nuc * ldk : 0) + work_bytes;
+                                 (f2tab ? sizeof(double) * n_nuc * ldk : 0);
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_INTEG);
@@ -279,20 +255,10 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                                         n2, ldk, G, ldk, n1e * ldk, f2tab, st);
         tr_end(tr, ev);
         TRY(rc);
-        if (opt->precision == CUTC_PREC_FP64) {
-            ev = tr_begin(tr, CAT_GEMM1);
-            rc = cutc_gemm1_f64(Apk, ldk, G, ldk, P, n1e, ldk, pairs, n_ao, X, n1, i0, 0, st);
-            tr_end(tr, ev);
-            TRY(rc);
-        } else {
-            double ms = 0.0;
-            int nl = 0;
-            ev = tr_begin(tr, CAT_GEMM1);
-            rc = cutc_stage1_split(opt->precision, opt->slices, Apk, ldk, P, G, ldk, n1e, ldk, pairs, n_ao, X, n1,
-                                   i0, work, work_bytes, st, &ms, &nl);
-            tr_end(tr, ev);
-            TRY(rc);
-        }
+        ev = tr_begin(tr, CAT_GEMM1);
+        rc = cutc_gemm1_f64(Apk, ldk, G, ldk, P, n1e, ldk, pairs, n_ao, X, n1, i0, 0, st);
+        tr_end(tr, ev);
+        TRY(rc);
         if (stats) {
             /* executed contraction work, FP64-equivalent: 2 x pairs x (4 n1e) x n2 */
             stats->flops_exec += 2.0 * (double)P * 4.0 * (double)n1e * (double)n2;
@@ -313,7 +279,101 @@ done:
     dfree(tab);
     dfree(dmono);
     dfree(dcls);
-    dfree(work);
+    return rc;
+}
+
+/* Stage 1 with the modular (CRT) contraction: A quantised on the fly from
+ * w2 and chi, each G chunk quantised after the Jastrow block, one int8 GEMM
+ * per modulus, CRT rebuild scattered into X. */
+static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
+                      const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
+                      const cutc_jprog *prog, double *X, tracer_t *tr, cutc_stats *stats)
+{
+    const hipStream_t st = tr->st;
+    const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
+    const int nmod = opt->slices;
+    int *pairs = NULL, *dcls = NULL;
+    cutc_jmono *dmono = NULL;
+    double *G = NULL, *f2tab = NULL, *tab = NULL;
+    uint64_t *rmaxA = NULL, *rmaxG = NULL;
+    int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
+    int rc = CUTC_OK, ev;
+    cutc_crt crt;
+    TRY(cutc_crt_setup(n2, nmod, &crt));
+    const int nparts = CUTC_CRT_PARTS(ldk);
+    const size_t bA = (size_t)nmod * CUTC_CRT_ROWS(P) * ldk, bG = (size_t)nmod * CUTC_CRT_ROWS(4 * n1c) * ldk;
+    const size_t bC = (size_t)nparts * nmod * 4 * n1c * P;
+
+    TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
+    TRYH(hipMalloc((void **)&rmaxA, sizeof(uint64_t) * P));
+    TRYH(hipMalloc((void **)&rmaxG, sizeof(uint64_t) * 4 * n1c));
+    TRYH(hipMalloc((void **)&Apl, bA));
+    TRYH(hipMalloc((void **)&Gpl, bG));
+    TRYH(hipMalloc((void **)&Cpl, bC));
+    TRYH(hipMalloc((void **)&G, sizeof(double) * 4 * n1c * ldk));
+    if (n_nuc > 32)
+        TRYH(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * ldk));
+    TRYH(hipMalloc((void **)&tab, sizeof(double) * (prog->ts * n1c + 1)));
+    TRY(program_upload(prog, &dmono, &dcls));
+    if (stats)
+        stats->workspace_bytes = sizeof(int) * 2 * P + sizeof(uint64_t) * (P + 4 * n1c) + bA + bG + bC +
+                                 sizeof(double) * 4 * n1c * ldk + (f2tab ? sizeof(double) * n_nuc * ldk : 0);
+
+    TRY(cutc_launch_pair_table(n_ao, pairs, st));
+    ev = tr_begin(tr, CAT_SPLIT);
+    rc = cutc_crt_rowmax_pairw(wr2, aos2, n2, n_ao, rmaxA, st);
+    if (rc == CUTC_OK)
+        rc = cutc_crt_quant_pairw(&crt, wr2, aos2, n2, n_ao, ldk, rmaxA, Apl, st);
+    tr_end(tr, ev);
+    TRY(rc);
+
+    int nchunks = 0;
+    for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
+        const int64_t n1e = (n1 - i0 < n1c) ? n1 - i0 : n1c;
+        nchunks++;
+        ev = tr_begin(tr, CAT_JAST);
+        rc = cutc_launch_jast_prep(r1, i0, n1e, rn, n_nuc, dmono, prog->n_mono, tab, prog->ts, st);
+        if (rc == CUTC_OK)
+            rc = cutc_launch_jast_pairs(r1, r2, rn, n_nuc, tab, prog->ts, dcls, prog->n_cls, prog->kmax, i0, n1e, 0,
+                                        n2, ldk, G, ldk, n1e * ldk, f2tab, st);
+        tr_end(tr, ev);
+        TRY(rc);
+        ev = tr_begin(tr, CAT_SPLIT);
+        rc = cutc_crt_rowmax_k(G, ldk, 4 * n1e, n2, rmaxG, st);
+        if (rc == CUTC_OK)
+            rc = cutc_crt_quant_k(&crt, G, ldk, 4 * n1e, n2, ldk, rmaxG, Gpl, st);
+        tr_end(tr, ev);
+        TRY(rc);
+        ev = tr_begin(tr, CAT_GEMM1);
+        rc = cutc_crt_gemm(&crt, Apl, P, Gpl, 4 * n1e, ldk, Cpl, st);
+        tr_end(tr, ev);
+        TRY(rc);
+        ev = tr_begin(tr, CAT_SPLIT);
+        rc = cutc_crt_rebuild_x(&crt, Cpl, nparts, P, 4 * n1e, rmaxA, rmaxG, pairs, n_ao, X, n1, n1e, i0, st);
+        tr_end(tr, ev);
+        TRY(rc);
+        if (stats)
+            stats->flops_exec += 2.0 * (double)P * 4.0 * (double)n1e * (double)n2;
+    }
+    if (stats) {
+        stats->n1_chunk = n1c;
+        stats->n1_chunks = nchunks;
+    }
+    TRYH(hipStreamSynchronize(st));
+done:
+    if (rc != CUTC_OK)
+        hipStreamSynchronize(st);
+    dfree(pairs);
+    dfree(rmaxA);
+    dfree(rmaxG);
+    dfree(Apl);
+    dfree(Gpl);
+    dfree(Cpl);
+    dfree(G);
+    dfree(f2tab);
+    dfree(tab);
+    dfree(dmono);
+    dfree(dcls);
     return rc;
 }
 
@@ -327,29 +387,24 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
     const int64_t nao2 = (int64_t)n_ao * n_ao;
     int *pairs = NULL;
     double *Hp = NULL, *NHp = NULL;
-    void *work = NULL;
-    size_t work_bytes = 0;
+    uint64_t *rmaxA = NULL, *rmaxB = NULL;
+    int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
+    const int crt_mode = opt->precision == CUTC_PREC_INT8;
+    const int nmod = opt->slices;
 
-    if (n_ao == 0)
-        return CUTC_OK;
-    if (n1 == 0) {
-        ev = tr_begin(tr, CAT_GEMM2);
-        rc = cutc_launch_fill(out, nao2 * nao2, 0.0, st);
-        tr_end(tr, ev);
-        if (rc == CUTC_OK)
-            rc = hip_rc(hipStreamSynchronize(st));
-        return rc;
-    }
     const int64_t rows = (n1c + 15) / 16 * 16;
     const int64_t plane = rows * P;
+    const int64_t kp_max = (n1c + 127) / 128 * 128;
     TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
     TRYH(hipMalloc((void **)&Hp, sizeof(double) * plane));
     TRYH(hipMalloc((void **)&NHp, sizeof(double) * 3 * plane));
-    if (opt->precision != CUTC_PREC_FP64) {
-        work_bytes = cutc_split2_work_bytes(opt->precision, opt->slices, P, rows);
-        if (work_bytes)
-            TRYH(hipMalloc(&work, work_bytes));
+    if (crt_mode) {
+        TRYH(hipMalloc((void **)&rmaxA, sizeof(uint64_t) * P));
+        TRYH(hipMalloc((void **)&rmaxB, sizeof(uint64_t) * P));
+        TRYH(hipMalloc((void **)&Apl, (size_t)nmod * CUTC_CRT_ROWS(P) * 3 * kp_max));
+        TRYH(hipMalloc((void **)&Bpl, (size_t)nmod * CUTC_CRT_ROWS(P) * 3 * kp_max));
+        TRYH(hipMalloc((void **)&Cpl, (size_t)CUTC_CRT_PARTS(3 * kp_max) * nmod * P * P));
     }
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
 
@@ -362,22 +417,57 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
         tr_end(tr, ev);
         TRY(rc);
-        ev = tr_begin(tr, CAT_GEMM2);
-        if (opt->precision == CUTC_PREC_FP64) {
+        if (!crt_mode) {
+            ev = tr_begin(tr, CAT_GEMM2);
             rc = cutc_gemm2_f64(X + (3 * n1 + i0) * nao2, nao2, Hp, P, P, n1e, 1.0, pairs, n_ao, out, i0 == 0 ? 0 : 1,
                                 st);
             for (int m = 0; m < 3 && rc == CUTC_OK; m++)
                 rc = cutc_gemm2_f64(X + (m * n1 + i0) * nao2, nao2, NHp + m * plane, P, P, n1e, -0.5, pairs, n_ao,
                                     out, 2, st);
+            tr_end(tr, ev);
+            TRY(rc);
         } else {
-            rc = cutc_stage2_split(opt->precision, opt->slices, X + (3 * n1 + i0) * nao2, nao2, Hp, P, P, n1e, 1.0,
-                                   pairs, n_ao, out, i0 == 0 ? 0 : 1, work, work_bytes, st);
-            for (int m = 0; m < 3 && rc == CUTC_OK; m++)
-                rc = cutc_stage2_split(opt->precision, opt->slices, X + (m * n1 + i0) * nao2, nao2, NHp + m * plane,
-                                       P, P, n1e, -0.5, pairs, n_ao, out, 2, work, work_bytes, st);
+            /* X^3 H, then X^{0..2} NH^{0..2} as one contraction over (m, r1) */
+            const int64_t kp = (n1e + 127) / 128 * 128;
+            for (int part = 0; part < 2; part++) {
+                const int nseg = part == 0 ? 1 : 3;
+                cutc_crt crt;
+                TRY(cutc_crt_setup(nseg * n1e, nmod, &crt));
+                cutc_colsrc sa, sb;
+                memset(&sa, 0, sizeof sa);
+                memset(&sb, 0, sizeof sb);
+                sa.nsrc = sb.nsrc = nseg;
+                sa.ld = nao2;
+                sb.ld = P;
+                sa.K = sb.K = n1e;
+                sa.pairs = pairs;
+                sa.n_ao = n_ao;
+                for (int g = 0; g < nseg; g++) {
+                    const int m = part == 0 ? 3 : g;
+                    sa.p[g] = X + (m * n1 + i0) * nao2;
+                    sb.p[g] = part == 0 ? Hp : NHp + g * plane;
+                }
+                ev = tr_begin(tr, CAT_SPLIT);
+                rc = cutc_crt_rowmax_colk(&sa, P, rmaxA, st);
+                if (rc == CUTC_OK)
+                    rc = cutc_crt_quant_colk(&crt, &sa, P, kp, rmaxA, Apl, st);
+                if (rc == CUTC_OK)
+                    rc = cutc_crt_rowmax_colk(&sb, P, rmaxB, st);
+                if (rc == CUTC_OK)
+                    rc = cutc_crt_quant_colk(&crt, &sb, P, kp, rmaxB, Bpl, st);
+                tr_end(tr, ev);
+                TRY(rc);
+                ev = tr_begin(tr, CAT_GEMM2);
+                rc = cutc_crt_gemm(&crt, Apl, P, Bpl, P, nseg * kp, Cpl, st);
+                tr_end(tr, ev);
+                TRY(rc);
+                ev = tr_begin(tr, CAT_SPLIT);
+                rc = cutc_crt_rebuild_m(&crt, Cpl, CUTC_CRT_PARTS(nseg * kp), P, rmaxA, rmaxB, part == 0 ? 1.0 : -0.5,
+                                        pairs, n_ao, out, part == 0 ? (i0 == 0 ? 0 : 1) : 2, st);
+                tr_end(tr, ev);
+                TRY(rc);
+            }
         }
-        tr_end(tr, ev);
-        TRY(rc);
         if (stats)
             stats->flops_exec += 2.0 * (double)P * (double)P * 4.0 * (double)n1e;
     }
@@ -392,7 +482,11 @@ done:
     dfree(pairs);
     dfree(Hp);
     dfree(NHp);
-    dfree(work);
+    dfree(rmaxA);
+    dfree(rmaxB);
+    dfree(Apl);
+    dfree(Bpl);
+    dfree(Cpl);
     return rc;
 }
 
@@ -461,11 +555,26 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
     tracer_t tr;
     tr_init(&tr, st, stats != NULL || opt->verbose);
     int evt = tr_begin(&tr, CAT_TOTAL);
-    if (stages & 1) {
-        rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
+    if ((stages & 1) && n1 > 0 && n_ao > 0) {
+        if (n2 == 0) {
+            int ev = tr_begin(&tr, CAT_GEMM1);
+            rc = cutc_launch_fill(X, 4 * (int64_t)n1 * n_ao * n_ao, 0.0, st);
+            tr_end(&tr, ev);
+        } else if (opt->precision == CUTC_PREC_INT8) {
+            rc = stage1_crt(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
+        } else {
+            rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
+        }
     }
-    if (rc == CUTC_OK && (stages & 2))
-        rc = stage2(opt, &pl, n1, n_ao, wr1, aos1, X, out, &tr, stats);
+    if (rc == CUTC_OK && (stages & 2) && n_ao > 0) {
+        if (n1 == 0) {
+            int ev = tr_begin(&tr, CAT_GEMM2);
+            rc = cutc_launch_fill(out, (int64_t)n_ao * n_ao * n_ao * n_ao, 0.0, st);
+            tr_end(&tr, ev);
+        } else {
+            rc = stage2(opt, &pl, n1, n_ao, wr1, aos1, X, out, &tr, stats);
+        }
+    }
     tr_end(&tr, evt);
     if (rc == CUTC_OK)
         rc = hip_rc(hipStreamSynchronize(st));

@@ -77,8 +77,64 @@ int cutc_gemm1_f64(const double *Apk, int64_t lda, const double *G, int64_t ldg,
 int cutc_gemm2_f64(const double *X, int64_t xld, const double *B, int64_t ldb, int64_t P, int64_t K, double alpha,
                    const int *pairs, int n_ao, double *M, int mode, hipStream_t st);
 
-/* Split-precision (Ozaki) path (split.hip). */
-int cutc_split_supported(int precision, int slices);
+/* Modular (CRT) split-precision contractions: crt_host.c (host constants) and
+ * crt.hip (kernels).  Residue planes are int8 [n][rows][kpad], kpad a multiple
+ * of 128, zero past the valid K.  Row maxima are kept as the uint64 bit
+ * pattern of max |x| (ordered like the doubles, so atomicMax works). */
+#define CUTC_CRT_MAX 16
+typedef struct cutc_crt {
+    int n, bits;
+    int m[CUTC_CRT_MAX];
+    int y[CUTC_CRT_MAX];
+    double inv_m[CUTC_CRT_MAX];
+    double Mi[CUTC_CRT_MAX][3];
+    double M[3];
+    double Minv;
+} cutc_crt;
+
+int cutc_crt_setup(int64_t K, int n, cutc_crt *c);
+
+/* rows of a residue plane: padded to the GEMM tile (256) */
+#define CUTC_CRT_ROWS(rows) ((((int64_t)(rows)) + 255) / 256 * 256)
+
+/* column-major sources for the stage-2 operands: element (row r, k) of
+ * segment s is p[s][k * ld + col(r)], col(r) = k_r + n_ao l_r through the
+ * pair table when pairs != NULL, else r. */
+typedef struct cutc_colsrc {
+    const double *p[3];
+    int nsrc;
+    int64_t ld, K;
+    const int *pairs;
+    int n_ao;
+} cutc_colsrc;
+
+int cutc_crt_rowmax_k(const double *X, int64_t ld, int64_t rows, int64_t K, uint64_t *rmax, hipStream_t st);
+int cutc_crt_quant_k(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
+                     const uint64_t *rmax, int8_t *planes, hipStream_t st);
+int cutc_crt_rowmax_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, uint64_t *rmax,
+                          hipStream_t st);
+int cutc_crt_quant_pairw(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
+                         int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st);
+int cutc_crt_rowmax_colk(const cutc_colsrc *src, int64_t rows, uint64_t *rmax, hipStream_t st);
+int cutc_crt_quant_colk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kpad,
+                        const uint64_t *rmax, int8_t *planes, hipStream_t st);
+/* K parts of at most CUTC_CRT_KPART columns keep the int32 sums exact. */
+#define CUTC_CRT_KPART 131072
+#define CUTC_CRT_PARTS(K) ((int)((((int64_t)(K)) + CUTC_CRT_KPART - 1) / CUTC_CRT_KPART))
+/* C[part][i][j][p] = (sum_{k in part} A[i][p][k] B[i][j][k]) mod m_i as int8
+ * in [-128, 127]; A: M rows, B: N rows (planes with CUTC_CRT_ROWS rows), both
+ * with Ktot (multiple of 128) columns.  C holds CUTC_CRT_PARTS(Ktot) n M N bytes. */
+int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t Ktot,
+                  int8_t *C, hipStream_t st);
+/* Rebuild C = V 2^(eA + eB - 2 bits) and scatter into X (4, n1, n_ao, n_ao):
+ * column j = m n1c + i1 -> X[m][i0 + i1][(k,l) and (l,k)]. */
+int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, int64_t N, const uint64_t *rmaxA,
+                       const uint64_t *rmaxB, const int *pairs, int n_ao, double *X, int64_t n1, int64_t n1c,
+                       int64_t i0, hipStream_t st);
+/* Rebuild alpha C into the n_ao^2 x n_ao^2 matrix M; mode as cutc_gemm2_f64. */
+int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, const uint64_t *rmaxA,
+                       const uint64_t *rmaxB, double alpha, const int *pairs, int n_ao, double *M, int mode,
+                       hipStream_t st);
 
 #ifdef __cplusplus
 }

@@ -4,24 +4,23 @@
  * 63-99), whose 32-bit arithmetic overflows at n_ao = 200 and whose r2 chunk
  * does not match its own memory formula.  Here every size is 64-bit, all r2
  * points stay in one K sweep, and only r1 is chunked:
- *   stage 1: pairs table + Apk (n_pairs x ldk) + G chunk (4 x n1c x ldk)
- *            [+ f2 table for > 32 nuclei] [+ operand slices in split modes]
- *   stage 2: Hp and NHp for an r1 chunk (4 planes x roundup(n1c2,16) x n_pairs)
+ *   FP64 stage 1: pairs table + Apk (n_pairs x ldk) + G chunk (4 x n1c x ldk)
+ *                 [+ f2 table for > 32 nuclei]
+ *   INT8 stage 1: pairs + row maxima + A residues (nmod x n_pairs x ldk bytes)
+ *                 + per r1 row: G (4 ldk doubles), its residues (4 nmod ldk
+ *                 bytes) and the GEMM residues (parts x nmod x 4 n_pairs bytes)
+ *   stage 2:      Hp and NHp for an r1 chunk (4 planes x roundup(n1c2,16) x
+ *                 n_pairs doubles) [INT8: + both operands' residues,
+ *                 2 x nmod x 3 x n_pairs bytes per r1 row, + the GEMM residues
+ *                 2 x nmod x n_pairs^2 bytes]
  * Chunks are balanced (equal sizes rounded up to a multiple of 64). */
 #include <stdint.h>
 
 #include "cutc_int.h"
+#include "kernels/launch.h"
 
 static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static int64_t rup(int64_t a, int64_t b) { return cdiv(a, b) * b; }
-
-/* Bytes of the split-mode operand copies per element of Apk and of G. */
-static int64_t split_bytes_per_elem(int precision, int slices)
-{
-    if (precision == CUTC_PREC_FP64)
-        return 0;
-    return 2 * (int64_t)slices; /* 16-bit slices */
-}
 
 static int64_t balanced(int64_t n, int64_t cmax, int64_t align)
 {
@@ -39,42 +38,55 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
 {
     if (n1 < 0 || n2 < 0 || n_ao < 0 || n_nuc < 0 || !pl)
         return CUTC_ERR_ARG;
+    const int crt = precision == CUTC_PREC_INT8;
+    const int64_t nmod = crt ? slices : 0;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
-    const int64_t ldk = rup(n2, 64);
-    const int64_t sb = split_bytes_per_elem(precision, slices);
+    const int64_t ldk = rup(n2, crt ? 128 : 64);
     pl->n_pairs = P;
     pl->ldk = ldk;
 
     /* stage 1 */
-    int64_t fixed1 = 8 * P + 8 * P * ldk + sb * P * ldk + 4096;
+    int64_t fixed1, per_r1;
+    if (crt) {
+        const int64_t parts = CUTC_CRT_PARTS(ldk);
+        fixed1 = 16 * P + nmod * CUTC_CRT_ROWS(P) * ldk + nmod * 255 * ldk + 4096;
+        per_r1 = 32 * ldk + 32 + 4 * nmod * ldk + parts * nmod * 4 * P;
+    } else {
+        fixed1 = 8 * P + 8 * P * ldk + 4096;
+        per_r1 = 32 * ldk;
+    }
     if (n_nuc > 32)
         fixed1 += 8 * (int64_t)n_nuc * ldk;
-    int64_t per_r1 = (32 + 4 * sb) * ldk; /* G rows + their slices */
     int64_t n1c = 0;
-    if (n1 > 0) {
-        if ((int64_t)budget <= fixed1)
+    if (n1 > 0 && P > 0 && n2 > 0) {
+        if ((int64_t)budget <= fixed1 + per_r1)
             return CUTC_ERR_NOMEM;
-        int64_t cmax = per_r1 > 0 ? ((int64_t)budget - fixed1) / per_r1 : n1;
-        if (cmax < 1)
-            return CUTC_ERR_NOMEM;
+        int64_t cmax = ((int64_t)budget - fixed1) / per_r1;
         if (n1c_req > 0 && n1c_req < cmax)
             cmax = n1c_req;
         n1c = balanced(n1, cmax, cmax >= 64 ? 64 : 1);
+    } else if (n1 > 0) {
+        n1c = n1;
     }
     pl->n1_chunk = n1c;
     pl->bytes_stage1 = (size_t)(fixed1 + per_r1 * n1c);
 
     /* stage 2 */
-    int64_t fixed2 = 8 * P + 4096;
-    int64_t per_r1b = 4 * 8 * P; /* Hp + 3 NHp planes */
+    int64_t fixed2 = 8 * P + 4096, per_r1b = 4 * 8 * P;
+    if (crt) {
+        fixed2 += 16 * P + 2 * nmod * P * P + 2 * nmod * 255 * 3 * 128;
+        per_r1b += 2 * nmod * 3 * CUTC_CRT_ROWS(P);
+    }
     int64_t n1c2 = 0;
     if (n1 > 0 && P > 0) {
-        if ((int64_t)budget <= fixed2 + 16 * per_r1b)
+        if ((int64_t)budget <= fixed2 + 144 * per_r1b)
             return CUTC_ERR_NOMEM;
-        int64_t cmax = ((int64_t)budget - fixed2) / per_r1b - 16;
+        int64_t cmax = ((int64_t)budget - fixed2) / per_r1b - 144;
+        if (crt && cmax > 87296) /* keeps the (m, r1) contraction within two int32 K parts */
+            cmax = 87296;
         n1c2 = balanced(n1, cmax, cmax >= 64 ? 64 : 1);
     }
     pl->n1_chunk_2e = n1c2;
-    pl->bytes_stage2 = (size_t)(fixed2 + per_r1b * rup(n1c2, 16));
+    pl->bytes_stage2 = (size_t)(fixed2 + per_r1b * (rup(n1c2, 16) + 128));
     return CUTC_OK;
 }

@@ -29,7 +29,9 @@ CUTC_ERR_UNSUPPORTED = -5
 PREC_FP64 = 0
 PREC_BF16 = 1
 PREC_FP16 = 2
-PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16}
+PREC_INT8 = 3
+PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16, "int8": PREC_INT8}
+DEFAULT_SLICES = {PREC_FP64: 4, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 12}
 
 
 class CutcError(RuntimeError):
@@ -126,20 +128,23 @@ def _check(rc, where):
         raise CutcError(rc, where)
 
 
-def options(precision="fp64", slices=4, verbose=False, n1_chunk=0, mem_limit=0) -> Options:
+def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0) -> Options:
+    """slices: bf16/fp16 slices per operand, or the number of moduli for int8
+    (None: 4, resp. 12)."""
     o = Options()
     lib().cutc_default_options(ctypes.byref(o))
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-    o.slices = slices
+    o.slices = DEFAULT_SLICES[o.precision] if slices is None else slices
     o.verbose = int(verbose)
     o.n1_chunk = n1_chunk
     o.mem_limit = mem_limit
     return o
 
 
-def make_plan(n1, n2, n_ao, n_nuc, budget, precision="fp64", slices=4, n1_chunk=0) -> Plan:
+def make_plan(n1, n2, n_ao, n_nuc, budget, precision="fp64", slices=None, n1_chunk=0) -> Plan:
     p = Plan()
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    slices = DEFAULT_SLICES[prec] if slices is None else slices
     _check(lib().cutc_make_plan(n1, n2, n_ao, n_nuc, prec, slices, budget, n1_chunk, ctypes.byref(p)),
            "cutc_make_plan")
     return p

@@ -104,13 +104,15 @@ int cutc_get_last_error(void);
 const char *cutc_error_string(int code);
 
 /* Arithmetic of the two dense contractions. */
-#define CUTC_PREC_FP64 0 /* FP64 MFMA                                         */
-#define CUTC_PREC_BF16 1 /* Ozaki slices in bf16, FP32 MFMA accumulate -> FP64 */
-#define CUTC_PREC_FP16 2 /* Ozaki slices in fp16, FP32 MFMA accumulate -> FP64 */
+#define CUTC_PREC_FP64 0 /* FP64 MFMA                                                 */
+#define CUTC_PREC_BF16 1 /* Ozaki slices in bf16, FP32 MFMA accumulate -> FP64         */
+#define CUTC_PREC_FP16 2 /* Ozaki slices in fp16, FP32 MFMA accumulate -> FP64         */
+#define CUTC_PREC_INT8 3 /* operands rounded to integers, products modulo `slices`
+                            moduli on int8 MFMA (exact int32), CRT rebuild in FP64  */
 
 typedef struct cutc_options {
     int precision;    /* CUTC_PREC_*                                             */
-    int slices;       /* split modes: slices per operand (3 or 4)                */
+    int slices;       /* bf16/fp16: slices per operand (3, 4); int8: moduli (6..16) */
     int verbose;      /* 1: print the reference's stage-time lines               */
     int n1_chunk;     /* r1 rows per G chunk, 0 = planner                        */
     size_t mem_limit; /* device bytes the workspaces may use, 0 = free memory    */
