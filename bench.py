@@ -1,17 +1,24 @@
-"""Benchmark of the TC integral build on one MI355X.
+"""Benchmark of the TC integral build on one MI355X: split precision vs FP64.
 
 A step is one full build on inputs already resident in HBM: Jastrow pair
 blocks -> integrands -> step-1 contraction X_kl(r1) -> step-2 contraction ->
-symmetrised int_2e_ao (cutc_int_device through the C ABI).  Workload: 20
-nuclei in a 12-bohr cube, n_ao = 200, n1 = 60,000, n2 = 200,000 (seeded
+symmetrised int_2e_ao (cutc_int_device through the C ABI), on the workload
+20 nuclei in a 12-bohr cube, n_ao = 200, n1 = 60,000, n2 = 200,000 (seeded
 synthetic inputs).
 
-value = effective TFLOP/s = W_ref / build seconds, with the reference
-formulation's work W_ref = 8 n_ao^2 n1 n2 + 8 n_ao^4 n1 (full AO pairs).  The
-build executes only the packed k <= l pairs; the roofline fraction is taken on
-the executed work W_exec, never on W_ref.
+Default run: one FP64 build (the plain path, after a small warm-up call),
+then W warm-up and K timed builds with the split-precision contraction
+(--precision, default int8 = modular int8 MFMA with 12 moduli), then the
+split results are compared with the FP64 ones: relative Frobenius and
+max-abs error of every X component and of int_2e_ao, and of
+sum_{kl,ab} D_kl D_ab (kl|ab) for a seeded rank-10 PSD density D.
 
-Prints one JSON line (rank 0, N = 1).
+value = effective TFLOP/s of the timed (split) build = W_ref / seconds, with
+the reference formulation's work W_ref = 8 n_ao^2 n1 n2 + 8 n_ao^4 n1 (full
+AO pairs, the metric's definition).  The build executes only the packed
+k <= l pairs: roofline fractions are computed on the executed work.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -24,8 +31,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "int_2e_ao build s + eff TFLOP/s: split-BF16 vs FP64 on GPU at \u22641e-10 rel err"
-PEAK = {"fp64": 78.6, "bf16": 2516.0, "fp16": 2516.0}  # dense MFMA TFLOP/s, MI355X
+METRIC = "int_2e_ao build s + eff TFLOP/s: split-BF16 vs FP64 on GPU at ≤1e-10 rel err"
+PEAK = {"fp64": 78.6, "int8": 5000.0, "bf16": 2516.0, "fp16": 2516.0}  # dense MFMA, MI355X (int8: TOPS)
 
 
 def w_ref(n_ao, n1, n2):
@@ -37,8 +44,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "bf16", "fp16"])
-    ap.add_argument("--slices", type=int, default=4)
+    ap.add_argument("--precision", default="int8", choices=["fp64", "int8"])
+    ap.add_argument("--slices", type=int, default=None, help="moduli of the int8 contraction (default 12)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
     ap.add_argument("--n1", type=int, default=60000)
     ap.add_argument("--n2", type=int, default=200000)
     ap.add_argument("--cpu-sample", type=int, default=32, help="r1 points of the CPU-baseline sample (0: skip)")
@@ -51,11 +59,11 @@ def log(args, *a):
         print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(args, sysfull):
-    """The FP64 oracle (C + OpenMP + OpenBLAS) on the first cpu_sample r1
-    points of the same system, timed on this host."""
+def cpu_baseline(sysfull, npts):
+    """The FP64 oracle (C + OpenMP + OpenBLAS) on the first npts r1 points
+    of the same system, timed on this host."""
     from oracle import oracle
-    s = sysfull.subset_r1(args.cpu_sample)
+    s = sysfull.subset_r1(npts)
     oracle.lib()
     t0 = time.perf_counter()
     X = oracle.int2_grad1_u12_ao(s)
@@ -64,9 +72,29 @@ def cpu_baseline(args, sysfull):
     del X, I
     w = w_ref(s.n_ao, s.n1, s.n2)
     return {"value": w / dt / 1e12, "unit": "TFLOP/s", "cores": oracle.threads(), "kind": "port",
-            "sample": f"oracle ({oracle.blas_name()} DGEMM) on the first {s.n1} of {sysfull.n1} r1 points, all "
-                      f"{s.n2} r2 points, n_ao={s.n_ao}: {dt:.2f} s; value = W_ref(sample)/s",
+            "sample": f"FP64 oracle ({oracle.blas_name()} DGEMM) on the first {s.n1} of {sysfull.n1} r1 points, all "
+                      f"{s.n2} r2 points, n_ao={s.n_ao}: {dt:.2f} s; value = W_ref(sample) / s",
             "seconds": dt}
+
+
+def timed_builds(cutc, torch, ds, X, I, opt, n, stream):
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    stats = []
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(n):
+        stats.append(cutc.cutc_int_device(ds, X, I, opt, stream))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / n, stats
+
+
+def density(n_ao, seed=20261205):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n_ao, 10))
+    return C @ C.T
 
 
 def main():
@@ -84,43 +112,65 @@ def main():
     log(args, f"generated {s.shape_str()} in {time.perf_counter() - t0:.1f} s")
     ds = cutc.DeviceSystem.upload(s)
     X, I = cutc.alloc_outputs(ds)
-    torch.cuda.synchronize()
-    opt = cutc.options(args.precision, args.slices)
     stream = torch.cuda.current_stream()
+    n_ao, P = s.n_ao, s.n_ao * (s.n_ao + 1) // 2
+    W = w_ref(n_ao, s.n1, s.n2)
+    D = torch.from_numpy(density(n_ao)).cuda().reshape(-1)
 
-    for i in range(args.warmup):
-        st = cutc.cutc_int_device(ds, X, I, opt, stream)
-        log(args, f"warmup {i}: {st['ms_total'] / 1e3:.2f} s")
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    stats = []
-    ev0.record(stream)
-    for i in range(args.steps):
-        stats.append(cutc.cutc_int_device(ds, X, I, opt, stream))
-        log(args, f"step {i}: {stats[-1]['ms_total'] / 1e3:.2f} s  {stats[-1]}")
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / args.steps
+    def dd(Imat):  # sum D_kl D_ab (kl|ab)
+        return float(D @ (Imat.reshape(n_ao * n_ao, n_ao * n_ao) @ D))
 
-    W = w_ref(s.n_ao, s.n1, s.n2)
-    last = stats[-1]
-    # dominant kernel: the step-1 contraction; per-launch time from the library's
-    # stream-ordered hipEvents around every launch
-    g1_ms = last["ms_gemm1"] / max(1, last["n_gemm1"])
-    P = s.n_ao * (s.n_ao + 1) // 2
-    n1c = last["n1_chunk"]
-    chunk_flops = 2.0 * P * 4 * n1c * s.n2  # executed FP64 work of one full-size launch
-    # average over launches with their own sizes
-    g1_flops_total = 2.0 * P * 4 * s.n1 * s.n2
-    achieved = g1_flops_total / (last["ms_gemm1"] / 1e3) / 1e12
-    peak = PEAK[args.precision] if args.precision == "fp64" else PEAK["fp64"]
-    roof = {"kernel": "gemm_f64_kernel (step-1 contraction)" if args.precision == "fp64" else "split step-1",
-            "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None, "launches": last["n_gemm1"], "avg_launch_ms": g1_ms,
-            "flops_per_full_launch": chunk_flops}
-    out = {
-        "metric": METRIC,
+    out = {"metric": METRIC}
+    fp64 = None
+    Xref_host = None
+    if args.precision == "fp64" or not args.no_fp64:
+        small = s.subset_r1(min(256, s.n1))
+        dsm = cutc.DeviceSystem.upload(small)
+        Xs, Is = cutc.alloc_outputs(dsm)
+        cutc.cutc_int_device(dsm, Xs, Is, cutc.options("fp64"), stream)  # loads the kernels
+        del Xs, Is, dsm
+        nrep = args.steps if args.precision == "fp64" else 1
+        ms, st = timed_builds(cutc, torch, ds, X, I, cutc.options("fp64"), nrep, stream)
+        log(args, f"fp64 build: {ms / 1e3:.2f} s  {st[-1]}")
+        fp64 = {"build_s": ms / 1e3, "eff_tflops": W / (ms / 1e3) / 1e12, "stats": st[-1]}
+        if args.precision != "fp64":
+            Iref = I.clone()
+            Xref_host = X.cpu()
+            dd_ref = dd(Iref)
+
+    if args.precision == "fp64":
+        ms, last = fp64["build_s"] * 1e3, fp64["stats"]
+        prec_label, opt = "fp64", cutc.options("fp64")
+    else:
+        opt = cutc.options(args.precision, args.slices)
+        prec_label = f"{args.precision}x{opt.slices}"
+        for i in range(args.warmup):
+            st = cutc.cutc_int_device(ds, X, I, opt, stream)
+            log(args, f"warmup {i}: {st['ms_total'] / 1e3:.2f} s")
+        ms, sts = timed_builds(cutc, torch, ds, X, I, opt, args.steps, stream)
+        last = sts[-1]
+        log(args, f"{prec_label} build: {ms / 1e3:.2f} s  {last}")
+
+    # roofline of the dominant kernel (the step-1 contraction)
+    exec1 = 2.0 * P * 4 * s.n1 * s.n2
+    if args.precision == "fp64":
+        achieved = exec1 / (last["ms_gemm1"] / 1e3) / 1e12
+        roof = {"kernel": "gemm_f64_kernel<0,0,*> (step-1 contraction)", "bound": "mfma", "achieved": achieved,
+                "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": None,
+                "launches": last["n_gemm1"], "avg_launch_ms": last["ms_gemm1"] / max(1, last["n_gemm1"]),
+                "work": "2 x 20100 pairs x 4 n1 x n2 FP64 flops (packed pairs)"}
+    else:
+        kpad = (s.n2 + 127) // 128 * 128
+        int8_ops = 2.0 * opt.slices * P * 4 * s.n1 * kpad
+        achieved = int8_ops / (last["ms_gemm1"] / 1e3) / 1e12
+        roof = {"kernel": "crt_gemm_kernel (step-1 contraction, one launch per modulus and K part)",
+                "bound": "mfma", "achieved": achieved, "peak": PEAK["int8"], "unit": "TOPS (int8)",
+                "frac": achieved / PEAK["int8"], "traffic": None, "launches": last["n_gemm1"],
+                "avg_launch_ms": last["ms_gemm1"] / max(1, last["n_gemm1"]),
+                "work": f"2 x {opt.slices} moduli x 20100 pairs x 4 n1 x roundup(n2,128) int8 ops",
+                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12}
+
+    out.update({
         "value": W / (ms / 1e3) / 1e12,
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -130,23 +180,51 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else f"{args.precision}x{args.slices}->f64",
+        "dtype": "f64" if args.precision == "fp64" else f"{prec_label}->f64",
         "data": "synthetic",
-        "config": {"workload": f"cube20: 20 nuclei, n_ao={s.n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6",
-                   "precision": args.precision, "slices": args.slices if args.precision != "fp64" else None,
-                   "n1_chunk": n1c},
+        "config": {"workload": f"cube20: 20 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6",
+                   "precision": prec_label, "n1_chunk": last["n1_chunk"]},
         "build_s": ms / 1e3,
         "w_ref_flops": W,
-        "w_exec_flops": last["flops_exec"],
-        "exec_tflops": last["flops_exec"] / (ms / 1e3) / 1e12,
-        "stage_ms": {k: last[k] for k in ("ms_jastrow", "ms_integrands", "ms_gemm1", "ms_gemm2", "ms_symm")},
+        "w_exec_fp64_equiv": last["flops_exec"],
+        "stage_ms": {k: last[k] for k in ("ms_jastrow", "ms_integrands", "ms_split", "ms_gemm1", "ms_gemm2",
+                                          "ms_symm")},
         "roofline": roof,
-        "cpu_baseline": None,
-    }
+    })
+    if fp64 is not None and args.precision != "fp64":
+        out["fp64_build_s"] = fp64["build_s"]
+        out["fp64_eff_tflops"] = fp64["eff_tflops"]
+        out["speedup_vs_fp64"] = fp64["build_s"] / (ms / 1e3)
+        fp64_g1 = fp64["stats"]["ms_gemm1"] / 1e3
+        out["fp64_roofline"] = {"kernel": "gemm_f64_kernel<0,0,*>", "achieved": exec1 / fp64_g1 / 1e12,
+                                "peak": PEAK["fp64"], "frac": exec1 / fp64_g1 / 1e12 / PEAK["fp64"]}
+        # errors of the split build against the FP64 build, every tensor
+        err = {}
+        dI = I - Iref
+        err["int_2e_ao"] = {"rel_frob": float(torch.linalg.norm(dI) / torch.linalg.norm(Iref)),
+                            "max_abs": float(dI.abs().max()), "ref_max_abs": float(Iref.abs().max())}
+        del dI
+        for m in range(4):
+            num = den = mx = rmx = 0.0
+            step = 2000
+            for r0 in range(0, s.n1, step):
+                a = X[m, r0:r0 + step]
+                b = Xref_host[m, r0:r0 + step].to(a.device)
+                d = a - b
+                num += float((d * d).sum())
+                den += float((b * b).sum())
+                mx = max(mx, float(d.abs().max()))
+                rmx = max(rmx, float(b.abs().max()))
+            err[f"X{m}"] = {"rel_frob": (num / den) ** 0.5 if den > 0 else 0.0, "max_abs": mx, "ref_max_abs": rmx}
+        dd_split = dd(I)
+        err["DD_contraction"] = {"value": dd_split, "fp64_value": dd_ref,
+                                 "rel": abs(dd_split - dd_ref) / abs(dd_ref)}
+        err["max_rel_frob"] = max(v["rel_frob"] for v in err.values() if isinstance(v, dict) and "rel_frob" in v)
+        out["errors_vs_fp64"] = err
+        out["meets_1e-10"] = err["max_rel_frob"] <= 1e-10
     del X, I, ds
     torch.cuda.empty_cache()
-    if args.cpu_sample > 0:
-        out["cpu_baseline"] = cpu_baseline(args, s)
+    out["cpu_baseline"] = cpu_baseline(s, args.cpu_sample) if args.cpu_sample > 0 else None
     print(json.dumps(out), flush=True)
 
 
