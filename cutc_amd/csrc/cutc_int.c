@@ -16,6 +16,7 @@
  * errors are returned instead of exit(0); kernels run back to back on one
  * stream with timing events read once at the end.
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -195,11 +196,15 @@ static int program_upload(const cutc_jprog *prog, cutc_jmono **dmono, int **dcls
     return rc;
 }
 
+static size_t arena_size_current(void);
+
+/* free device memory plus this device's cached workspace, minus a margin */
 static size_t device_budget(const cutc_options *o)
 {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess)
         return 0;
+    fr += arena_size_current();
     const size_t margin = (size_t)1 << 30;
     size_t b = fr > margin ? fr - margin : 0;
     if (o && o->mem_limit && o->mem_limit < b)
@@ -212,6 +217,101 @@ static void dfree(void *p)
     if (p)
         hipFree(p);
 }
+
+/* Device workspace: one grow-only arena per device, kept between calls of
+ * the additive device entry point so that repeated builds allocate nothing
+ * (the reference allocates and frees every buffer in every call).  A stage
+ * sums its buffer sizes, makes sure the arena is at least that large, and
+ * carves the buffers with a 256-byte-aligned bump pointer.  All calls are
+ * serialised by g_lock because they share the arenas. */
+#define WS_MAX_DEV 64
+static struct {
+    char *base;
+    size_t size;
+} g_arena[WS_MAX_DEV];
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+
+typedef struct {
+    char *base;
+    size_t off, cap;
+} ws_t;
+
+static size_t ws_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static size_t arena_size_current(void)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
+        return 0;
+    return g_arena[dev].size;
+}
+
+static int ws_begin(ws_t *w, size_t need)
+{
+    int dev = 0;
+    memset(w, 0, sizeof *w);
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
+        return CUTC_ERR_HIP;
+    if (g_arena[dev].size < need) {
+        if (g_arena[dev].base) {
+            hipDeviceSynchronize();
+            hipFree(g_arena[dev].base);
+        }
+        g_arena[dev].base = NULL;
+        g_arena[dev].size = 0;
+        void *p = NULL;
+        int rc = hip_rc(hipMalloc(&p, need));
+        if (rc)
+            return rc;
+        g_arena[dev].base = p;
+        g_arena[dev].size = need;
+    }
+    w->base = g_arena[dev].base;
+    w->cap = g_arena[dev].size;
+    return CUTC_OK;
+}
+
+static void *ws_take(ws_t *w, size_t bytes)
+{
+    bytes = ws_round(bytes);
+    if (w->off + bytes > w->cap)
+        return NULL;
+    void *p = w->base + w->off;
+    w->off += bytes;
+    return p;
+}
+
+static void release_arenas(void)
+{
+    for (int d = 0; d < WS_MAX_DEV; d++) {
+        if (g_arena[d].base) {
+            int cur = 0;
+            hipGetDevice(&cur);
+            hipSetDevice(d);
+            hipDeviceSynchronize();
+            hipFree(g_arena[d].base);
+            hipSetDevice(cur);
+        }
+        g_arena[d].base = NULL;
+        g_arena[d].size = 0;
+    }
+}
+
+void cutc_release_workspace(void)
+{
+    pthread_mutex_lock(&g_lock);
+    release_arenas();
+    pthread_mutex_unlock(&g_lock);
+}
+
+#define WS_TAKE(ptr, bytes)                                                                                         \
+    do {                                                                                                             \
+        *(void **)&(ptr) = ws_take(&ws, (bytes));                                                                    \
+        if (!(ptr)) {                                                                                                \
+            rc = CUTC_ERR_NOMEM;                                                                                     \
+            goto done;                                                                                               \
+        }                                                                                                            \
+    } while (0)
 
 /* ------------------------------------------------------------ stage 1 */
 
@@ -227,16 +327,23 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     int rc = CUTC_OK, ev;
     (void)opt;
 
-    TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
-    TRYH(hipMalloc((void **)&Apk, sizeof(double) * P * ldk));
-    TRYH(hipMalloc((void **)&G, sizeof(double) * 4 * n1c * ldk));
+    ws_t ws;
+    {
+        const size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * P * ldk) +
+                            ws_round(sizeof(double) * 4 * n1c * ldk) +
+                            (n_nuc > 32 ? ws_round(sizeof(double) * n_nuc * ldk) : 0) +
+                            ws_round(sizeof(double) * (prog->ts * n1c + 1));
+        TRY(ws_begin(&ws, need));
+        if (stats)
+            stats->workspace_bytes = need;
+    }
+    WS_TAKE(pairs, sizeof(int) * 2 * P);
+    WS_TAKE(Apk, sizeof(double) * P * ldk);
+    WS_TAKE(G, sizeof(double) * 4 * n1c * ldk);
     if (n_nuc > 32)
-        TRYH(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * ldk));
-    TRYH(hipMalloc((void **)&tab, sizeof(double) * (prog->ts * n1c + 1)));
+        WS_TAKE(f2tab, sizeof(double) * n_nuc * ldk);
+    WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
     TRY(program_upload(prog, &dmono, &dcls));
-    if (stats)
-        stats->workspace_bytes = sizeof(int) * 2 * P + sizeof(double) * (P * ldk + 4 * n1c * ldk) +
-                                 (f2tab ? sizeof(double) * n_nuc * ldk : 0);
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_INTEG);
@@ -272,11 +379,6 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
 done:
     if (rc != CUTC_OK)
         hipStreamSynchronize(st);
-    dfree(pairs);
-    dfree(Apk);
-    dfree(G);
-    dfree(f2tab);
-    dfree(tab);
     dfree(dmono);
     dfree(dcls);
     return rc;
@@ -304,20 +406,28 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
     const size_t bA = (size_t)nmod * CUTC_CRT_ROWS(P) * ldk, bG = (size_t)nmod * CUTC_CRT_ROWS(4 * n1c) * ldk;
     const size_t bC = (size_t)nparts * nmod * 4 * n1c * P;
 
-    TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
-    TRYH(hipMalloc((void **)&rmaxA, sizeof(uint64_t) * P));
-    TRYH(hipMalloc((void **)&rmaxG, sizeof(uint64_t) * 4 * n1c));
-    TRYH(hipMalloc((void **)&Apl, bA));
-    TRYH(hipMalloc((void **)&Gpl, bG));
-    TRYH(hipMalloc((void **)&Cpl, bC));
-    TRYH(hipMalloc((void **)&G, sizeof(double) * 4 * n1c * ldk));
+    ws_t ws;
+    {
+        const size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(uint64_t) * P) +
+                            ws_round(sizeof(uint64_t) * 4 * n1c) + ws_round(bA) + ws_round(bG) + ws_round(bC) +
+                            ws_round(sizeof(double) * 4 * n1c * ldk) +
+                            (n_nuc > 32 ? ws_round(sizeof(double) * n_nuc * ldk) : 0) +
+                            ws_round(sizeof(double) * (prog->ts * n1c + 1));
+        TRY(ws_begin(&ws, need));
+        if (stats)
+            stats->workspace_bytes = need;
+    }
+    WS_TAKE(pairs, sizeof(int) * 2 * P);
+    WS_TAKE(rmaxA, sizeof(uint64_t) * P);
+    WS_TAKE(rmaxG, sizeof(uint64_t) * 4 * n1c);
+    WS_TAKE(Apl, bA);
+    WS_TAKE(Gpl, bG);
+    WS_TAKE(Cpl, bC);
+    WS_TAKE(G, sizeof(double) * 4 * n1c * ldk);
     if (n_nuc > 32)
-        TRYH(hipMalloc((void **)&f2tab, sizeof(double) * n_nuc * ldk));
-    TRYH(hipMalloc((void **)&tab, sizeof(double) * (prog->ts * n1c + 1)));
+        WS_TAKE(f2tab, sizeof(double) * n_nuc * ldk);
+    WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
     TRY(program_upload(prog, &dmono, &dcls));
-    if (stats)
-        stats->workspace_bytes = sizeof(int) * 2 * P + sizeof(uint64_t) * (P + 4 * n1c) + bA + bG + bC +
-                                 sizeof(double) * 4 * n1c * ldk + (f2tab ? sizeof(double) * n_nuc * ldk : 0);
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_SPLIT);
@@ -363,15 +473,6 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
 done:
     if (rc != CUTC_OK)
         hipStreamSynchronize(st);
-    dfree(pairs);
-    dfree(rmaxA);
-    dfree(rmaxG);
-    dfree(Apl);
-    dfree(Gpl);
-    dfree(Cpl);
-    dfree(G);
-    dfree(f2tab);
-    dfree(tab);
     dfree(dmono);
     dfree(dcls);
     return rc;
@@ -396,15 +497,27 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
     const int64_t rows = (n1c + 15) / 16 * 16;
     const int64_t plane = rows * P;
     const int64_t kp_max = (n1c + 127) / 128 * 128;
-    TRYH(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
-    TRYH(hipMalloc((void **)&Hp, sizeof(double) * plane));
-    TRYH(hipMalloc((void **)&NHp, sizeof(double) * 3 * plane));
+    ws_t ws;
+    const size_t bpl = (size_t)nmod * CUTC_CRT_ROWS(P) * 3 * kp_max;
+    const size_t bcp = (size_t)CUTC_CRT_PARTS(3 * kp_max) * nmod * P * P;
+    {
+        size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
+                      ws_round(sizeof(double) * 3 * plane);
+        if (crt_mode)
+            need += 2 * ws_round(sizeof(uint64_t) * P) + 2 * ws_round(bpl) + ws_round(bcp);
+        TRY(ws_begin(&ws, need));
+        if (stats && need > stats->workspace_bytes)
+            stats->workspace_bytes = need;
+    }
+    WS_TAKE(pairs, sizeof(int) * 2 * P);
+    WS_TAKE(Hp, sizeof(double) * plane);
+    WS_TAKE(NHp, sizeof(double) * 3 * plane);
     if (crt_mode) {
-        TRYH(hipMalloc((void **)&rmaxA, sizeof(uint64_t) * P));
-        TRYH(hipMalloc((void **)&rmaxB, sizeof(uint64_t) * P));
-        TRYH(hipMalloc((void **)&Apl, (size_t)nmod * CUTC_CRT_ROWS(P) * 3 * kp_max));
-        TRYH(hipMalloc((void **)&Bpl, (size_t)nmod * CUTC_CRT_ROWS(P) * 3 * kp_max));
-        TRYH(hipMalloc((void **)&Cpl, (size_t)CUTC_CRT_PARTS(3 * kp_max) * nmod * P * P));
+        WS_TAKE(rmaxA, sizeof(uint64_t) * P);
+        WS_TAKE(rmaxB, sizeof(uint64_t) * P);
+        WS_TAKE(Apl, bpl);
+        WS_TAKE(Bpl, bpl);
+        WS_TAKE(Cpl, bcp);
     }
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
 
@@ -479,14 +592,6 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
 done:
     if (rc != CUTC_OK)
         hipStreamSynchronize(st);
-    dfree(pairs);
-    dfree(Hp);
-    dfree(NHp);
-    dfree(rmaxA);
-    dfree(rmaxB);
-    dfree(Apl);
-    dfree(Bpl);
-    dfree(Cpl);
     return rc;
 }
 
@@ -613,8 +718,10 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
         cutc_default_options(&def);
         opt = &def;
     }
+    pthread_mutex_lock(&g_lock);
     int rc = device_path(opt, 3, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
                          c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats);
+    pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     return rc;
 }
@@ -627,8 +734,11 @@ void get_int2_grad1_u12_ao(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int n_grid1, i
     (void)dimBlock;
     cutc_options opt;
     cutc_default_options(&opt);
+    pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 1, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, NULL, r2, wr2, rn, NULL, aos_data2,
                          c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL);
+    release_arenas(); /* the reference keeps no device memory between calls */
+    pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     if (rc)
         fprintf(stderr, "get_int2_grad1_u12_ao: %s\n", cutc_error_string(rc));
@@ -639,8 +749,11 @@ void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double
 {
     cutc_options opt;
     cutc_default_options(&opt);
+    pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 2, n_grid1, 0, n_ao, 0, 0, NULL, wr1, NULL, NULL, NULL, aos_data1, NULL, NULL, NULL,
                          NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL);
+    release_arenas();
+    pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     if (rc)
         fprintf(stderr, "get_int_2e_ao: %s\n", cutc_error_string(rc));
@@ -775,8 +888,11 @@ int cutc_int(int nxBlocks, int nyBlocks, int nzBlocks, int blockxSize, int block
         cutc_options opt;
         cutc_default_options(&opt);
         opt.verbose = 1;
+        pthread_mutex_lock(&g_lock);
         rc = host_path(&opt, n_grid1, n_grid2, n_ao, n_nuc, size_bh, h_r1, h_wr1, h_r2, h_wr2, h_rn, h_aos_data1,
                        h_aos_data2, h_c_bh, h_m_bh, h_n_bh, h_o_bh, h_int2_grad1_u12_ao, h_int_2e_ao, NULL);
+        release_arenas(); /* the reference keeps no device memory between calls */
+        pthread_mutex_unlock(&g_lock);
     }
     cutc_set_last_error(rc);
     return rc;
@@ -794,9 +910,12 @@ int cutc_int_ex(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int
         opt = &def;
     }
     int rc = valid_precision(opt) ? CUTC_OK : CUTC_ERR_ARG;
-    if (rc == CUTC_OK)
+    if (rc == CUTC_OK) {
+        pthread_mutex_lock(&g_lock);
         rc = host_path(opt, n_grid1, n_grid2, n_ao, n_nuc, size_bh, h_r1, h_wr1, h_r2, h_wr2, h_rn, h_aos_data1,
                        h_aos_data2, h_c_bh, h_m_bh, h_n_bh, h_o_bh, h_int2_grad1_u12_ao, h_int_2e_ao, stats);
+        pthread_mutex_unlock(&g_lock);
+    }
     cutc_set_last_error(rc);
     return rc;
 }

@@ -323,6 +323,9 @@ __device__ static inline int reduce_mod(int x, int m, double inv_m) { return res
 // tail tiles read valid memory; their extra rows are never stored.
 // K: row length of the planes; [k0, k0 + kn) is this launch's K range
 // (kn <= 131072, so |sum| <= kn 2^14 < 2^31 stays exact in int32).
+// Operand fragments are double-buffered in registers: the LDS reads of K step
+// ks + 1 are issued before the 16 MFMAs of step ks, so the MFMA pipe never
+// waits on an LDS round trip inside a stage.
 __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const int8_t *__restrict__ A, int64_t M,
                                                           const int8_t *__restrict__ B, int64_t N, int64_t K,
                                                           int64_t k0, int64_t kn, int mod, double inv_mod,
@@ -336,16 +339,21 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const int8_t *__restri
     cutc_group_tile<4>(t, tiles_m, tiles_n, tm, tn);
     const int64_t row0 = (int64_t)tm * IG_TILE, col0 = (int64_t)tn * IG_TILE;
 
-    // loader: thread -> rows (tid / 8) + 32 i, 16-byte segment tid % 8
+    // loader: thread -> rows (tid / 8) + 32 i, 16-byte segment tid % 8;
+    // wave-uniform bases and 32-bit per-lane offsets
     const int seg = threadIdx.x & 7, lr = threadIdx.x >> 3;
-    const int8_t *pa = A + (row0 + lr) * K + k0 + 16 * seg;
-    const int8_t *pb = B + (col0 + lr) * K + k0 + 16 * seg;
-    const int64_t rstep = 32 * K;
+    const int8_t *Ab = A + row0 * K + k0;
+    const int8_t *Bb = B + col0 * K + k0;
+    const uint32_t off0 = (uint32_t)(lr * K + 16 * seg);
+    const uint32_t rstep = (uint32_t)(32 * K);
     v4i ra[8], rb[8];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave >> 1) * 128, wn = (wave & 1) * 128;
     const int li = lane & 31, lg = lane >> 5;
+    const int fa_off = (wm + li) * IG_PITCH + 16 * lg;
+    const int fb_off = (wn + li) * IG_PITCH + 16 * lg;
+    const int st_off = lr * IG_PITCH + 16 * seg;
 
     v16i acc[4][4];
 #pragma unroll
@@ -357,13 +365,13 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const int8_t *__restri
     const int64_t nk = kn / CRT_KSTEP;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        ra[i] = *(const v4i *)(pa + i * rstep);
-        rb[i] = *(const v4i *)(pb + i * rstep);
+        ra[i] = *(const v4i *)(Ab + off0 + i * rstep);
+        rb[i] = *(const v4i *)(Bb + off0 + i * rstep);
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        *(v4i *)&sA[0][(lr + 32 * i) * IG_PITCH + 16 * seg] = ra[i];
-        *(v4i *)&sB[0][(lr + 32 * i) * IG_PITCH + 16 * seg] = rb[i];
+        *(v4i *)&sA[0][st_off + 32 * i * IG_PITCH] = ra[i];
+        *(v4i *)&sB[0][st_off + 32 * i * IG_PITCH] = rb[i];
     }
     __syncthreads();
 
@@ -371,32 +379,45 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const int8_t *__restri
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            const int64_t off = (kt + 1) * CRT_KSTEP;
+            const uint32_t off = off0 + (uint32_t)((kt + 1) * CRT_KSTEP);
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                ra[i] = *(const v4i *)(pa + i * rstep + off);
-                rb[i] = *(const v4i *)(pb + i * rstep + off);
+                ra[i] = *(const v4i *)(Ab + off + i * rstep);
+                rb[i] = *(const v4i *)(Bb + off + i * rstep);
             }
+        }
+        const int8_t *la = &sA[cur][fa_off];
+        const int8_t *lb = &sB[cur][fb_off];
+        v4i fa[2][4], fb[2][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            fa[0][i] = *(const v4i *)(la + 32 * i * IG_PITCH);
+            fb[0][i] = *(const v4i *)(lb + 32 * i * IG_PITCH);
         }
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
-            v4i a[4], b[4];
+            const int cb = ks & 1;
+            if (ks < 3) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                a[i] = *(const v4i *)&sA[cur][(wm + 32 * i + li) * IG_PITCH + 32 * ks + 16 * lg];
-                b[i] = *(const v4i *)&sB[cur][(wn + 32 * i + li) * IG_PITCH + 32 * ks + 16 * lg];
+                for (int i = 0; i < 4; i++) {
+                    fa[cb ^ 1][i] = *(const v4i *)(la + 32 * i * IG_PITCH + 32 * (ks + 1));
+                    fb[cb ^ 1][i] = *(const v4i *)(lb + 32 * i * IG_PITCH + 32 * (ks + 1));
+                }
             }
+            // keep the next step's LDS reads ahead of this step's MFMAs
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cb][i], fb[cb][j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (more) {
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                *(v4i *)&sA[cur ^ 1][(lr + 32 * i) * IG_PITCH + 16 * seg] = ra[i];
-                *(v4i *)&sB[cur ^ 1][(lr + 32 * i) * IG_PITCH + 16 * seg] = rb[i];
+                *(v4i *)&sA[cur ^ 1][st_off + 32 * i * IG_PITCH] = ra[i];
+                *(v4i *)&sB[cur ^ 1][st_off + 32 * i * IG_PITCH] = rb[i];
             }
         }
         __syncthreads();

@@ -71,6 +71,7 @@ HEADER_SYMBOLS = [
     "cutc_int", "get_int2_grad1_u12_ao", "get_int_2e_ao", "checkCudaErrors", "checkCublasErrors", "cutc_int_bh",
     "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
     "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
+    "cutc_release_workspace",
 ]
 
 _vp = ctypes.c_void_p
@@ -109,6 +110,8 @@ def lib():
         L.cutc_error_string.argtypes = [_ci]
         L.cutc_default_options.restype = None
         L.cutc_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.cutc_release_workspace.restype = None
+        L.cutc_release_workspace.argtypes = []
         L.cutc_make_plan.restype = _ci
         L.cutc_make_plan.argtypes = [ctypes.c_int64, ctypes.c_int64, _ci, _ci, _ci, _ci, ctypes.c_size_t,
                                      ctypes.c_int64, ctypes.POINTER(Plan)]
@@ -251,6 +254,11 @@ def cutc_int_device(ds, X, I, opt: Options | None = None, stream=None):
                                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
     _check(rc, "cutc_int_device")
     return st.as_dict()
+
+
+def release_workspace():
+    """Free the device workspace cached by cutc_int_device / cutc_int_ex."""
+    lib().cutc_release_workspace()
 
 
 def get_int2_grad1_u12_ao(ds, X, grid=(1, 1, 1), block=(32, 1, 1)):

@@ -147,6 +147,13 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
                     const int *n_bh, const int *o_bh, double *int2_grad1_u12_ao, double *int_2e_ao, void *stream,
                     cutc_stats *stats);
 
+/* cutc_int_device and cutc_int_ex keep their device workspace (one arena
+ * per device, grown on demand) between calls, so repeated builds allocate
+ * nothing; this frees it.  cutc_int and the get_* functions release it
+ * before returning, as the reference frees its buffers.  Calls are
+ * serialised process-wide (they share the arenas). */
+void cutc_release_workspace(void);
+
 /* cutc_int with options and statistics (host arrays, like cutc_int). */
 int cutc_int_ex(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
                 const double *h_r1, const double *h_wr1, const double *h_r2, const double *h_wr2, const double *h_rn,
