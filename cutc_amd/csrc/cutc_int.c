@@ -402,9 +402,8 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
     int rc = CUTC_OK, ev;
     cutc_crt crt;
     TRY(cutc_crt_setup(n2, nmod, &crt));
-    const int nparts = CUTC_CRT_PARTS(ldk);
-    const size_t bA = (size_t)nmod * CUTC_CRT_ROWS(P) * ldk, bG = (size_t)nmod * CUTC_CRT_ROWS(4 * n1c) * ldk;
-    const size_t bC = (size_t)nparts * nmod * 4 * n1c * P;
+    const size_t bA = (size_t)nmod * CUTC_PK_ROWS_B(P) * ldk, bG = (size_t)nmod * CUTC_PK_ROWS_A(4 * n1c) * ldk;
+    const size_t bC = (size_t)nmod * 4 * n1c * P;
 
     ws_t ws;
     {
@@ -433,7 +432,7 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
     ev = tr_begin(tr, CAT_SPLIT);
     rc = cutc_crt_rowmax_pairw(wr2, aos2, n2, n_ao, rmaxA, st);
     if (rc == CUTC_OK)
-        rc = cutc_crt_quant_pairw(&crt, wr2, aos2, n2, n_ao, ldk, rmaxA, Apl, st);
+        rc = cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rmaxA, Apl, st);
     tr_end(tr, ev);
     TRY(rc);
 
@@ -451,15 +450,15 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
         ev = tr_begin(tr, CAT_SPLIT);
         rc = cutc_crt_rowmax_k(G, ldk, 4 * n1e, n2, rmaxG, st);
         if (rc == CUTC_OK)
-            rc = cutc_crt_quant_k(&crt, G, ldk, 4 * n1e, n2, ldk, rmaxG, Gpl, st);
+            rc = cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rmaxG, Gpl, st);
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_GEMM1);
-        rc = cutc_crt_gemm(&crt, Apl, P, Gpl, 4 * n1e, ldk, Cpl, st);
+        rc = cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, ldk, Cpl, st);
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
-        rc = cutc_crt_rebuild_x(&crt, Cpl, nparts, P, 4 * n1e, rmaxA, rmaxG, pairs, n_ao, X, n1, n1e, i0, st);
+        rc = cutc_crt_rebuild_x(&crt, Cpl, 1, P, 4 * n1e, rmaxA, rmaxG, pairs, n_ao, X, n1, n1e, i0, st);
         tr_end(tr, ev);
         TRY(rc);
         if (stats)

@@ -322,7 +322,7 @@ __device__ static inline int reduce_mod(int x, int m, double inv_m) { return res
 // A, B: planes with rows padded to a multiple of 256 (cutc_crt_rows), so the
 // tail tiles read valid memory; their extra rows are never stored.
 // K: row length of the planes; [k0, k0 + kn) is this launch's K range
-// (kn <= 131072, so |sum| <= kn 2^14 < 2^31 stays exact in int32).
+// (kn <= CUTC_CRT_KPART, so |sum| <= kn 2^14 < 2^31 stays exact in int32).
 // Operand fragments are double-buffered in registers: the LDS reads of K step
 // ks + 1 are issued before the 16 MFMAs of step ks, so the MFMA pipe never
 // waits on an LDS round trip inside a stage.
@@ -463,6 +463,282 @@ extern "C" int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, cons
                                C + ((int64_t)part * c->n + i) * M * N, (int)tm, (int)tn);
         }
     }
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
+// ------------------------------------------------ fragment-packed stage 1
+
+// Packed byte offset of (row, k) for rows of row block rb: the layout of the
+// residue planes fed to crt_gemm_pk_kernel (see launch.h).
+__device__ static inline int64_t pk_offset(int64_t rb, int r, int64_t k, int64_t KT)
+{
+    const int64_t kt = k >> 6;
+    const int kk = (int)(k & 63);
+    const int h = kk >> 5, c = (kk >> 4) & 1;
+    return (rb * KT + kt) * 2048 + c * 1024 + (32 * h + r) * 16;
+}
+
+// grid (kpad / 128, ceil(rows / 32)): a block covers one row block and two K
+// steps; thread r = tid % 32 takes row 32 rb + r, 16 bytes j = tid / 32, so
+// the 16-byte stores of consecutive threads are consecutive.
+__global__ __launch_bounds__(256) void quant_k_pk_kernel(cutc_crt c, const double *__restrict__ X, int64_t ld,
+                                                         int64_t rows, int64_t K, int64_t KT, int64_t plane,
+                                                         const uint64_t *__restrict__ rmax, int8_t *planes)
+{
+    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const int64_t rb = blockIdx.y, row = rb * 32 + r;
+    const int64_t k0 = (int64_t)blockIdx.x * 128 + 16 * j;
+    double q[16];
+    if (row < rows) {
+        const int sh = c.bits - row_exponent(rmax[row]);
+        const double *src = X + row * ld;
+#pragma unroll
+        for (int e = 0; e < 16; e++)
+            q[e] = (k0 + e < K) ? rint(ldexp(src[k0 + e], sh)) : 0.0;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; e++)
+            q[e] = 0.0;
+    }
+    store_residues(c, q, planes + pk_offset(rb, r, k0, KT), plane);
+}
+
+extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K,
+                                   int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st)
+{
+    if (rows <= 0 || kpad <= 0)
+        return 0;
+    if (kpad % CUTC_PK_KALIGN || (rows + 31) / 32 > 65535)
+        return CUTC_ERR_ARG;
+    const int64_t plane = CUTC_PK_ROWS_A(rows) * kpad;
+    hipLaunchKernelGGL(quant_k_pk_kernel, dim3((unsigned)(kpad / 128), (unsigned)((rows + 31) / 32)), dim3(256), 0,
+                       st, *c, X, ld, rows, K, kpad / 64, plane, rmax, planes);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
+// pair rows p = (k, l) of A[p][r2] = (w chi_l) chi_k, computed on the fly
+__global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const double *__restrict__ wr2,
+                                                             const double *__restrict__ aos2, int64_t n2, int n_ao,
+                                                             int64_t KT, int64_t plane,
+                                                             const uint64_t *__restrict__ rmax, int8_t *planes)
+{
+    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const int64_t rb = blockIdx.y, p = rb * 32 + r;
+    const int64_t k0 = (int64_t)blockIdx.x * 128 + 16 * j;
+    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    double q[16];
+    if (p < P) {
+        int l = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+        while ((int64_t)(l + 1) * (l + 2) / 2 <= p)
+            l++;
+        while ((int64_t)l * (l + 1) / 2 > p)
+            l--;
+        const int k = (int)(p - (int64_t)l * (l + 1) / 2);
+        const int sh = c.bits - row_exponent(rmax[p]);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int64_t i = k0 + e;
+            q[e] = (i < n2) ? rint(ldexp(wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k], sh)) : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; e++)
+            q[e] = 0.0;
+    }
+    store_residues(c, q, planes + pk_offset(rb, r, k0, KT), plane);
+}
+
+extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2,
+                                       int n_ao, int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st)
+{
+    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    if (P <= 0 || kpad <= 0)
+        return 0;
+    if (kpad % CUTC_PK_KALIGN || (P + 31) / 32 > 65535)
+        return CUTC_ERR_ARG;
+    const int64_t plane = CUTC_PK_ROWS_B(P) * kpad;
+    hipLaunchKernelGGL(quant_pairw_pk_kernel, dim3((unsigned)(kpad / 128), (unsigned)((P + 31) / 32)), dim3(256), 0,
+                       st, *c, wr2, aos2, n2, n_ao, kpad / 64, plane, rmax, planes);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
+// x - m rint(x / m) in float: keeps x mod m with |result| < 3 m for
+// |x| < 2^31 (the float quotient is within 2 of the exact one)
+__device__ static inline int shrink_mod(int x, int m, float inv_m)
+{
+    const int q = (int)__builtin_rintf((float)x * inv_m);
+    return x - q * m;
+}
+
+typedef __attribute__((address_space(3))) int8_t lds_i8;
+#define PK_TILE 2048 // bytes of one (32-row block, 64-byte K step) tile
+#define PK_DEPTH 4   // K steps in flight (register ring for A, LDS ring for B)
+#define PK_SEG 2044  // K steps between reductions: 2044 * 64 * 2^14 < 2^31
+
+// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
+#define PK_WAIT_VM(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8)))
+
+// Block tile 512 A rows x 128 B rows, 4 waves of 128 x 128 (4 x 4 MFMA
+// 32x32x32 i8, 256 accumulator registers).  Each wave streams its own A rows
+// straight from the packed planes into a 4-deep register ring (one coalesced
+// 1 KB load per fragment); the block's B rows, shared by the 4 waves, go
+// global -> LDS with global_load_lds_dwordx4 into a 4-stage ring.  Per K step
+// (64 bytes, 32 MFMAs per wave) every wave issues 2 LDS-DMA loads for step
+// k + 3, then 8 A loads for step k + 3; the barrier of step k follows the
+// wait for the DMA of step k + 1, whose c = 0 fragments are read during step
+// k.  Loads past the last step repeat the last step (valid memory, unused).
+__global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__restrict__ A,
+                                                             const int8_t *__restrict__ B, int64_t KT, int tiles_a,
+                                                             int tiles_b, int64_t M, int64_t N, int mod,
+                                                             double inv_mod, int8_t *__restrict__ C)
+{
+    // the 4 B stages of 8 KB (4 row blocks x 2 KB)
+    __shared__ __attribute__((aligned(16))) int8_t sB0[4 * PK_TILE];
+    __shared__ __attribute__((aligned(16))) int8_t sB1[4 * PK_TILE];
+    __shared__ __attribute__((aligned(16))) int8_t sB2[4 * PK_TILE];
+    __shared__ __attribute__((aligned(16))) int8_t sB3[4 * PK_TILE];
+#define PK_SB(u) ((u) == 0 ? sB0 : (u) == 1 ? sB1 : (u) == 2 ? sB2 : sB3)
+
+    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    int ta, tb;
+    cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t rbs = KT * PK_TILE; // bytes per row block
+    const int8_t *Aw = A + ((int64_t)ta * 16 + wave * 4) * rbs;
+    const int8_t *Bw = B + ((int64_t)tb * 4 + wave) * rbs;
+    const uint32_t lo = (uint32_t)lane * 16;
+    const float inv_modf = (float)inv_mod;
+
+    v4i ar[PK_DEPTH][4][2];
+    v4i bf0[4], bf1[4];
+    v16i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[i][j] = v16i{};
+
+// global -> LDS copy of 16 bytes per lane (lane i lands at M0 + 16 i).
+// Written as asm so that the compiler's wait insertion, which cannot tell
+// the four stages apart and would wait for every outstanding load, leaves
+// the DMA alone; its completion is awaited explicitly (PK_WAIT_VM).  The
+// compiler's own waits for the A registers then only over-count (the DMA
+// loads it does not see are younger), which is safe.  m0 has no other use in
+// this kernel.
+#define PK_DMA(gptr, lptr)                                                                                            \
+    do {                                                                                                              \
+        const unsigned la_ = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_i8 *)(lptr));                   \
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la_), "v"(gptr)          \
+                     : "memory");                                                                                     \
+    } while (0)
+#define PK_ISSUE(slot, s)                                                                                             \
+    do {                                                                                                              \
+        const uint32_t so = (uint32_t)(s) * PK_TILE;                                                                  \
+        PK_DMA(Bw + so + lo, &PK_SB(slot)[wave * PK_TILE]);                                                           \
+        PK_DMA(Bw + so + 1024 + lo, &PK_SB(slot)[wave * PK_TILE + 1024]);                                             \
+        __builtin_amdgcn_sched_barrier(0);                                                                            \
+        _Pragma("unroll") for (int i = 0; i < 4; i++)                                                                 \
+        {                                                                                                             \
+            ar[slot][i][0] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + lo));                           \
+            ar[slot][i][1] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + 1024 + lo));                    \
+        }                                                                                                             \
+        __builtin_amdgcn_sched_barrier(0);                                                                            \
+    } while (0)
+
+    // K runs in segments of at most PK_SEG steps, each a full pipeline
+    // (prologue, steady state, drain), so that the int32 sums can be shrunk
+    // modulo m between segments while no fragment registers are live.
+    for (int64_t k0 = 0; k0 < KT; k0 += PK_SEG) {
+        const int64_t k1 = (KT - k0 < PK_SEG) ? KT : k0 + PK_SEG;
+        // prologue: steps k0 .. k0 + 2 in flight; DMA of the first two complete
+        PK_ISSUE(0, k0);
+        PK_ISSUE(1, (k0 + 1 < k1 ? k0 + 1 : k1 - 1));
+        PK_ISSUE(2, (k0 + 2 < k1 ? k0 + 2 : k1 - 1));
+        PK_WAIT_VM(18);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            bf0[j] = *(const v4i *)&sB0[j * PK_TILE + lo];
+
+#define PK_STEP(u)                                                                                                    \
+    do {                                                                                                              \
+        const int64_t k = kb + (u);                                                                                   \
+        PK_WAIT_VM(18);                                                                                               \
+        __syncthreads();                                                                                              \
+        _Pragma("unroll") for (int j = 0; j < 4; j++) bf1[j] = *(const v4i *)&PK_SB(u)[j * PK_TILE + 1024 + lo];          \
+        PK_ISSUE(((u) + 3) & 3, (k + 3 < k1 ? k + 3 : k1 - 1));                                                       \
+        _Pragma("unroll") for (int i = 0; i < 4; i++) _Pragma("unroll") for (int j = 0; j < 4; j++) acc[i][j] =      \
+            __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][0], bf0[j], acc[i][j], 0, 0, 0);                           \
+        __builtin_amdgcn_sched_barrier(0);                                                                            \
+        _Pragma("unroll") for (int j = 0; j < 4; j++) bf0[j] = *(const v4i *)&PK_SB(((u) + 1) & 3)[j * PK_TILE + lo];    \
+        _Pragma("unroll") for (int i = 0; i < 4; i++) _Pragma("unroll") for (int j = 0; j < 4; j++) acc[i][j] =      \
+            __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][1], bf1[j], acc[i][j], 0, 0, 0);                           \
+        __builtin_amdgcn_sched_barrier(0);                                                                            \
+    } while (0)
+
+        for (int64_t kb = k0; kb < k1; kb += PK_DEPTH) {
+            PK_STEP(0);
+            PK_STEP(1);
+            PK_STEP(2);
+            PK_STEP(3);
+        }
+        // drain: the repeated loads of the last steps land before the next
+        // segment's prologue reuses their LDS stages
+        PK_WAIT_VM(0);
+        __syncthreads();
+        if (k1 < KT) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int r = 0; r < 16; r++)
+                        acc[i][j][r] = shrink_mod(acc[i][j][r], mod, inv_modf);
+        }
+    }
+#undef PK_STEP
+#undef PK_ISSUE
+#undef PK_DMA
+#undef PK_SB
+
+    // result element r of lane l: A row 8 (r/4) + 4 (l/32) + r%4, B row l%32
+    const int li = lane & 31, lg = lane >> 5;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t col = (int64_t)tb * 128 + 32 * j + li;
+            if (col >= N)
+                continue;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int64_t row = (int64_t)ta * 512 + wave * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
+                if (row < M)
+                    C[row * N + col] = (int8_t)reduce_mod(acc[i][j][r], mod, inv_mod);
+            }
+        }
+    }
+}
+
+extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
+                                int64_t kpad, int8_t *C, hipStream_t st)
+{
+    if (M <= 0 || N <= 0)
+        return 0;
+    if (kpad <= 0 || kpad % CUTC_PK_KALIGN)
+        return CUTC_ERR_ARG;
+    const int64_t KT = kpad / 64;
+    if (4 * KT * PK_TILE > 4294967295LL) /* 32-bit offsets inside a wave's four row blocks */
+        return CUTC_ERR_ARG;
+    const int64_t ta = CUTC_PK_ROWS_A(M) / 512, tb = CUTC_PK_ROWS_B(N) / 128;
+    if (ta * tb > 2147483647)
+        return CUTC_ERR_ARG;
+    const int64_t pa = CUTC_PK_ROWS_A(M) * kpad, pb = CUTC_PK_ROWS_B(N) * kpad;
+    for (int i = 0; i < c->n; i++)
+        hipLaunchKernelGGL(crt_gemm_pk_kernel, dim3((unsigned)(ta * tb)), dim3(256), 0, st, A + i * pa, B + i * pb, KT,
+                           (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], C + (int64_t)i * M * N);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 

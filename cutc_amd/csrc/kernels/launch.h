@@ -118,14 +118,37 @@ int cutc_crt_quant_pairw(const cutc_crt *c, const double *wr2, const double *aos
 int cutc_crt_rowmax_colk(const cutc_colsrc *src, int64_t rows, uint64_t *rmax, hipStream_t st);
 int cutc_crt_quant_colk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kpad,
                         const uint64_t *rmax, int8_t *planes, hipStream_t st);
-/* K parts of at most CUTC_CRT_KPART columns keep the int32 sums exact. */
-#define CUTC_CRT_KPART 131072
+/* K parts of at most CUTC_CRT_KPART columns keep the int32 sums exact:
+ * |sum| <= 130944 * 128^2 < 2^31. */
+#define CUTC_CRT_KPART 130944
 #define CUTC_CRT_PARTS(K) ((int)((((int64_t)(K)) + CUTC_CRT_KPART - 1) / CUTC_CRT_KPART))
 /* C[part][i][j][p] = (sum_{k in part} A[i][p][k] B[i][j][k]) mod m_i as int8
  * in [-128, 127]; A: M rows, B: N rows (planes with CUTC_CRT_ROWS rows), both
  * with Ktot (multiple of 128) columns.  C holds CUTC_CRT_PARTS(Ktot) n M N bytes. */
 int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t Ktot,
                   int8_t *C, hipStream_t st);
+/* Fragment-packed residue planes for the stage-1 contraction.  Rows are
+ * grouped in blocks of 32 and K in steps of 64 bytes; the 2048-byte tile of
+ * (row block rb, step kt) sits at (rb * KT + kt) * 2048 (KT = kpad / 64) and
+ * holds byte k = 64 kt + 32 h + 16 c + e of row 32 rb + r at
+ * c * 1024 + (32 h + r) * 16 + e: the 16 bytes lane 32 h + r of a wave feeds
+ * to the c-th v_mfma_i32_32x32x32_i8 of the step, and every 1 KB half-tile is
+ * one fully coalesced wave load.  Both operands use the same K order, so the
+ * dot products are unchanged.  kpad is a multiple of CUTC_PK_KALIGN; the
+ * G-side planes (A) hold CUTC_PK_ROWS_A(rows) rows, the pair-side planes (B)
+ * CUTC_PK_ROWS_B(rows); rows past the valid ones are never stored. */
+#define CUTC_PK_KALIGN 256
+#define CUTC_PK_ROWS_A(rows) ((((int64_t)(rows)) + 511) / 512 * 512)
+#define CUTC_PK_ROWS_B(rows) ((((int64_t)(rows)) + 127) / 128 * 128)
+int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
+                        const uint64_t *rmax, int8_t *planes, hipStream_t st);
+int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
+                            int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st);
+/* C[i][r][p] = (sum_k A[i][r][k] B[i][p][k]) mod m_i as int8 in [-128, 127]
+ * for r < M, p < N, packed planes as above; C holds n M N bytes (row r of the
+ * A side is contiguous over p). */
+int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
+                     int8_t *C, hipStream_t st);
 /* Rebuild C = V 2^(eA + eB - 2 bits) and scatter into X (4, n1, n_ao, n_ao):
  * column j = m n1c + i1 -> X[m][i0 + i1][(k,l) and (l,k)]. */
 int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, int64_t N, const uint64_t *rmaxA,

@@ -6,9 +6,10 @@
  * points stay in one K sweep, and only r1 is chunked:
  *   FP64 stage 1: pairs table + Apk (n_pairs x ldk) + G chunk (4 x n1c x ldk)
  *                 [+ f2 table for > 32 nuclei]
- *   INT8 stage 1: pairs + row maxima + A residues (nmod x n_pairs x ldk bytes)
- *                 + per r1 row: G (4 ldk doubles), its residues (4 nmod ldk
- *                 bytes) and the GEMM residues (parts x nmod x 4 n_pairs bytes)
+ *   INT8 stage 1: pairs + row maxima + pair residues (nmod x n_pairs x ldk
+ *                 bytes, fragment-packed) + per r1 row: G (4 ldk doubles), its
+ *                 residues (4 nmod ldk bytes) and the GEMM residues (nmod x 4
+ *                 n_pairs bytes)
  *   stage 2:      Hp and NHp for an r1 chunk (4 planes x roundup(n1c2,16) x
  *                 n_pairs doubles) [INT8: + both operands' residues,
  *                 2 x nmod x 3 x n_pairs bytes per r1 row, + the GEMM residues
@@ -41,16 +42,15 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     const int crt = precision == CUTC_PREC_INT8;
     const int64_t nmod = crt ? slices : 0;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
-    const int64_t ldk = rup(n2, crt ? 128 : 64);
+    const int64_t ldk = rup(n2, crt ? CUTC_PK_KALIGN : 64);
     pl->n_pairs = P;
     pl->ldk = ldk;
 
     /* stage 1 */
     int64_t fixed1, per_r1;
     if (crt) {
-        const int64_t parts = CUTC_CRT_PARTS(ldk);
-        fixed1 = 16 * P + nmod * CUTC_CRT_ROWS(P) * ldk + nmod * 255 * ldk + 4096;
-        per_r1 = 32 * ldk + 32 + 4 * nmod * ldk + parts * nmod * 4 * P;
+        fixed1 = 16 * P + nmod * CUTC_PK_ROWS_B(P) * ldk + nmod * 511 * ldk + 4096;
+        per_r1 = 32 * ldk + 32 + 4 * nmod * ldk + nmod * 4 * P;
     } else {
         fixed1 = 8 * P + 8 * P * ldk + 4096;
         per_r1 = 32 * ldk;
@@ -64,7 +64,8 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
         int64_t cmax = ((int64_t)budget - fixed1) / per_r1;
         if (n1c_req > 0 && n1c_req < cmax)
             cmax = n1c_req;
-        n1c = balanced(n1, cmax, cmax >= 64 ? 64 : 1);
+        const int64_t align = crt ? 128 : 64; /* int8: 4 n1c fills whole 512-row tiles */
+        n1c = balanced(n1, cmax, cmax >= align ? align : 1);
     } else if (n1 > 0) {
         n1c = n1;
     }

@@ -1,6 +1,8 @@
 """Split-precision (modular int8 / CRT) contractions against the FP64 oracle
 and the FP64 GPU path.  Bar: relative Frobenius error <= 1e-10 on
 int2_grad1_u12_ao and int_2e_ao with the default 12 moduli."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -84,3 +86,61 @@ def test_int8_cube20_full_basis_subset(torch):
     ei, eif = relerr(I, Io), relerr(I, If)
     print(f"cube20 r1<24 int8: relerr int_2e_ao vs oracle {ei:.3e} vs FP64 GPU {eif:.3e}", flush=True)
     assert ei <= TOL
+
+
+# ------------------------------------------------------------------ packed GEMM
+
+class Crt(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("bits", ctypes.c_int), ("m", ctypes.c_int * 16), ("y", ctypes.c_int * 16),
+                ("inv_m", ctypes.c_double * 16), ("Mi", (ctypes.c_double * 3) * 16), ("M", ctypes.c_double * 3),
+                ("Minv", ctypes.c_double)]
+
+
+def pack(X, rows_pad):
+    """Fragment-packed layout of the int8 matrix X (rows, kpad): the 2048-byte
+    tile (row block rb, K step kt) at (rb KT + kt) 2048 holds byte
+    k = 64 kt + 32 h + 16 c + e of row 32 rb + r at c 1024 + (32 h + r) 16 + e."""
+    rows, kpad = X.shape
+    Xp = np.zeros((rows_pad, kpad), np.int8)
+    Xp[:rows] = X
+    RB, KT = rows_pad // 32, kpad // 64
+    t = Xp.reshape(RB, 32, KT, 2, 2, 16)  # rb, r, kt, h, c, e
+    return np.ascontiguousarray(t.transpose(0, 2, 4, 3, 1, 5)).reshape(-1)
+
+
+@pytest.mark.parametrize("M,N,K", [(37, 5, 100), (1100, 300, 5000), (512, 130, 131328)])
+def test_gemm_pk_matches_exact_products(torch, M, N, K):
+    """cutc_crt_gemm_pk against float64 products (exact: |sum| < 2^53) mod m,
+    with rows of -128 (the largest |product|) at K beyond one int32 segment."""
+    print(f"gemm_pk M={M} N={N} K={K}", flush=True)
+    L = cutc.lib()
+    L.cutc_crt_setup.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(Crt)]
+    L.cutc_crt_gemm_pk.argtypes = [ctypes.POINTER(Crt), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    c = Crt()
+    assert L.cutc_crt_setup(K, 6, ctypes.byref(c)) == 0
+    n = 2  # moduli 256 and 255
+    c.n = n
+    kpad = (K + 255) // 256 * 256
+    rng = np.random.default_rng(K)
+    ra = rng.integers(-128, 128, (n, M, kpad)).astype(np.int8)
+    rb = rng.integers(-128, 128, (n, N, kpad)).astype(np.int8)
+    ra[:, :, K:] = 0
+    rb[:, :, K:] = 0
+    ra[0, 0, :K] = -128
+    rb[0, 0, :K] = -128
+    ma, nb = (M + 511) // 512 * 512, (N + 127) // 128 * 128
+    A = torch.from_numpy(np.concatenate([pack(ra[i], ma) for i in range(n)])).cuda()
+    B = torch.from_numpy(np.concatenate([pack(rb[i], nb) for i in range(n)])).cuda()
+    C = torch.zeros(n * M * N, dtype=torch.int8, device="cuda")
+    torch.cuda.synchronize()
+    rc = L.cutc_crt_gemm_pk(ctypes.byref(c), A.data_ptr(), M, B.data_ptr(), N, kpad, C.data_ptr(), None)
+    torch.cuda.synchronize()
+    assert rc == 0
+    got = C.cpu().numpy().reshape(n, M, N).astype(np.int64)
+    for i in range(n):
+        m = c.m[i]
+        exact = (ra[i].astype(np.float64) @ rb[i].astype(np.float64).T).astype(np.int64)
+        want = exact % m
+        assert np.array_equal(got[i] % m, want), f"modulus {m}"
+        assert got[i].min() >= -128 and got[i].max() <= 127
