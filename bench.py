@@ -134,6 +134,7 @@ def main():
         log(args, f"fp64 build: {ms / 1e3:.2f} s  {st[-1]}")
         fp64 = {"build_s": ms / 1e3, "eff_tflops": W / (ms / 1e3) / 1e12, "stats": st[-1]}
         if args.precision != "fp64":
+            cutc.release_workspace()  # the device workspace of one precision is not reused by the other
             Iref = I.clone()
             Xref_host = X.cpu()
             dd_ref = dd(Iref)
@@ -150,6 +151,7 @@ def main():
         ms, sts = timed_builds(cutc, torch, ds, X, I, opt, args.steps, stream)
         last = sts[-1]
         log(args, f"{prec_label} build: {ms / 1e3:.2f} s  {last}")
+    cutc.release_workspace()  # room for the error analysis below
 
     # roofline of the dominant kernel (the step-1 contraction)
     exec1 = 2.0 * P * 4 * s.n1 * s.n2

@@ -572,33 +572,48 @@ __device__ static inline int shrink_mod(int x, int m, float inv_m)
 
 typedef __attribute__((address_space(3))) int8_t lds_i8;
 #define PK_TILE 2048 // bytes of one (32-row block, 64-byte K step) tile
-#define PK_DEPTH 4   // K steps in flight (register ring for A, LDS ring for B)
-#define PK_SEG 2044  // K steps between reductions: 2044 * 64 * 2^14 < 2^31
 
 // s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt in bits 3:0 and
 // 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
 #define PK_WAIT_VM(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8)))
 
+// global -> LDS copy of 16 bytes per lane (lane i lands at M0 + 16 i).
+// Written as asm so that the compiler's wait insertion, which cannot tell
+// the LDS stages apart and would wait for every outstanding load, leaves the
+// DMA alone; its completion is awaited explicitly (PK_WAIT_VM).  The
+// compiler's own waits for the A registers then only over-count (the DMA
+// loads it does not see are younger), which is safe.  m0 has no other use in
+// this kernel.
+__device__ static inline void pk_dma16(const int8_t *g, int8_t *l)
+{
+    const unsigned la = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_i8 *)l);
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(g) : "memory");
+}
+
 // Block tile 512 A rows x 128 B rows, 4 waves of 128 x 128 (4 x 4 MFMA
 // 32x32x32 i8, 256 accumulator registers).  Each wave streams its own A rows
-// straight from the packed planes into a 4-deep register ring (one coalesced
+// straight from the packed planes into a D-slot register ring (one coalesced
 // 1 KB load per fragment); the block's B rows, shared by the 4 waves, go
-// global -> LDS with global_load_lds_dwordx4 into a 4-stage ring.  Per K step
-// (64 bytes, 32 MFMAs per wave) every wave issues 2 LDS-DMA loads for step
-// k + 3, then 8 A loads for step k + 3; the barrier of step k follows the
-// wait for the DMA of step k + 1, whose c = 0 fragments are read during step
-// k.  Loads past the last step repeat the last step (valid memory, unused).
+// global -> LDS (pk_dma16) into a D-stage ring.  During K step k (64 bytes,
+// 32 MFMAs per wave) every wave issues the 2 DMA and 8 A loads of step k + L
+// and reads the c = 0 fragments of step k + 1.  A barrier opens every BAR-th
+// step, after the wait for the DMA of the next BAR steps.  The stage written
+// in step k belongs to step k + L - D, which every wave has finished once the
+// last barrier (at k - BAR + 1 or later) has passed: D >= L + BAR.  Loads
+// past the last step repeat the last step (valid memory, unused).  KT is a
+// multiple of D; K runs in segments of SEG steps (a multiple of D,
+// SEG 64 2^14 < 2^31), each a full pipeline (prologue, steady state, drain),
+// so that the int32 sums can be shrunk modulo m between segments while no
+// fragment registers are live.
+template <int D, int L, int BAR, int SEG>
 __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__restrict__ A,
                                                              const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                              int tiles_b, int64_t M, int64_t N, int mod,
                                                              double inv_mod, int8_t *__restrict__ C)
 {
-    // the 4 B stages of 8 KB (4 row blocks x 2 KB)
-    __shared__ __attribute__((aligned(16))) int8_t sB0[4 * PK_TILE];
-    __shared__ __attribute__((aligned(16))) int8_t sB1[4 * PK_TILE];
-    __shared__ __attribute__((aligned(16))) int8_t sB2[4 * PK_TILE];
-    __shared__ __attribute__((aligned(16))) int8_t sB3[4 * PK_TILE];
-#define PK_SB(u) ((u) == 0 ? sB0 : (u) == 1 ? sB1 : (u) == 2 ? sB2 : sB3)
+    static_assert(SEG % D == 0 && D % BAR == 0 && (int64_t)SEG * 64 * 16384 < 2147483648LL, "segment");
+    static_assert(D >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
+    __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * PK_TILE];
 
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
     int ta, tb;
@@ -610,8 +625,13 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     const int8_t *Bw = B + ((int64_t)tb * 4 + wave) * rbs;
     const uint32_t lo = (uint32_t)lane * 16;
     const float inv_modf = (float)inv_mod;
+    // at a barrier step b the DMA of step b + BAR (issued during step
+    // b + BAR - L, loads in issue order: 2 DMA then 8 A per step) is complete
+    // once no more than its own 8 A loads and the 10 loads of each of the
+    // L - BAR - 1 steps in between remain
+    constexpr int WAIT_NEXT = 8 + 10 * (L - BAR - 1);
 
-    v4i ar[PK_DEPTH][4][2];
+    v4i ar[D][4][2];
     v4i bf0[4], bf1[4];
     v16i acc[4][4];
 #pragma unroll
@@ -620,70 +640,74 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
         for (int j = 0; j < 4; j++)
             acc[i][j] = v16i{};
 
-// global -> LDS copy of 16 bytes per lane (lane i lands at M0 + 16 i).
-// Written as asm so that the compiler's wait insertion, which cannot tell
-// the four stages apart and would wait for every outstanding load, leaves
-// the DMA alone; its completion is awaited explicitly (PK_WAIT_VM).  The
-// compiler's own waits for the A registers then only over-count (the DMA
-// loads it does not see are younger), which is safe.  m0 has no other use in
-// this kernel.
-#define PK_DMA(gptr, lptr)                                                                                            \
-    do {                                                                                                              \
-        const unsigned la_ = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_i8 *)(lptr));                   \
-        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la_), "v"(gptr)          \
-                     : "memory");                                                                                     \
-    } while (0)
-#define PK_ISSUE(slot, s)                                                                                             \
-    do {                                                                                                              \
-        const uint32_t so = (uint32_t)(s) * PK_TILE;                                                                  \
-        PK_DMA(Bw + so + lo, &PK_SB(slot)[wave * PK_TILE]);                                                           \
-        PK_DMA(Bw + so + 1024 + lo, &PK_SB(slot)[wave * PK_TILE + 1024]);                                             \
-        __builtin_amdgcn_sched_barrier(0);                                                                            \
-        _Pragma("unroll") for (int i = 0; i < 4; i++)                                                                 \
-        {                                                                                                             \
-            ar[slot][i][0] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + lo));                           \
-            ar[slot][i][1] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + 1024 + lo));                    \
-        }                                                                                                             \
-        __builtin_amdgcn_sched_barrier(0);                                                                            \
-    } while (0)
+    auto issue = [&](auto slot_c, int64_t s) {
+        constexpr int slot = decltype(slot_c)::value;
+        const uint32_t so = (uint32_t)s * PK_TILE;
+        pk_dma16(Bw + so + lo, &sB[slot][wave * PK_TILE]);
+        pk_dma16(Bw + so + 1024 + lo, &sB[slot][wave * PK_TILE + 1024]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            ar[slot][i][0] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + lo));
+            ar[slot][i][1] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + 1024 + lo));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
 
-    // K runs in segments of at most PK_SEG steps, each a full pipeline
-    // (prologue, steady state, drain), so that the int32 sums can be shrunk
-    // modulo m between segments while no fragment registers are live.
-    for (int64_t k0 = 0; k0 < KT; k0 += PK_SEG) {
-        const int64_t k1 = (KT - k0 < PK_SEG) ? KT : k0 + PK_SEG;
-        // prologue: steps k0 .. k0 + 2 in flight; DMA of the first two complete
-        PK_ISSUE(0, k0);
-        PK_ISSUE(1, (k0 + 1 < k1 ? k0 + 1 : k1 - 1));
-        PK_ISSUE(2, (k0 + 2 < k1 ? k0 + 2 : k1 - 1));
-        PK_WAIT_VM(18);
+    // One K step.  With one wave per SIMD every instruction issues in order,
+    // so the loads are spread between the MFMAs (one after each of the first
+    // ten c = 0 MFMAs, the next step's fragment reads after the first four
+    // c = 1 MFMAs) instead of stalling the matrix pipe behind a burst.
+    auto step = [&](auto u_c, int64_t k, int64_t k1) {
+        constexpr int u = decltype(u_c)::value;
+        constexpr int slot = (u + L) % D;
+        if constexpr (u % BAR == 0) {
+            PK_WAIT_VM(WAIT_NEXT);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            bf1[j] = *(const v4i *)&sB[u][j * PK_TILE + 1024 + lo];
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t so = (uint32_t)(k + L < k1 ? k + L : k1 - 1) * PK_TILE;
+        cutc_static_for<0, 16>([&](auto q_c) {
+            constexpr int q = decltype(q_c)::value;
+            constexpr int i = q / 4, j = q % 4;
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][0], bf0[j], acc[i][j], 0, 0, 0);
+            if constexpr (q < 2) {
+                pk_dma16(Bw + so + q * 1024 + lo, &sB[slot][wave * PK_TILE + q * 1024]);
+            } else if constexpr (q < 10) {
+                constexpr int ii = (q - 2) / 2, cc = (q - 2) % 2;
+                ar[slot][ii][cc] = *(const v4i *)(Aw + ((uint32_t)ii * (uint32_t)rbs + so + cc * 1024 + lo));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        cutc_static_for<0, 16>([&](auto q_c) {
+            constexpr int q = decltype(q_c)::value;
+            constexpr int i = q / 4, j = q % 4;
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][1], bf1[j], acc[i][j], 0, 0, 0);
+            if constexpr (q < 4)
+                bf0[q] = *(const v4i *)&sB[(u + 1) % D][q * PK_TILE + lo];
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    };
+
+    for (int64_t k0 = 0; k0 < KT; k0 += SEG) {
+        const int64_t k1 = (KT - k0 < SEG) ? KT : k0 + SEG;
+        // prologue: steps k0 .. k0 + L - 1 in flight; DMA of the first BAR + 1 complete
+        cutc_static_for<0, L>([&](auto s_c) {
+            constexpr int s = decltype(s_c)::value;
+            issue(s_c, (k0 + s < k1 ? k0 + s : k1 - 1));
+        });
+        PK_WAIT_VM(WAIT_NEXT);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            bf0[j] = *(const v4i *)&sB0[j * PK_TILE + lo];
+            bf0[j] = *(const v4i *)&sB[0][j * PK_TILE + lo];
 
-#define PK_STEP(u)                                                                                                    \
-    do {                                                                                                              \
-        const int64_t k = kb + (u);                                                                                   \
-        PK_WAIT_VM(18);                                                                                               \
-        __syncthreads();                                                                                              \
-        _Pragma("unroll") for (int j = 0; j < 4; j++) bf1[j] = *(const v4i *)&PK_SB(u)[j * PK_TILE + 1024 + lo];          \
-        PK_ISSUE(((u) + 3) & 3, (k + 3 < k1 ? k + 3 : k1 - 1));                                                       \
-        _Pragma("unroll") for (int i = 0; i < 4; i++) _Pragma("unroll") for (int j = 0; j < 4; j++) acc[i][j] =      \
-            __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][0], bf0[j], acc[i][j], 0, 0, 0);                           \
-        __builtin_amdgcn_sched_barrier(0);                                                                            \
-        _Pragma("unroll") for (int j = 0; j < 4; j++) bf0[j] = *(const v4i *)&PK_SB(((u) + 1) & 3)[j * PK_TILE + lo];    \
-        _Pragma("unroll") for (int i = 0; i < 4; i++) _Pragma("unroll") for (int j = 0; j < 4; j++) acc[i][j] =      \
-            __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][1], bf1[j], acc[i][j], 0, 0, 0);                           \
-        __builtin_amdgcn_sched_barrier(0);                                                                            \
-    } while (0)
+        for (int64_t kb = k0; kb < k1; kb += D)
+            cutc_static_for<0, D>([&](auto u_c) { step(u_c, kb + decltype(u_c)::value, k1); });
 
-        for (int64_t kb = k0; kb < k1; kb += PK_DEPTH) {
-            PK_STEP(0);
-            PK_STEP(1);
-            PK_STEP(2);
-            PK_STEP(3);
-        }
         // drain: the repeated loads of the last steps land before the next
         // segment's prologue reuses their LDS stages
         PK_WAIT_VM(0);
@@ -698,10 +722,6 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
                         acc[i][j][r] = shrink_mod(acc[i][j][r], mod, inv_modf);
         }
     }
-#undef PK_STEP
-#undef PK_ISSUE
-#undef PK_DMA
-#undef PK_SB
 
     // result element r of lane l: A row 8 (r/4) + 4 (l/32) + r%4, B row l%32
     const int li = lane & 31, lg = lane >> 5;
@@ -736,9 +756,10 @@ extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, c
     if (ta * tb > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad, pb = CUTC_PK_ROWS_B(N) * kpad;
+    /* 6-stage ring, loads 4 steps ahead, a barrier every second step */
     for (int i = 0; i < c->n; i++)
-        hipLaunchKernelGGL(crt_gemm_pk_kernel, dim3((unsigned)(ta * tb)), dim3(256), 0, st, A + i * pa, B + i * pb, KT,
-                           (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], C + (int64_t)i * M * N);
+        hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040>), dim3((unsigned)(ta * tb)), dim3(256), 0, st, A + i * pa,
+                           B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], C + (int64_t)i * M * N);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 

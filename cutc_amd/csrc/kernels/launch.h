@@ -134,10 +134,11 @@ int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B
  * c * 1024 + (32 h + r) * 16 + e: the 16 bytes lane 32 h + r of a wave feeds
  * to the c-th v_mfma_i32_32x32x32_i8 of the step, and every 1 KB half-tile is
  * one fully coalesced wave load.  Both operands use the same K order, so the
- * dot products are unchanged.  kpad is a multiple of CUTC_PK_KALIGN; the
+ * dot products are unchanged.  kpad is a multiple of CUTC_PK_KALIGN (the
+ * K steps of every pipeline depth of the GEMM tile it); the
  * G-side planes (A) hold CUTC_PK_ROWS_A(rows) rows, the pair-side planes (B)
  * CUTC_PK_ROWS_B(rows); rows past the valid ones are never stored. */
-#define CUTC_PK_KALIGN 256
+#define CUTC_PK_KALIGN 768
 #define CUTC_PK_ROWS_A(rows) ((((int64_t)(rows)) + 511) / 512 * 512)
 #define CUTC_PK_ROWS_B(rows) ((((int64_t)(rows)) + 127) / 128 * 128)
 int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
