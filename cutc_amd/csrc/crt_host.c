@@ -3,13 +3,16 @@
  *
  * Scheme (Ozaki, Uchino & Imamura, "Ozaki Scheme II: a GEMM-oriented
  * emulation of floating-point matrix multiplication using an integer modular
- * technique"): each operand row is scaled by a power of two and rounded to
- * an integer of at most `bits` bits; the exact integer product
- * V = sum_k a_k b_k is obtained from its residues modulo n pairwise-coprime
- * moduli m_i <= 256, each residue product being an int8 x int8 -> int32 GEMM;
- * V is rebuilt by the Chinese remainder theorem and scaled back to FP64.
- * Correct reconstruction needs |V| < M/2 with M = prod m_i; `bits` is chosen
- * so that K 2^(2 bits) <= M/4.
+ * technique"): each operand row x is scaled by 2^(bits - e), ||x||_2 < 2^e,
+ * and rounded to integers q; the exact integer product V = sum_k a_k b_k is
+ * obtained from its residues modulo n pairwise-coprime moduli m_i <= 256,
+ * each residue product being an int8 x int8 -> int32 GEMM; V is rebuilt by
+ * the Chinese remainder theorem and scaled back to FP64.  Correct
+ * reconstruction needs |V| < M/2 with M = prod m_i.  Rounding moves a row by
+ * at most sqrt(K)/2 in 2-norm, so ||q||_2 < 2^bits + sqrt(K)/2 and, by
+ * Cauchy-Schwarz (the scheme's "fast mode" bound), |V| < (2^bits +
+ * sqrt(K)/2)^2; `bits` is the largest value (at most 50, which keeps every
+ * |q| < 2^51 exact in FP64) for which that square stays below M/2.
  *
  * CRT: with M_i = M / m_i and y_i = M_i^-1 mod m_i,
  *   V == S = sum_i u_i M_i  (mod M),  u_i = (c_i y_i) mod m_i in [0, m_i),
@@ -63,9 +66,12 @@ int cutc_crt_setup(int64_t K, int n, cutc_crt *c)
         M *= (u128)k_moduli[i];
         log2M += log2((double)k_moduli[i]);
     }
-    int bits = (int)floor((log2M - 2.0 - log2((double)K)) / 2.0);
-    if (bits > 50)
-        bits = 50;
+    /* largest bits with 2 log2(2^bits + sqrt(K)/2) < log2(M) - 1, with a
+     * margin far above the rounding of log2M */
+    const double half_rootk = 0.5 * sqrt((double)K);
+    int bits = 50;
+    while (bits >= 8 && 2.0 * log2(ldexp(1.0, bits) + half_rootk) >= log2M - 1.0 - 1e-9)
+        bits--;
     if (bits < 8)
         return CUTC_ERR_ARG;
     c->n = n;

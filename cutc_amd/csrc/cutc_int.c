@@ -137,6 +137,11 @@ void cutc_default_options(cutc_options *opt)
     opt->slices = 4;
 }
 
+/* Moduli of the int8 contraction when the caller leaves slices at 0: with
+ * 2-norm row scaling, 11 moduli keep the contractions of the 20-nucleus,
+ * n_ao = 200 workload near 1e-11 relative error. */
+#define CUTC_INT8_DEFAULT_MODULI 11
+
 static int valid_precision(const cutc_options *o)
 {
     if (o->precision == CUTC_PREC_FP64)
@@ -144,6 +149,15 @@ static int valid_precision(const cutc_options *o)
     if (o->precision == CUTC_PREC_INT8 && o->slices >= 6 && o->slices <= CUTC_CRT_MAX)
         return 1;
     return 0;
+}
+
+/* a copy of *in with defaults filled in (slices = 0 -> the precision's default) */
+static cutc_options resolved_options(const cutc_options *in)
+{
+    cutc_options o = *in;
+    if (o.precision == CUTC_PREC_INT8 && o.slices == 0)
+        o.slices = CUTC_INT8_DEFAULT_MODULI;
+    return o;
 }
 
 /* Boys-Handy exponents must be non-negative; a^b is evaluated by repeated
@@ -397,7 +411,7 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
     int *pairs = NULL, *dcls = NULL;
     cutc_jmono *dmono = NULL;
     double *G = NULL, *f2tab = NULL, *tab = NULL;
-    uint64_t *rmaxA = NULL, *rmaxG = NULL;
+    int *rexpA = NULL, *rexpG = NULL;
     int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
     cutc_crt crt;
@@ -407,8 +421,8 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
 
     ws_t ws;
     {
-        const size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(uint64_t) * P) +
-                            ws_round(sizeof(uint64_t) * 4 * n1c) + ws_round(bA) + ws_round(bG) + ws_round(bC) +
+        const size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(int) * P) +
+                            ws_round(sizeof(int) * 4 * n1c) + ws_round(bA) + ws_round(bG) + ws_round(bC) +
                             ws_round(sizeof(double) * 4 * n1c * ldk) +
                             (n_nuc > 32 ? ws_round(sizeof(double) * n_nuc * ldk) : 0) +
                             ws_round(sizeof(double) * (prog->ts * n1c + 1));
@@ -417,8 +431,8 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
             stats->workspace_bytes = need;
     }
     WS_TAKE(pairs, sizeof(int) * 2 * P);
-    WS_TAKE(rmaxA, sizeof(uint64_t) * P);
-    WS_TAKE(rmaxG, sizeof(uint64_t) * 4 * n1c);
+    WS_TAKE(rexpA, sizeof(int) * P);
+    WS_TAKE(rexpG, sizeof(int) * 4 * n1c);
     WS_TAKE(Apl, bA);
     WS_TAKE(Gpl, bG);
     WS_TAKE(Cpl, bC);
@@ -430,9 +444,9 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_SPLIT);
-    rc = cutc_crt_rowmax_pairw(wr2, aos2, n2, n_ao, rmaxA, st);
+    rc = cutc_crt_rownorm_pairw(wr2, aos2, n2, n_ao, rexpA, st);
     if (rc == CUTC_OK)
-        rc = cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rmaxA, Apl, st);
+        rc = cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rexpA, Apl, st);
     tr_end(tr, ev);
     TRY(rc);
 
@@ -448,9 +462,9 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
-        rc = cutc_crt_rowmax_k(G, ldk, 4 * n1e, n2, rmaxG, st);
+        rc = cutc_crt_rownorm_k(G, ldk, 4 * n1e, n2, rexpG, st);
         if (rc == CUTC_OK)
-            rc = cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rmaxG, Gpl, st);
+            rc = cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rexpG, Gpl, st);
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_GEMM1);
@@ -458,7 +472,7 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
-        rc = cutc_crt_rebuild_x(&crt, Cpl, 1, P, 4 * n1e, rmaxA, rmaxG, pairs, n_ao, X, n1, n1e, i0, st);
+        rc = cutc_crt_rebuild_x(&crt, Cpl, P, 4 * n1e, rexpA, rexpG, pairs, n_ao, X, n1, n1e, i0, st);
         tr_end(tr, ev);
         TRY(rc);
         if (stats)
@@ -487,7 +501,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
     const int64_t nao2 = (int64_t)n_ao * n_ao;
     int *pairs = NULL;
     double *Hp = NULL, *NHp = NULL;
-    uint64_t *rmaxA = NULL, *rmaxB = NULL;
+    int *rexpA = NULL, *rexpB = NULL;
+    double *npart = NULL;
     int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
     const int crt_mode = opt->precision == CUTC_PREC_INT8;
@@ -495,15 +510,16 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
 
     const int64_t rows = (n1c + 15) / 16 * 16;
     const int64_t plane = rows * P;
-    const int64_t kp_max = (n1c + 127) / 128 * 128;
+    const int64_t kp_max = (n1c + CUTC_PK_KALIGN - 1) / CUTC_PK_KALIGN * CUTC_PK_KALIGN;
     ws_t ws;
-    const size_t bpl = (size_t)nmod * CUTC_CRT_ROWS(P) * 3 * kp_max;
-    const size_t bcp = (size_t)CUTC_CRT_PARTS(3 * kp_max) * nmod * P * P;
+    const size_t bpl = (size_t)nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
+    const size_t bcp = (size_t)nmod * P * P;
     {
         size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
                       ws_round(sizeof(double) * 3 * plane);
         if (crt_mode)
-            need += 2 * ws_round(sizeof(uint64_t) * P) + 2 * ws_round(bpl) + ws_round(bcp);
+            need += 2 * ws_round(sizeof(int) * P) + ws_round(sizeof(double) * CUTC_CRT_KSPLIT * P) + 2 * ws_round(bpl) +
+                    ws_round(bcp);
         TRY(ws_begin(&ws, need));
         if (stats && need > stats->workspace_bytes)
             stats->workspace_bytes = need;
@@ -512,8 +528,9 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
     WS_TAKE(Hp, sizeof(double) * plane);
     WS_TAKE(NHp, sizeof(double) * 3 * plane);
     if (crt_mode) {
-        WS_TAKE(rmaxA, sizeof(uint64_t) * P);
-        WS_TAKE(rmaxB, sizeof(uint64_t) * P);
+        WS_TAKE(rexpA, sizeof(int) * P);
+        WS_TAKE(rexpB, sizeof(int) * P);
+        WS_TAKE(npart, sizeof(double) * CUTC_CRT_KSPLIT * P);
         WS_TAKE(Apl, bpl);
         WS_TAKE(Bpl, bpl);
         WS_TAKE(Cpl, bcp);
@@ -539,8 +556,10 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
             tr_end(tr, ev);
             TRY(rc);
         } else {
-            /* X^3 H, then X^{0..2} NH^{0..2} as one contraction over (m, r1) */
-            const int64_t kp = (n1e + 127) / 128 * 128;
+            /* X^3 H, then X^{0..2} NH^{0..2} as one contraction over (m, r1);
+             * the H side is the GEMM's A operand, so C[q][p] has the X-side
+             * pair p contiguous */
+            const int64_t kp = (n1e + CUTC_PK_KALIGN - 1) / CUTC_PK_KALIGN * CUTC_PK_KALIGN;
             for (int part = 0; part < 2; part++) {
                 const int nseg = part == 0 ? 1 : 3;
                 cutc_crt crt;
@@ -560,22 +579,22 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
                     sb.p[g] = part == 0 ? Hp : NHp + g * plane;
                 }
                 ev = tr_begin(tr, CAT_SPLIT);
-                rc = cutc_crt_rowmax_colk(&sa, P, rmaxA, st);
+                rc = cutc_crt_rownorm_colk(&sa, P, npart, rexpA, st);
                 if (rc == CUTC_OK)
-                    rc = cutc_crt_quant_colk(&crt, &sa, P, kp, rmaxA, Apl, st);
+                    rc = cutc_crt_quant_colk_pk(&crt, &sa, P, kp, CUTC_PK_ROWS_B(P), rexpA, Apl, st);
                 if (rc == CUTC_OK)
-                    rc = cutc_crt_rowmax_colk(&sb, P, rmaxB, st);
+                    rc = cutc_crt_rownorm_colk(&sb, P, npart, rexpB, st);
                 if (rc == CUTC_OK)
-                    rc = cutc_crt_quant_colk(&crt, &sb, P, kp, rmaxB, Bpl, st);
+                    rc = cutc_crt_quant_colk_pk(&crt, &sb, P, kp, CUTC_PK_ROWS_A(P), rexpB, Bpl, st);
                 tr_end(tr, ev);
                 TRY(rc);
                 ev = tr_begin(tr, CAT_GEMM2);
-                rc = cutc_crt_gemm(&crt, Apl, P, Bpl, P, nseg * kp, Cpl, st);
+                rc = cutc_crt_gemm_pk(&crt, Bpl, P, Apl, P, nseg * kp, Cpl, st);
                 tr_end(tr, ev);
                 TRY(rc);
                 ev = tr_begin(tr, CAT_SPLIT);
-                rc = cutc_crt_rebuild_m(&crt, Cpl, CUTC_CRT_PARTS(nseg * kp), P, rmaxA, rmaxB, part == 0 ? 1.0 : -0.5,
-                                        pairs, n_ao, out, part == 0 ? (i0 == 0 ? 0 : 1) : 2, st);
+                rc = cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, part == 0 ? 1.0 : -0.5, pairs, n_ao, out,
+                                        part == 0 ? (i0 == 0 ? 0 : 1) : 2, st);
                 tr_end(tr, ev);
                 TRY(rc);
             }
@@ -629,6 +648,8 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
     int rc = check_sizes(n1, n2, n_ao, n_nuc, size_bh);
     if (rc)
         return rc;
+    const cutc_options ropt = resolved_options(opt);
+    opt = &ropt;
     if (!valid_precision(opt))
         return CUTC_ERR_ARG;
     if (n_ao > 0 && n1 > 0 && !X)
@@ -908,6 +929,8 @@ int cutc_int_ex(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int
         cutc_default_options(&def);
         opt = &def;
     }
+    const cutc_options ropt = resolved_options(opt);
+    opt = &ropt;
     int rc = valid_precision(opt) ? CUTC_OK : CUTC_ERR_ARG;
     if (rc == CUTC_OK) {
         pthread_mutex_lock(&g_lock);

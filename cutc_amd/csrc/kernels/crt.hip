@@ -5,28 +5,22 @@
 // the rebuild formula in crt_host.c).
 //
 // Kernels:
-//   rowmax_*     max |x| per operand row (uint64 bit pattern, atomicMax)
-//   quant_*      x -> q = rint(x 2^(bits - E_row)) -> residues q mod m_i, int8
-//                planes [i][row][kpad] (K contiguous, zero past K)
-//   crt_gemm     per modulus: 256 x 256 block tile, 4 waves of 128 x 128
-//                (4 x 4 MFMA 32x32x32), K step 128 bytes through LDS,
-//                double-buffered; int32 sums reduced mod m every 65536 K and
-//                at the end; int8 residues stored transposed C[i][j][p]
-//   rebuild_*    CRT of the n residues per element, scaling, and the scatter
-//                to the reference's full layouts (as the FP64 epilogues)
+//   rownorm_*       row exponents e with ||x_row||_2 < 2^e (fixed-order sums)
+//   quant_*_pk      x -> q = rint(x 2^(bits - e_row)) -> residues q mod m_i,
+//                   int8 planes in the fragment-packed layout (launch.h)
+//   crt_gemm_pk     per modulus: 512 x 128 block tile, 4 waves of 128 x 128
+//                   (4 x 4 MFMA 32x32x32); A fragments streamed into a
+//                   register ring, B through an LDS ring fed by LDS DMA;
+//                   int8 residues C[i][r][p]
+//   rebuild_*       CRT of the n residues per element, scaling, and the
+//                   scatter to the reference's full layouts (as the FP64
+//                   epilogues)
 #include "common.h"
 #include "launch.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-#define CRT_KSTEP 128
-
-__device__ static inline int row_exponent(uint64_t bits)
-{
-    const double mx = __longlong_as_double((long long)bits);
-    return (mx > 0.0) ? ilogb(mx) + 1 : 0;
-}
 
 // residue of the integer-valued double q (|q| < 2^52) modulo m, in [-128, 127]
 __device__ static inline int residue(double q, int m, double inv_m)
@@ -40,65 +34,60 @@ __device__ static inline int residue(double q, int m, double inv_m)
     return r;
 }
 
-__device__ static inline void atomic_max_u64(uint64_t *p, uint64_t v)
+// Row exponent e with ||x||_2 < 2^e from the computed sum of squares ss:
+// the sum carries a relative rounding error far below 2^-29 for the row
+// lengths here (at most ~2^20 terms in fixed-order sums), so ss is inflated
+// by that much before the root.  A zero row (or one whose squares underflow)
+// gets e = 0, which rounds all its entries to 0.
+__device__ static inline int norm_exponent(double ss)
 {
-    atomicMax((unsigned long long *)p, (unsigned long long)v);
+    if (!(ss > 0.0))
+        return 0;
+    return ilogb(sqrt(ss * (1.0 + 0x1p-29))) + 1;
 }
 
-// block-wide max of v (256 threads), returned to thread 0
-__device__ static inline uint64_t block_max_u64(uint64_t v)
+// block-wide sum of v (256 threads) in a fixed order, returned to thread 0
+__device__ static inline double block_sum(double v)
 {
-    __shared__ uint64_t red[4];
-    for (int off = 32; off > 0; off >>= 1) {
-        uint64_t o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
+    __shared__ double red[4];
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
     if ((threadIdx.x & 63) == 0)
         red[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; w++)
-            v = red[w] > v ? red[w] : v;
-    }
+    if (threadIdx.x == 0)
+        v = (red[0] + red[1]) + (red[2] + red[3]);
     return v;
 }
 
-__device__ static inline uint64_t abs_bits(double x)
-{
-    return (uint64_t)__double_as_longlong(fabs(x));
-}
+// ---------------------------------------------------------------- row norms
 
-// ---------------------------------------------------------------- row maxima
-
-__global__ __launch_bounds__(256) void rowmax_k_kernel(const double *__restrict__ X, int64_t ld, int64_t K,
-                                                       uint64_t *rmax)
+__global__ __launch_bounds__(256) void rownorm_k_kernel(const double *__restrict__ X, int64_t ld, int64_t K,
+                                                        int *rexp)
 {
     const double *row = X + (int64_t)blockIdx.x * ld;
-    uint64_t v = 0;
-    for (int64_t k = threadIdx.x; k < K; k += 256) {
-        uint64_t b = abs_bits(row[k]);
-        v = b > v ? b : v;
-    }
-    v = block_max_u64(v);
+    double v = 0.0;
+    for (int64_t k = threadIdx.x; k < K; k += 256)
+        v = fma(row[k], row[k], v);
+    v = block_sum(v);
     if (threadIdx.x == 0)
-        rmax[blockIdx.x] = v;
+        rexp[blockIdx.x] = norm_exponent(v);
 }
 
-extern "C" int cutc_crt_rowmax_k(const double *X, int64_t ld, int64_t rows, int64_t K, uint64_t *rmax,
-                                 hipStream_t st)
+extern "C" int cutc_crt_rownorm_k(const double *X, int64_t ld, int64_t rows, int64_t K, int *rexp, hipStream_t st)
 {
     if (rows <= 0)
         return 0;
     if (rows > 2147483647)
         return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(rowmax_k_kernel, dim3((unsigned)rows), dim3(256), 0, st, X, ld, K, rmax);
+    hipLaunchKernelGGL(rownorm_k_kernel, dim3((unsigned)rows), dim3(256), 0, st, X, ld, K, rexp);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
 // row p = (k, l) of A[p][r2] = (w chi_l) chi_k
-__global__ __launch_bounds__(256) void rowmax_pairw_kernel(const double *__restrict__ wr2,
-                                                           const double *__restrict__ aos2, int64_t n2, int n_ao,
-                                                           uint64_t *rmax)
+__global__ __launch_bounds__(256) void rownorm_pairw_kernel(const double *__restrict__ wr2,
+                                                            const double *__restrict__ aos2, int64_t n2, int n_ao,
+                                                            int *rexp)
 {
     const int64_t p = blockIdx.x;
     int l = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
@@ -107,23 +96,23 @@ __global__ __launch_bounds__(256) void rowmax_pairw_kernel(const double *__restr
     while ((int64_t)l * (l + 1) / 2 > p)
         l--;
     const int k = (int)(p - (int64_t)l * (l + 1) / 2);
-    uint64_t v = 0;
+    double v = 0.0;
     for (int64_t i = threadIdx.x; i < n2; i += 256) {
-        uint64_t b = abs_bits(wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k]);
-        v = b > v ? b : v;
+        const double a = wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k];
+        v = fma(a, a, v);
     }
-    v = block_max_u64(v);
+    v = block_sum(v);
     if (threadIdx.x == 0)
-        rmax[p] = v;
+        rexp[p] = norm_exponent(v);
 }
 
-extern "C" int cutc_crt_rowmax_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, uint64_t *rmax,
-                                     hipStream_t st)
+extern "C" int cutc_crt_rownorm_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int *rexp,
+                                      hipStream_t st)
 {
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     if (P <= 0)
         return 0;
-    hipLaunchKernelGGL(rowmax_pairw_kernel, dim3((unsigned)P), dim3(256), 0, st, wr2, aos2, n2, n_ao, rmax);
+    hipLaunchKernelGGL(rownorm_pairw_kernel, dim3((unsigned)P), dim3(256), 0, st, wr2, aos2, n2, n_ao, rexp);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -134,8 +123,9 @@ __device__ static inline int64_t colsrc_col(const cutc_colsrc &s, int64_t r)
     return r;
 }
 
-// grid (ceil(rows/256), KSPLIT): each block sweeps a K slice of every segment
-__global__ __launch_bounds__(256) void rowmax_colk_kernel(cutc_colsrc s, int64_t rows, uint64_t *rmax)
+// grid (ceil(rows/256), ks): block y sums the squares of its K slice of every
+// segment into partial[y][r]; the slices are then added in order
+__global__ __launch_bounds__(256) void rownorm_colk_kernel(cutc_colsrc s, int64_t rows, double *partial)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= rows)
@@ -143,32 +133,43 @@ __global__ __launch_bounds__(256) void rowmax_colk_kernel(cutc_colsrc s, int64_t
     const int64_t col = colsrc_col(s, r);
     const int64_t per = (s.K + gridDim.y - 1) / gridDim.y;
     const int64_t k0 = per * blockIdx.y, k1 = (k0 + per < s.K) ? k0 + per : s.K;
-    uint64_t v = 0;
+    double v = 0.0;
     for (int seg = 0; seg < s.nsrc; seg++) {
         const double *base = s.p[seg] + col;
         for (int64_t k = k0; k < k1; k++) {
-            uint64_t b = abs_bits(base[k * s.ld]);
-            v = b > v ? b : v;
+            const double a = base[k * s.ld];
+            v = fma(a, a, v);
         }
     }
-    if (v)
-        atomic_max_u64(rmax + r, v);
+    partial[(int64_t)blockIdx.y * rows + r] = v;
 }
 
-extern "C" int cutc_crt_rowmax_colk(const cutc_colsrc *src, int64_t rows, uint64_t *rmax, hipStream_t st)
+__global__ __launch_bounds__(256) void rownorm_finish_kernel(const double *__restrict__ partial, int ks, int64_t rows,
+                                                             int *rexp)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows)
+        return;
+    double v = 0.0;
+    for (int y = 0; y < ks; y++)
+        v += partial[(int64_t)y * rows + r];
+    rexp[r] = norm_exponent(v);
+}
+
+extern "C" int cutc_crt_rownorm_colk(const cutc_colsrc *src, int64_t rows, double *partial, int *rexp,
+                                     hipStream_t st)
 {
     if (rows <= 0)
         return 0;
-    int rc = cutc_launch_fill((double *)rmax, rows, 0.0, st);
-    if (rc)
-        return rc;
-    if (src->K <= 0)
-        return 0;
     int64_t ks = (src->K + 511) / 512;
-    if (ks > 256)
-        ks = 256;
-    hipLaunchKernelGGL(rowmax_colk_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)ks), dim3(256), 0, st, *src,
-                       rows, rmax);
+    if (ks > CUTC_CRT_KSPLIT)
+        ks = CUTC_CRT_KSPLIT;
+    if (ks < 1)
+        ks = 1;
+    hipLaunchKernelGGL(rownorm_colk_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)ks), dim3(256), 0, st,
+                       *src, rows, partial);
+    hipLaunchKernelGGL(rownorm_finish_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, partial, (int)ks,
+                       rows, rexp);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -191,280 +192,8 @@ __device__ static inline void store_residues(const cutc_crt &c, const double (&q
     }
 }
 
-// grid (ceil(kpad / 16 / 256), rows)
-__global__ __launch_bounds__(256) void quant_k_kernel(cutc_crt c, const double *__restrict__ X, int64_t ld,
-                                                      int64_t plane, int64_t K, int64_t kpad,
-                                                      const uint64_t *__restrict__ rmax, int8_t *planes)
-{
-    const int64_t r = blockIdx.y;
-    const int64_t k0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 16;
-    if (k0 >= kpad)
-        return;
-    const int sh = c.bits - row_exponent(rmax[r]);
-    const double *row = X + r * ld;
-    double q[16];
-#pragma unroll
-    for (int e = 0; e < 16; e++)
-        q[e] = (k0 + e < K) ? rint(ldexp(row[k0 + e], sh)) : 0.0;
-    store_residues(c, q, planes + r * kpad + k0, plane);
-}
-
-extern "C" int cutc_crt_quant_k(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K,
-                                int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st)
-{
-    if (rows <= 0 || kpad <= 0)
-        return 0;
-    if (kpad % CRT_KSTEP)
-        return CUTC_ERR_ARG;
-    const int64_t gx = (kpad / 16 + 255) / 256;
-    const int64_t plane = CUTC_CRT_ROWS(rows) * kpad;
-    for (int64_t r0 = 0; r0 < rows; r0 += 65535) {
-        const int64_t nr = (rows - r0 < 65535) ? rows - r0 : 65535;
-        hipLaunchKernelGGL(quant_k_kernel, dim3((unsigned)gx, (unsigned)nr), dim3(256), 0, st, *c, X + r0 * ld, ld,
-                           plane, K, kpad, rmax + r0, planes + r0 * kpad);
-    }
-    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
-}
-
-// A rows computed on the fly: grid (ceil(kpad / 16 / 256), P)
-__global__ __launch_bounds__(256) void quant_pairw_kernel(cutc_crt c, const double *__restrict__ wr2,
-                                                          const double *__restrict__ aos2, int64_t n2, int n_ao,
-                                                          int64_t kpad, const uint64_t *__restrict__ rmax,
-                                                          int8_t *planes)
-{
-    const int64_t p = blockIdx.y;
-    const int64_t k0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 16;
-    if (k0 >= kpad)
-        return;
-    int l = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(l + 1) * (l + 2) / 2 <= p)
-        l++;
-    while ((int64_t)l * (l + 1) / 2 > p)
-        l--;
-    const int k = (int)(p - (int64_t)l * (l + 1) / 2);
-    const int sh = c.bits - row_exponent(rmax[p]);
-    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
-    const int64_t plane = CUTC_CRT_ROWS(P) * kpad;
-    double q[16];
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const int64_t i = k0 + e;
-        q[e] = (i < n2) ? rint(ldexp(wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k], sh)) : 0.0;
-    }
-    store_residues(c, q, planes + p * kpad + k0, plane);
-}
-
-extern "C" int cutc_crt_quant_pairw(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
-                                    int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st)
-{
-    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
-    if (P <= 0 || kpad <= 0)
-        return 0;
-    if (kpad % CRT_KSTEP || P > 65535)
-        return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(quant_pairw_kernel, dim3((unsigned)((kpad / 16 + 255) / 256), (unsigned)P), dim3(256), 0, st,
-                       *c, wr2, aos2, n2, n_ao, kpad, rmax, planes);
-    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
-}
-
-// column-major sources: a 64-row x 64-k tile of one segment per block,
-// transposed through LDS.  grid (ceil(rows/64), kpad/64, nsrc)
-__global__ __launch_bounds__(256) void quant_colk_kernel(cutc_crt c, cutc_colsrc s, int64_t rows, int64_t kpad,
-                                                         const uint64_t *__restrict__ rmax, int8_t *planes)
-{
-    __shared__ double tile[64][65];
-    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-    const int seg = blockIdx.z;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    {
-        int64_t r = r0 + tx;
-        const bool rok = r < rows;
-        const int64_t col = colsrc_col(s, rok ? r : rows - 1);
-        for (int kk = ty; kk < 64; kk += 4) {
-            const int64_t k = k0 + kk;
-            tile[tx][kk] = (rok && k < s.K) ? s.p[seg][k * s.ld + col] : 0.0;
-        }
-    }
-    __syncthreads();
-    const int rr = threadIdx.x >> 2, part = threadIdx.x & 3;
-    const int64_t r = r0 + rr;
-    if (r >= rows)
-        return;
-    const int sh = c.bits - row_exponent(rmax[r]);
-    double q[16];
-#pragma unroll
-    for (int e = 0; e < 16; e++)
-        q[e] = rint(ldexp(tile[rr][16 * part + e], sh));
-    const int64_t ktot = kpad * s.nsrc;
-    store_residues(c, q, planes + r * ktot + seg * kpad + k0 + 16 * part, CUTC_CRT_ROWS(rows) * ktot);
-}
-
-extern "C" int cutc_crt_quant_colk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kpad,
-                                   const uint64_t *rmax, int8_t *planes, hipStream_t st)
-{
-    if (rows <= 0 || kpad <= 0)
-        return 0;
-    if (kpad % 64 || src->nsrc < 1 || src->nsrc > 3)
-        return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(quant_colk_kernel, dim3((unsigned)((rows + 63) / 64), (unsigned)(kpad / 64), src->nsrc),
-                       dim3(256), 0, st, *c, *src, rows, kpad, rmax, planes);
-    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
-}
-
-// ---------------------------------------------------------------- int8 GEMM
-
-#define IG_TILE 256
-#define IG_PITCH 144 // LDS row pitch in bytes: 128 + 16
-
 // x mod m into [-128, 127] for |x| < 2^31
 __device__ static inline int reduce_mod(int x, int m, double inv_m) { return residue((double)x, m, inv_m); }
-
-// A, B: planes with rows padded to a multiple of 256 (cutc_crt_rows), so the
-// tail tiles read valid memory; their extra rows are never stored.
-// K: row length of the planes; [k0, k0 + kn) is this launch's K range
-// (kn <= CUTC_CRT_KPART, so |sum| <= kn 2^14 < 2^31 stays exact in int32).
-// Operand fragments are double-buffered in registers: the LDS reads of K step
-// ks + 1 are issued before the 16 MFMAs of step ks, so the MFMA pipe never
-// waits on an LDS round trip inside a stage.
-__global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const int8_t *__restrict__ A, int64_t M,
-                                                          const int8_t *__restrict__ B, int64_t N, int64_t K,
-                                                          int64_t k0, int64_t kn, int mod, double inv_mod,
-                                                          int8_t *__restrict__ C, int tiles_m, int tiles_n)
-{
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][IG_TILE * IG_PITCH];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][IG_TILE * IG_PITCH];
-
-    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
-    int tm, tn;
-    cutc_group_tile<4>(t, tiles_m, tiles_n, tm, tn);
-    const int64_t row0 = (int64_t)tm * IG_TILE, col0 = (int64_t)tn * IG_TILE;
-
-    // loader: thread -> rows (tid / 8) + 32 i, 16-byte segment tid % 8;
-    // wave-uniform bases and 32-bit per-lane offsets
-    const int seg = threadIdx.x & 7, lr = threadIdx.x >> 3;
-    const int8_t *Ab = A + row0 * K + k0;
-    const int8_t *Bb = B + col0 * K + k0;
-    const uint32_t off0 = (uint32_t)(lr * K + 16 * seg);
-    const uint32_t rstep = (uint32_t)(32 * K);
-    v4i ra[8], rb[8];
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 128, wn = (wave & 1) * 128;
-    const int li = lane & 31, lg = lane >> 5;
-    const int fa_off = (wm + li) * IG_PITCH + 16 * lg;
-    const int fb_off = (wn + li) * IG_PITCH + 16 * lg;
-    const int st_off = lr * IG_PITCH + 16 * seg;
-
-    v16i acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            acc[i][j] = v16i{};
-
-    const int64_t nk = kn / CRT_KSTEP;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        ra[i] = *(const v4i *)(Ab + off0 + i * rstep);
-        rb[i] = *(const v4i *)(Bb + off0 + i * rstep);
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        *(v4i *)&sA[0][st_off + 32 * i * IG_PITCH] = ra[i];
-        *(v4i *)&sB[0][st_off + 32 * i * IG_PITCH] = rb[i];
-    }
-    __syncthreads();
-
-    for (int64_t kt = 0; kt < nk; kt++) {
-        const int cur = kt & 1;
-        const bool more = kt + 1 < nk;
-        if (more) {
-            const uint32_t off = off0 + (uint32_t)((kt + 1) * CRT_KSTEP);
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                ra[i] = *(const v4i *)(Ab + off + i * rstep);
-                rb[i] = *(const v4i *)(Bb + off + i * rstep);
-            }
-        }
-        const int8_t *la = &sA[cur][fa_off];
-        const int8_t *lb = &sB[cur][fb_off];
-        v4i fa[2][4], fb[2][4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            fa[0][i] = *(const v4i *)(la + 32 * i * IG_PITCH);
-            fb[0][i] = *(const v4i *)(lb + 32 * i * IG_PITCH);
-        }
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            const int cb = ks & 1;
-            if (ks < 3) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    fa[cb ^ 1][i] = *(const v4i *)(la + 32 * i * IG_PITCH + 32 * (ks + 1));
-                    fb[cb ^ 1][i] = *(const v4i *)(lb + 32 * i * IG_PITCH + 32 * (ks + 1));
-                }
-            }
-            // keep the next step's LDS reads ahead of this step's MFMAs
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cb][i], fb[cb][j], acc[i][j], 0, 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        if (more) {
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                *(v4i *)&sA[cur ^ 1][st_off + 32 * i * IG_PITCH] = ra[i];
-                *(v4i *)&sB[cur ^ 1][st_off + 32 * i * IG_PITCH] = rb[i];
-            }
-        }
-        __syncthreads();
-    }
-
-    // result element r of lane l: row 8 (r/4) + 4 (l/32) + r%4, column l%32;
-    // stored transposed: C[col][row] (rows = A rows contiguous)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int64_t col = col0 + wn + 32 * j + li;
-            if (col >= N)
-                continue;
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int64_t row = row0 + wm + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
-                if (row < M)
-                    C[col * M + row] = (int8_t)reduce_mod(acc[i][j][r], mod, inv_mod);
-            }
-        }
-    }
-}
-
-extern "C" int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
-                             int64_t Ktot, int8_t *C, hipStream_t st)
-{
-    if (M <= 0 || N <= 0)
-        return 0;
-    if (Ktot <= 0 || Ktot % CRT_KSTEP)
-        return CUTC_ERR_ARG;
-    const int64_t tm = (M + IG_TILE - 1) / IG_TILE, tn = (N + IG_TILE - 1) / IG_TILE;
-    if (tm * tn > 2147483647)
-        return CUTC_ERR_ARG;
-    const int64_t pa = CUTC_CRT_ROWS(M) * Ktot, pb = CUTC_CRT_ROWS(N) * Ktot;
-    const int nparts = CUTC_CRT_PARTS(Ktot);
-    for (int part = 0; part < nparts; part++) {
-        const int64_t k0 = (int64_t)part * CUTC_CRT_KPART;
-        const int64_t kn = (Ktot - k0 < CUTC_CRT_KPART) ? Ktot - k0 : CUTC_CRT_KPART;
-        for (int i = 0; i < c->n; i++) {
-            hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)(tm * tn)), dim3(256), 0, st, A + i * pa, M,
-                               B + i * pb, N, Ktot, k0, kn, c->m[i], c->inv_m[i],
-                               C + ((int64_t)part * c->n + i) * M * N, (int)tm, (int)tn);
-        }
-    }
-    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
-}
 
 // ------------------------------------------------ fragment-packed stage 1
 
@@ -483,14 +212,14 @@ __device__ static inline int64_t pk_offset(int64_t rb, int r, int64_t k, int64_t
 // the 16-byte stores of consecutive threads are consecutive.
 __global__ __launch_bounds__(256) void quant_k_pk_kernel(cutc_crt c, const double *__restrict__ X, int64_t ld,
                                                          int64_t rows, int64_t K, int64_t KT, int64_t plane,
-                                                         const uint64_t *__restrict__ rmax, int8_t *planes)
+                                                         const int *__restrict__ rexp, int8_t *planes)
 {
     const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
     const int64_t rb = blockIdx.y, row = rb * 32 + r;
     const int64_t k0 = (int64_t)blockIdx.x * 128 + 16 * j;
     double q[16];
     if (row < rows) {
-        const int sh = c.bits - row_exponent(rmax[row]);
+        const int sh = c.bits - rexp[row];
         const double *src = X + row * ld;
 #pragma unroll
         for (int e = 0; e < 16; e++)
@@ -504,7 +233,7 @@ __global__ __launch_bounds__(256) void quant_k_pk_kernel(cutc_crt c, const doubl
 }
 
 extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K,
-                                   int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st)
+                                   int64_t kpad, const int *rexp, int8_t *planes, hipStream_t st)
 {
     if (rows <= 0 || kpad <= 0)
         return 0;
@@ -512,7 +241,7 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
         return CUTC_ERR_ARG;
     const int64_t plane = CUTC_PK_ROWS_A(rows) * kpad;
     hipLaunchKernelGGL(quant_k_pk_kernel, dim3((unsigned)(kpad / 128), (unsigned)((rows + 31) / 32)), dim3(256), 0,
-                       st, *c, X, ld, rows, K, kpad / 64, plane, rmax, planes);
+                       st, *c, X, ld, rows, K, kpad / 64, plane, rexp, planes);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -520,7 +249,7 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
 __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const double *__restrict__ wr2,
                                                              const double *__restrict__ aos2, int64_t n2, int n_ao,
                                                              int64_t KT, int64_t plane,
-                                                             const uint64_t *__restrict__ rmax, int8_t *planes)
+                                                             const int *__restrict__ rexp, int8_t *planes)
 {
     const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
     const int64_t rb = blockIdx.y, p = rb * 32 + r;
@@ -534,7 +263,7 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
         while ((int64_t)l * (l + 1) / 2 > p)
             l--;
         const int k = (int)(p - (int64_t)l * (l + 1) / 2);
-        const int sh = c.bits - row_exponent(rmax[p]);
+        const int sh = c.bits - rexp[p];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int64_t i = k0 + e;
@@ -549,7 +278,7 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
 }
 
 extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2,
-                                       int n_ao, int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st)
+                                       int n_ao, int64_t kpad, const int *rexp, int8_t *planes, hipStream_t st)
 {
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     if (P <= 0 || kpad <= 0)
@@ -558,7 +287,7 @@ extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, con
         return CUTC_ERR_ARG;
     const int64_t plane = CUTC_PK_ROWS_B(P) * kpad;
     hipLaunchKernelGGL(quant_pairw_pk_kernel, dim3((unsigned)(kpad / 128), (unsigned)((P + 31) / 32)), dim3(256), 0,
-                       st, *c, wr2, aos2, n2, n_ao, kpad / 64, plane, rmax, planes);
+                       st, *c, wr2, aos2, n2, n_ao, kpad / 64, plane, rexp, planes);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -568,6 +297,53 @@ __device__ static inline int shrink_mod(int x, int m, float inv_m)
 {
     const int q = (int)__builtin_rintf((float)x * inv_m);
     return x - q * m;
+}
+
+// column-major sources (stage 2): a block packs 32 rows x 128 K of one
+// segment, read coalesced along the rows into LDS and written as 16-byte
+// fragments.  grid (ceil(rows / 32), kp / 128, nsrc); segment s occupies K
+// [s kp, (s + 1) kp), zero past the valid s.K.
+__global__ __launch_bounds__(256) void quant_colk_pk_kernel(cutc_crt c, cutc_colsrc s, int64_t rows, int64_t kp,
+                                                            int64_t KT, int64_t plane,
+                                                            const int *__restrict__ rexp, int8_t *planes)
+{
+    __shared__ double tile[128][33];
+    const int64_t rb = blockIdx.x, r0 = rb * 32, k0 = (int64_t)blockIdx.y * 128;
+    const int seg = blockIdx.z;
+    {
+        const int rr = threadIdx.x & 31;
+        const int64_t r = r0 + rr;
+        const bool rok = r < rows;
+        const int64_t col = colsrc_col(s, rok ? r : 0);
+        const double *src = s.p[seg] + col;
+        for (int kk = threadIdx.x >> 5; kk < 128; kk += 8) {
+            const int64_t k = k0 + kk;
+            tile[kk][rr] = (rok && k < s.K) ? src[k * s.ld] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const int64_t row = r0 + r;
+    double q[16];
+    const int sh = (row < rows) ? c.bits - rexp[row] : 0;
+#pragma unroll
+    for (int e = 0; e < 16; e++)
+        q[e] = rint(ldexp(tile[16 * j + e][r], sh));
+    store_residues(c, q, planes + pk_offset(rb, r, seg * kp + k0 + 16 * j, KT), plane);
+}
+
+extern "C" int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp,
+                                      int64_t plane_rows, const int *rexp, int8_t *planes, hipStream_t st)
+{
+    if (rows <= 0 || kp <= 0)
+        return 0;
+    if (kp % 128 || (kp * src->nsrc) % CUTC_PK_KALIGN || src->nsrc < 1 || src->nsrc > 3 || plane_rows < rows ||
+        (rows + 31) / 32 > 2147483647)
+        return CUTC_ERR_ARG;
+    const int64_t ktot = kp * src->nsrc;
+    hipLaunchKernelGGL(quant_colk_pk_kernel, dim3((unsigned)((rows + 31) / 32), (unsigned)(kp / 128), src->nsrc),
+                       dim3(256), 0, st, *c, *src, rows, kp, ktot / 64, plane_rows * ktot, rexp, planes);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
 typedef __attribute__((address_space(3))) int8_t lds_i8;
@@ -765,15 +541,12 @@ extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, c
 
 // ---------------------------------------------------------------- rebuild
 
-// V from the residues c_i of one element (see crt_host.c), as a double; the
-// residues of K parts are summed first.
-__device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, int64_t stride, int nparts)
+// V from the residues c_i of one element (see crt_host.c), as a double
+__device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, int64_t stride)
 {
     double sh = 0.0, sm = 0.0, sl = 0.0;
     for (int i = 0; i < c.n; i++) {
-        int ci = 0;
-        for (int part = 0; part < nparts; part++)
-            ci += (int)C[((int64_t)part * c.n + i) * stride];
+        const int ci = (int)C[(int64_t)i * stride];
         int u = (ci * c.y[i]) % c.m[i];
         if (u < 0)
             u += c.m[i];
@@ -790,9 +563,9 @@ __device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, in
 }
 
 // thread per (p, j), p fastest
-__global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t *__restrict__ C, int nparts,
-                                                        int64_t P, int64_t N, const uint64_t *__restrict__ rmaxA,
-                                                        const uint64_t *__restrict__ rmaxB,
+__global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t *__restrict__ C,
+                                                        int64_t P, int64_t N, const int *__restrict__ rexpA,
+                                                        const int *__restrict__ rexpB,
                                                         const int *__restrict__ pairs, int n_ao, double *X,
                                                         int64_t n1, int64_t n1c, int64_t i0)
 {
@@ -801,10 +574,10 @@ __global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t
         return;
     const int k = pairs[2 * p], l = pairs[2 * p + 1];
     const int64_t n2 = (int64_t)n_ao * n_ao;
-    const int ea = row_exponent(rmaxA[p]) - 2 * c.bits;
+    const int ea = rexpA[p] - 2 * c.bits;
     for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
-        const double V = crt_value(c, C + j * P + p, P * N, nparts);
-        const double v = ldexp(V, ea + row_exponent(rmaxB[j]));
+        const double V = crt_value(c, C + j * P + p, P * N);
+        const double v = ldexp(V, ea + rexpB[j]);
         const int64_t m = j / n1c, i1 = j - m * n1c;
         double *o = X + (m * n1 + i0 + i1) * n2;
         o[k + (int64_t)n_ao * l] = v;
@@ -812,24 +585,23 @@ __global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t
     }
 }
 
-extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, int64_t N,
-                                  const uint64_t *rmaxA,
-                                  const uint64_t *rmaxB, const int *pairs, int n_ao, double *X, int64_t n1,
+extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N,
+                                  const int *rexpA, const int *rexpB, const int *pairs, int n_ao, double *X, int64_t n1,
                                   int64_t n1c, int64_t i0, hipStream_t st)
 {
     if (P <= 0 || N <= 0)
         return 0;
     const int64_t gy = N < 65535 ? N : 65535;
     hipLaunchKernelGGL(rebuild_x_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, *c, C,
-                       nparts, P, N, rmaxA, rmaxB, pairs, n_ao, X, n1, n1c, i0);
+                       P, N, rexpA, rexpB, pairs, n_ao, X, n1, n1c, i0);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
 // thread per (p, q), p fastest; modes as the FP64 step-2 epilogue
-__global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t *__restrict__ C, int nparts,
+__global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t *__restrict__ C,
                                                         int64_t P,
-                                                        const uint64_t *__restrict__ rmaxA,
-                                                        const uint64_t *__restrict__ rmaxB, double alpha,
+                                                        const int *__restrict__ rexpA,
+                                                        const int *__restrict__ rexpB, double alpha,
                                                         const int *__restrict__ pairs, int n_ao, double *M, int mode)
 {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -838,10 +610,10 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
     const int k = pairs[2 * p], l = pairs[2 * p + 1];
     const int64_t n2 = (int64_t)n_ao * n_ao;
     const int64_t kl = k + (int64_t)n_ao * l, lk = l + (int64_t)n_ao * k;
-    const int ea = row_exponent(rmaxA[p]) - 2 * c.bits;
+    const int ea = rexpA[p] - 2 * c.bits;
     for (int64_t q = blockIdx.y; q < P; q += gridDim.y) {
-    const double V = crt_value(c, C + q * P + p, P * P, nparts);
-    const double v = alpha * ldexp(V, ea + row_exponent(rmaxB[q]));
+    const double V = crt_value(c, C + q * P + p, P * P);
+    const double v = alpha * ldexp(V, ea + rexpB[q]);
     const int a = pairs[2 * q], b = pairs[2 * q + 1];
     double *o1 = M + n2 * (a + (int64_t)n_ao * b);
     double *o2 = M + n2 * (b + (int64_t)n_ao * a);
@@ -864,14 +636,14 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
     }
 }
 
-extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, const uint64_t *rmaxA,
-                                  const uint64_t *rmaxB, double alpha, const int *pairs, int n_ao, double *M, int mode,
+extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
+                                  const int *rexpB, double alpha, const int *pairs, int n_ao, double *M, int mode,
                                   hipStream_t st)
 {
     if (P <= 0)
         return 0;
     const int64_t gy = P < 65535 ? P : 65535;
     hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, *c, C,
-                       nparts, P, rmaxA, rmaxB, alpha, pairs, n_ao, M, mode);
+                       P, rexpA, rexpB, alpha, pairs, n_ao, M, mode);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -78,9 +78,11 @@ int cutc_gemm2_f64(const double *X, int64_t xld, const double *B, int64_t ldb, i
                    const int *pairs, int n_ao, double *M, int mode, hipStream_t st);
 
 /* Modular (CRT) split-precision contractions: crt_host.c (host constants) and
- * crt.hip (kernels).  Residue planes are int8 [n][rows][kpad], kpad a multiple
- * of 128, zero past the valid K.  Row maxima are kept as the uint64 bit
- * pattern of max |x| (ordered like the doubles, so atomicMax works). */
+ * crt.hip (kernels).  Residue planes are int8 [n][packed rows x kpad] in the
+ * fragment-packed layout below, zero past the valid K.  Each operand row x
+ * is scaled by 2^(bits - e) with ||x||_2 < 2^e (its row exponent), so that
+ * ||q||_2 < 2^bits + sqrt(K)/2 for the rounded row q and, by Cauchy-Schwarz,
+ * every integer dot product stays below M/2 in magnitude. */
 #define CUTC_CRT_MAX 16
 typedef struct cutc_crt {
     int n, bits;
@@ -94,9 +96,6 @@ typedef struct cutc_crt {
 
 int cutc_crt_setup(int64_t K, int n, cutc_crt *c);
 
-/* rows of a residue plane: padded to the GEMM tile (256) */
-#define CUTC_CRT_ROWS(rows) ((((int64_t)(rows)) + 255) / 256 * 256)
-
 /* column-major sources for the stage-2 operands: element (row r, k) of
  * segment s is p[s][k * ld + col(r)], col(r) = k_r + n_ao l_r through the
  * pair table when pairs != NULL, else r. */
@@ -108,26 +107,12 @@ typedef struct cutc_colsrc {
     int n_ao;
 } cutc_colsrc;
 
-int cutc_crt_rowmax_k(const double *X, int64_t ld, int64_t rows, int64_t K, uint64_t *rmax, hipStream_t st);
-int cutc_crt_quant_k(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
-                     const uint64_t *rmax, int8_t *planes, hipStream_t st);
-int cutc_crt_rowmax_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, uint64_t *rmax,
-                          hipStream_t st);
-int cutc_crt_quant_pairw(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
-                         int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st);
-int cutc_crt_rowmax_colk(const cutc_colsrc *src, int64_t rows, uint64_t *rmax, hipStream_t st);
-int cutc_crt_quant_colk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kpad,
-                        const uint64_t *rmax, int8_t *planes, hipStream_t st);
-/* K parts of at most CUTC_CRT_KPART columns keep the int32 sums exact:
- * |sum| <= 130944 * 128^2 < 2^31. */
-#define CUTC_CRT_KPART 130944
-#define CUTC_CRT_PARTS(K) ((int)((((int64_t)(K)) + CUTC_CRT_KPART - 1) / CUTC_CRT_KPART))
-/* C[part][i][j][p] = (sum_{k in part} A[i][p][k] B[i][j][k]) mod m_i as int8
- * in [-128, 127]; A: M rows, B: N rows (planes with CUTC_CRT_ROWS rows), both
- * with Ktot (multiple of 128) columns.  C holds CUTC_CRT_PARTS(Ktot) n M N bytes. */
-int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t Ktot,
-                  int8_t *C, hipStream_t st);
-/* Fragment-packed residue planes for the stage-1 contraction.  Rows are
+int cutc_crt_rownorm_k(const double *X, int64_t ld, int64_t rows, int64_t K, int *rexp, hipStream_t st);
+int cutc_crt_rownorm_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int *rexp, hipStream_t st);
+/* partial: CUTC_CRT_KSPLIT x rows doubles of scratch */
+#define CUTC_CRT_KSPLIT 64
+int cutc_crt_rownorm_colk(const cutc_colsrc *src, int64_t rows, double *partial, int *rexp, hipStream_t st);
+/* Fragment-packed residue planes of the int8 contractions.  Rows are
  * grouped in blocks of 32 and K in steps of 64 bytes; the 2048-byte tile of
  * (row block rb, step kt) sits at (rb * KT + kt) * 2048 (KT = kpad / 64) and
  * holds byte k = 64 kt + 32 h + 16 c + e of row 32 rb + r at
@@ -142,9 +127,14 @@ int cutc_crt_gemm(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B
 #define CUTC_PK_ROWS_A(rows) ((((int64_t)(rows)) + 511) / 512 * 512)
 #define CUTC_PK_ROWS_B(rows) ((((int64_t)(rows)) + 127) / 128 * 128)
 int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
-                        const uint64_t *rmax, int8_t *planes, hipStream_t st);
+                        const int *rexp, int8_t *planes, hipStream_t st);
 int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
-                            int64_t kpad, const uint64_t *rmax, int8_t *planes, hipStream_t st);
+                            int64_t kpad, const int *rexp, int8_t *planes, hipStream_t st);
+/* column-major sources into packed planes of plane_rows rows (CUTC_PK_ROWS_A or _B of rows, by the operand's side); the
+ * K of segment s starts at s kp, kp a multiple of 128 and nsrc kp one of
+ * CUTC_PK_KALIGN. */
+int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp, int64_t plane_rows,
+                           const int *rexp, int8_t *planes, hipStream_t st);
 /* C[i][r][p] = (sum_k A[i][r][k] B[i][p][k]) mod m_i as int8 in [-128, 127]
  * for r < M, p < N, packed planes as above; C holds n M N bytes (row r of the
  * A side is contiguous over p). */
@@ -152,12 +142,12 @@ int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t
                      int8_t *C, hipStream_t st);
 /* Rebuild C = V 2^(eA + eB - 2 bits) and scatter into X (4, n1, n_ao, n_ao):
  * column j = m n1c + i1 -> X[m][i0 + i1][(k,l) and (l,k)]. */
-int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, int64_t N, const uint64_t *rmaxA,
-                       const uint64_t *rmaxB, const int *pairs, int n_ao, double *X, int64_t n1, int64_t n1c,
+int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N, const int *rexpA,
+                       const int *rexpB, const int *pairs, int n_ao, double *X, int64_t n1, int64_t n1c,
                        int64_t i0, hipStream_t st);
 /* Rebuild alpha C into the n_ao^2 x n_ao^2 matrix M; mode as cutc_gemm2_f64. */
-int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int nparts, int64_t P, const uint64_t *rmaxA,
-                       const uint64_t *rmaxB, double alpha, const int *pairs, int n_ao, double *M, int mode,
+int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
+                       const int *rexpB, double alpha, const int *pairs, int n_ao, double *M, int mode,
                        hipStream_t st);
 
 #ifdef __cplusplus

@@ -11,9 +11,9 @@
  *                 residues (4 nmod ldk bytes) and the GEMM residues (nmod x 4
  *                 n_pairs bytes)
  *   stage 2:      Hp and NHp for an r1 chunk (4 planes x roundup(n1c2,16) x
- *                 n_pairs doubles) [INT8: + both operands' residues,
+ *                 n_pairs doubles) [INT8: + both operands' packed residues,
  *                 2 x nmod x 3 x n_pairs bytes per r1 row, + the GEMM residues
- *                 2 x nmod x n_pairs^2 bytes]
+ *                 nmod x n_pairs^2 bytes]
  * Chunks are balanced (equal sizes rounded up to a multiple of 64). */
 #include <stdint.h>
 
@@ -75,16 +75,14 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     /* stage 2 */
     int64_t fixed2 = 8 * P + 4096, per_r1b = 4 * 8 * P;
     if (crt) {
-        fixed2 += 16 * P + 2 * nmod * P * P + 2 * nmod * 255 * 3 * 128;
-        per_r1b += 2 * nmod * 3 * CUTC_CRT_ROWS(P);
+        fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + nmod * P * P + 2 * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1);
+        per_r1b += 2 * nmod * 3 * CUTC_PK_ROWS_A(P);
     }
     int64_t n1c2 = 0;
     if (n1 > 0 && P > 0) {
         if ((int64_t)budget <= fixed2 + 144 * per_r1b)
             return CUTC_ERR_NOMEM;
         int64_t cmax = ((int64_t)budget - fixed2) / per_r1b - 144;
-        if (crt && cmax > 87296) /* keeps the (m, r1) contraction within two int32 K parts */
-            cmax = 87296;
         n1c2 = balanced(n1, cmax, cmax >= 64 ? 64 : 1);
     }
     pl->n1_chunk_2e = n1c2;

@@ -31,7 +31,7 @@ PREC_BF16 = 1
 PREC_FP16 = 2
 PREC_INT8 = 3
 PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16, "int8": PREC_INT8}
-DEFAULT_SLICES = {PREC_FP64: 4, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 12}
+DEFAULT_SLICES = {PREC_FP64: 4, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 11}
 
 
 class CutcError(RuntimeError):

@@ -112,7 +112,7 @@ const char *cutc_error_string(int code);
 
 typedef struct cutc_options {
     int precision;    /* CUTC_PREC_*                                             */
-    int slices;       /* bf16/fp16: slices per operand (3, 4); int8: moduli (6..16) */
+    int slices;       /* bf16/fp16: slices per operand (3, 4); int8: moduli (6..16, 0 = 11) */
     int verbose;      /* 1: print the reference's stage-time lines               */
     int n1_chunk;     /* r1 rows per G chunk, 0 = planner                        */
     size_t mem_limit; /* device bytes the workspaces may use, 0 = free memory    */

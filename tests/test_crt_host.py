@@ -52,9 +52,14 @@ def test_constants(K, n):
         assert (c.y[i] * Mi) % m[i] == 1
         assert all(0 <= c.Mi[i][t] < 2 ** 40 for t in range(3))
     b = c.bits
-    # |V| <= K 2^(2b) must stay within M / 4, and b is the largest such
-    assert K * 2 ** (2 * b) <= M // 4 or b == 50
-    assert b == 50 or K * 2 ** (2 * (b + 1)) > M // 4 or K * 2 ** (2 * (b + 1)) > M / 4
+    # Cauchy-Schwarz: |V| < (2^b + sqrt(K)/2)^2 must stay below M/2 (checked
+    # with exact integers: 4 (2^b + sqrt(K)/2)^2 = (2^(b+1) + sqrt(K))^2 <=
+    # (2^(b+1) + isqrt(K) + 1)^2), and b is the largest such value up to 50
+    def fits(bb):
+        return (2 ** (bb + 1) + math.isqrt(K) + 1) ** 2 < 2 * M
+
+    assert b <= 50 and fits(b)
+    assert b == 50 or not (2 ** (b + 2) + math.isqrt(K)) ** 2 < 2 * M
 
 
 def test_bad_arguments():
@@ -96,7 +101,7 @@ def test_rebuild_recovers_integers(n):
     K = 200000
     rc, c = setup(K, n)
     rng = np.random.default_rng(n)
-    bound = K * 2 ** (2 * c.bits)
+    bound = (2 ** c.bits + math.isqrt(K) // 2 + 1) ** 2  # largest |V| the scaling allows
     vals = [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 2000)]
     vals = [v * (bound // 2 ** 62) for v in vals] + [0, 1, -1, bound, -bound, 12345, -2 ** 40]
     res = np.array([[((v % c.m[i]) + 128) % c.m[i] - 128 if c.m[i] == 256 else
@@ -108,23 +113,29 @@ def test_rebuild_recovers_integers(n):
 
 
 def test_quantised_dot_products_match_exact_products():
-    """Whole scheme on small vectors: rounding per row, residues, modular
-    products, rebuild, scaling (the device steps, restated in numpy)."""
+    """Whole scheme on small vectors: rounding per row (2-norm exponents),
+    residues, modular products, rebuild, scaling (the device steps, restated
+    in numpy)."""
     K = 3000
     rc, c = setup(K, 12)
     rng = np.random.default_rng(1)
     A = rng.standard_normal((7, K)) * np.exp(rng.standard_normal((7, K)) * 3)
     B = rng.standard_normal((5, K))
-    B[2] *= 1e-200  # a tiny row
+    B[2] *= 1e-100  # a tiny row
+    A[3, :] = 0.0  # a zero row
+    A[4, :] = 0.0
+    A[4, 17] = 3.0  # a row with a single entry: 2-norm = max
 
     def quant(Mat):
-        mx = np.abs(Mat).max(axis=1)
-        E = np.where(mx > 0, np.floor(np.log2(mx)).astype(int) + 1, 0)
+        ss = (Mat * Mat).sum(axis=1)
+        E = np.where(ss > 0, np.frexp(np.sqrt(ss * (1 + 2.0 ** -29)))[1], 0)  # ||x|| < 2^E
         return np.rint(np.ldexp(Mat, (c.bits - E)[:, None])), E
 
     qa, ea = quant(A)
     qb, eb = quant(B)
-    assert np.abs(qa).max() <= 2.0 ** c.bits and np.abs(qb).max() <= 2.0 ** c.bits
+    bound = 2.0 ** c.bits + np.sqrt(K) / 2
+    assert (np.sqrt((qa * qa).sum(1)) < bound).all() and (np.sqrt((qb * qb).sum(1)) < bound).all()
+    assert np.abs(qa).max() < 2.0 ** 52
     ra = np.stack([residue(qa, c.m[i], c.inv_m[i]) for i in range(c.n)])
     rb = np.stack([residue(qb, c.m[i], c.inv_m[i]) for i in range(c.n)])
     assert ra.min() >= -128 and ra.max() <= 127
@@ -135,5 +146,9 @@ def test_quantised_dot_products_match_exact_products():
     np.testing.assert_allclose(V, exact.astype(np.float64), rtol=2.0 ** -50, atol=1.0)
     C = np.ldexp(V, (ea[:, None] + eb[None, :]) - 2 * c.bits)
     ref = A @ B.T
-    rel = np.abs(C - ref) / (np.abs(A).max(axis=1)[:, None] * np.abs(B).max(axis=1)[None, :] * K)
-    assert rel.max() <= 2.0 ** (-c.bits + 1)
+    # each rounding moves a row by at most sqrt(K)/2 quanta of 2^(e - bits)
+    na = np.sqrt((A * A).sum(1))[:, None]
+    nb = np.sqrt((B * B).sum(1))[None, :]
+    tol = np.sqrt(K) * 2.0 ** -c.bits * (na * 2 * nb + 2 * na * nb) + 1e-300
+    assert (np.abs(C - ref) <= tol).all()
+    assert np.array_equal(C[3], np.zeros(5))
