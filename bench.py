@@ -35,6 +35,13 @@ METRIC = "int_2e_ao build s + eff TFLOP/s: split-BF16 vs FP64 on GPU at ≤1e-10
 PEAK = {"fp64": 78.6, "int8": 5000.0, "bf16": 2516.0, "fp16": 2516.0}  # dense MFMA, MI355X (int8: TOPS)
 
 
+# HBM-side bytes per stage-1 contraction launch on the default workload (n1 = 60000,
+# n2 = 200000, 11 moduli), from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by
+# a 1 GiB copy kernel measured in the same run (profiles/cube20_traffic.txt); not measured
+# for other shapes.
+MEASURED_TRAFFIC = {"int8": 271.8e9, "fp64": 17.83e12}
+
+
 def w_ref(n_ao, n1, n2):
     return 8.0 * n_ao ** 2 * n1 * n2 + 8.0 * n_ao ** 4 * n1
 
@@ -51,6 +58,9 @@ def parse():
     ap.add_argument("--n2", type=int, default=200000)
     ap.add_argument("--cpu-sample", type=int, default=32, help="r1 points of the CPU-baseline sample (0: skip)")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="first run one device-to-device copy of 1 GiB, a plain copy kernel of known size that "
+                         "calibrates FETCH_SIZE / WRITE_SIZE counters read by rocprofv3 --pmc in the same run")
     return ap.parse_args()
 
 
@@ -107,6 +117,14 @@ def main():
 
     from cutc_amd import cutc, synth
 
+    if args.traffic_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(args, "traffic probe: copied 1 GiB device to device")
+
     t0 = time.perf_counter()
     s = synth.cube20(n1=args.n1, n2=args.n2)
     log(args, f"generated {s.shape_str()} in {time.perf_counter() - t0:.1f} s")
@@ -157,23 +175,27 @@ def main():
     # algorithmic work (packed pairs, unpadded n2); launch times are the
     # tracer's HIP events on the build's stream
     exec1 = 2.0 * P * 4 * s.n1 * s.n2
+    default_shape = (s.n1, s.n2) == (60000, 200000) and (args.precision == "fp64" or opt.slices == 11)
+    traffic = MEASURED_TRAFFIC[args.precision] if default_shape else None
     if args.precision == "fp64":
         launches = last["n_gemm1"]
         achieved = exec1 / (last["ms_gemm1"] / 1e3) / 1e12
         roof = {"kernel": "gemm_f64_kernel (step-1 contraction)", "bound": "mfma", "achieved": achieved,
-                "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": None,
+                "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic,
                 "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": "2 x 20100 pairs x 4 n1 x n2 FP64 flops"}
+                "work": "2 x 20100 pairs x 4 n1 x n2 FP64 flops",
+                "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
     else:
         launches = last["n_gemm1"] * opt.slices  # one launch per modulus and r1 chunk
         int8_ops = 2.0 * opt.slices * P * 4 * s.n1 * s.n2
         achieved = int8_ops / (last["ms_gemm1"] / 1e3) / 1e12
         roof = {"kernel": "crt_gemm_pk_kernel (step-1 contraction, one launch per modulus and r1 chunk)",
                 "bound": "mfma", "achieved": achieved, "peak": PEAK["int8"], "unit": "TOPS (int8)",
-                "frac": achieved / PEAK["int8"], "traffic": None, "launches": launches,
+                "frac": achieved / PEAK["int8"], "traffic": traffic, "launches": launches,
                 "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
                 "work": f"2 x {opt.slices} moduli x 20100 pairs x 4 n1 x n2 int8 multiply-adds (x2)",
-                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12}
+                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12,
+                "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
 
     out.update({
         "value": W / (ms / 1e3) / 1e12,
