@@ -13,6 +13,11 @@ split results are compared with the FP64 ones: relative Frobenius and
 max-abs error of every X component and of int_2e_ao, and of
 sum_{kl,ab} D_kl D_ab (kl|ab) for a seeded rank-10 PSD density D.
 
+--workload cube40 runs the large streamed case instead (40 nuclei, n_ao = 300,
+n1 = 80,000, n2 = 300,000, cutc_int_streamed_device: X is never stored, each
+r1 chunk goes straight into the int_2e_ao accumulator); errors are reported
+for int_2e_ao and the D D contraction.
+
 value = effective TFLOP/s of the timed (split) build = W_ref / seconds, with
 the reference formulation's work W_ref = 8 n_ao^2 n1 n2 + 8 n_ao^4 n1 (full
 AO pairs, the metric's definition).  The build executes only the packed
@@ -54,8 +59,11 @@ def parse():
     ap.add_argument("--precision", default="int8", choices=["fp64", "int8"])
     ap.add_argument("--slices", type=int, default=None, help="moduli of the int8 contraction (default 11)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
-    ap.add_argument("--n1", type=int, default=60000)
-    ap.add_argument("--n2", type=int, default=200000)
+    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40"],
+                    help="cube20: 20 nuclei, n_ao=200, X stored (the metric's workload); cube40: 40 nuclei, "
+                         "n_ao=300, n1=80000, n2=300000, streamed (X never stored)")
+    ap.add_argument("--n1", type=int, default=None)
+    ap.add_argument("--n2", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=32, help="r1 points of the CPU-baseline sample (0: skip)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -87,6 +95,12 @@ def cpu_baseline(sysfull, npts):
             "seconds": dt}
 
 
+def build(cutc, ds, X, I, opt, stream):
+    if X is None:
+        return cutc.cutc_int_streamed_device(ds, I, opt, stream)
+    return cutc.cutc_int_device(ds, X, I, opt, stream)
+
+
 def timed_builds(cutc, torch, ds, X, I, opt, n, stream):
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -94,7 +108,7 @@ def timed_builds(cutc, torch, ds, X, I, opt, n, stream):
     torch.cuda.synchronize()
     ev0.record(stream)
     for _ in range(n):
-        stats.append(cutc.cutc_int_device(ds, X, I, opt, stream))
+        stats.append(build(cutc, ds, X, I, opt, stream))
     ev1.record(stream)
     torch.cuda.synchronize()
     return ev0.elapsed_time(ev1) / n, stats
@@ -126,10 +140,18 @@ def main():
         log(args, "traffic probe: copied 1 GiB device to device")
 
     t0 = time.perf_counter()
-    s = synth.cube20(n1=args.n1, n2=args.n2)
+    streamed = args.workload == "cube40"
+    if streamed:
+        s = synth.cube40(n1=args.n1 or 80000, n2=args.n2 or 300000)
+    else:
+        s = synth.cube20(n1=args.n1 or 60000, n2=args.n2 or 200000)
     log(args, f"generated {s.shape_str()} in {time.perf_counter() - t0:.1f} s")
     ds = cutc.DeviceSystem.upload(s)
-    X, I = cutc.alloc_outputs(ds)
+    if streamed:
+        X = None
+        I = torch.empty((s.n_ao,) * 4, dtype=torch.float64, device="cuda")
+    else:
+        X, I = cutc.alloc_outputs(ds)
     stream = torch.cuda.current_stream()
     n_ao, P = s.n_ao, s.n_ao * (s.n_ao + 1) // 2
     W = w_ref(n_ao, s.n1, s.n2)
@@ -145,7 +167,7 @@ def main():
         small = s.subset_r1(min(256, s.n1))
         dsm = cutc.DeviceSystem.upload(small)
         Xs, Is = cutc.alloc_outputs(dsm)
-        cutc.cutc_int_device(dsm, Xs, Is, cutc.options("fp64"), stream)  # loads the kernels
+        build(cutc, dsm, None if streamed else Xs, Is, cutc.options("fp64"), stream)  # loads the kernels
         del Xs, Is, dsm
         nrep = args.steps if args.precision == "fp64" else 1
         ms, st = timed_builds(cutc, torch, ds, X, I, cutc.options("fp64"), nrep, stream)
@@ -153,9 +175,9 @@ def main():
         fp64 = {"build_s": ms / 1e3, "eff_tflops": W / (ms / 1e3) / 1e12, "stats": st[-1]}
         if args.precision != "fp64":
             cutc.release_workspace()  # the device workspace of one precision is not reused by the other
-            Iref = I.clone()
-            Xref_host = X.cpu()
-            dd_ref = dd(Iref)
+            dd_ref = dd(I)
+            Iref = I.cpu() if streamed else I.clone()
+            Xref_host = None if streamed else X.cpu()
 
     if args.precision == "fp64":
         ms, last = fp64["build_s"] * 1e3, fp64["stats"]
@@ -164,7 +186,7 @@ def main():
         opt = cutc.options(args.precision, args.slices)
         prec_label = f"{args.precision}x{opt.slices}"
         for i in range(args.warmup):
-            st = cutc.cutc_int_device(ds, X, I, opt, stream)
+            st = build(cutc, ds, X, I, opt, stream)
             log(args, f"warmup {i}: {st['ms_total'] / 1e3:.2f} s")
         ms, sts = timed_builds(cutc, torch, ds, X, I, opt, args.steps, stream)
         last = sts[-1]
@@ -175,7 +197,8 @@ def main():
     # algorithmic work (packed pairs, unpadded n2); launch times are the
     # tracer's HIP events on the build's stream
     exec1 = 2.0 * P * 4 * s.n1 * s.n2
-    default_shape = (s.n1, s.n2) == (60000, 200000) and (args.precision == "fp64" or opt.slices == 11)
+    default_shape = (not streamed and (s.n1, s.n2) == (60000, 200000)
+                     and (args.precision == "fp64" or opt.slices == 11))
     traffic = MEASURED_TRAFFIC[args.precision] if default_shape else None
     if args.precision == "fp64":
         launches = last["n_gemm1"]
@@ -183,7 +206,7 @@ def main():
         roof = {"kernel": "gemm_f64_kernel (step-1 contraction)", "bound": "mfma", "achieved": achieved,
                 "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic,
                 "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": "2 x 20100 pairs x 4 n1 x n2 FP64 flops",
+                "work": f"2 x {P} pairs x 4 n1 x n2 FP64 flops",
                 "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
     else:
         launches = last["n_gemm1"] * opt.slices  # one launch per modulus and r1 chunk
@@ -193,7 +216,7 @@ def main():
                 "bound": "mfma", "achieved": achieved, "peak": PEAK["int8"], "unit": "TOPS (int8)",
                 "frac": achieved / PEAK["int8"], "traffic": traffic, "launches": launches,
                 "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": f"2 x {opt.slices} moduli x 20100 pairs x 4 n1 x n2 int8 multiply-adds (x2)",
+                "work": f"2 x {opt.slices} moduli x {P} pairs x 4 n1 x n2 int8 ops",
                 "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12,
                 "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
 
@@ -209,7 +232,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else f"{prec_label}->f64",
         "data": "synthetic",
-        "config": {"workload": f"cube20: 20 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6",
+        "config": {"workload": (f"cube40: 40 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6, streamed "
+                                "(X never stored)") if streamed else
+                               f"cube20: 20 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6",
                    "precision": prec_label, "n1_chunk": last["n1_chunk"]},
         "build_s": ms / 1e3,
         "w_ref_flops": W,
@@ -227,11 +252,23 @@ def main():
                                 "peak": PEAK["fp64"], "frac": exec1 / fp64_g1 / 1e12 / PEAK["fp64"]}
         # errors of the split build against the FP64 build, every tensor
         err = {}
-        dI = I - Iref
-        err["int_2e_ao"] = {"rel_frob": float(torch.linalg.norm(dI) / torch.linalg.norm(Iref)),
-                            "max_abs": float(dI.abs().max()), "ref_max_abs": float(Iref.abs().max())}
-        del dI
-        for m in range(4):
+        if streamed:  # the FP64 result is on the host: compare in slabs
+            num = den = mx = rmx = 0.0
+            for a0 in range(0, n_ao, 10):
+                b = Iref[a0:a0 + 10].to(I.device)
+                d = I[a0:a0 + 10] - b
+                num += float((d * d).sum())
+                den += float((b * b).sum())
+                mx = max(mx, float(d.abs().max()))
+                rmx = max(rmx, float(b.abs().max()))
+                del b, d
+            err["int_2e_ao"] = {"rel_frob": (num / den) ** 0.5 if den > 0 else 0.0, "max_abs": mx, "ref_max_abs": rmx}
+        else:
+            dI = I - Iref
+            err["int_2e_ao"] = {"rel_frob": float(torch.linalg.norm(dI) / torch.linalg.norm(Iref)),
+                                "max_abs": float(dI.abs().max()), "ref_max_abs": float(Iref.abs().max())}
+            del dI
+        for m in range(0 if streamed else 4):
             num = den = mx = rmx = 0.0
             step = 2000
             for r0 in range(0, s.n1, step):

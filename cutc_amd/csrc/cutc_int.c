@@ -493,8 +493,13 @@ done:
 
 /* ------------------------------------------------------------ stage 2 */
 
-static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int n_ao, const double *wr1,
-                  const double *aos1, const double *X, double *out, tracer_t *tr, cutc_stats *stats)
+/* Stage 2 over the r1 points r0 .. r0 + cnt - 1: wr1 / aos1 are the caller's
+ * full arrays (leading dimension n1), X holds the cnt points' four components
+ * with component stride cnt.  accumulate = 0: the first contraction stores
+ * into out, else everything adds; symm = 1: out = M + M^T at the end. */
+static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t r0, int64_t cnt, int n_ao,
+                  const double *wr1, const double *aos1, const double *X, double *out, int accumulate, int symm,
+                  tracer_t *tr, cutc_stats *stats)
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, n1c = pl->n1_chunk_2e;
@@ -537,21 +542,22 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
     }
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
 
-    for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
-        const int64_t n1e = (n1 - i0 < n1c) ? n1 - i0 : n1c;
+    for (int64_t i0 = 0; i0 < cnt; i0 += n1c) {
+        const int64_t n1e = (cnt - i0 < n1c) ? cnt - i0 : n1c;
+        const int first = i0 == 0 && !accumulate;
         const int64_t pad0 = n1e * P, padn = ((n1e + 15) / 16 * 16 - n1e) * P;
         ev = tr_begin(tr, CAT_INTEG);
-        rc = cutc_launch_r1_integrands(wr1, aos1, n1, i0, n1e, n_ao, pairs, Hp, NHp, P, plane, st);
+        rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, NHp, P, plane, st);
         for (int m = 0; m < 4 && rc == CUTC_OK && padn > 0; m++)
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
         tr_end(tr, ev);
         TRY(rc);
         if (!crt_mode) {
             ev = tr_begin(tr, CAT_GEMM2);
-            rc = cutc_gemm2_f64(X + (3 * n1 + i0) * nao2, nao2, Hp, P, P, n1e, 1.0, pairs, n_ao, out, i0 == 0 ? 0 : 1,
+            rc = cutc_gemm2_f64(X + (3 * cnt + i0) * nao2, nao2, Hp, P, P, n1e, 1.0, pairs, n_ao, out, first ? 0 : 1,
                                 st);
             for (int m = 0; m < 3 && rc == CUTC_OK; m++)
-                rc = cutc_gemm2_f64(X + (m * n1 + i0) * nao2, nao2, NHp + m * plane, P, P, n1e, -0.5, pairs, n_ao,
+                rc = cutc_gemm2_f64(X + (m * cnt + i0) * nao2, nao2, NHp + m * plane, P, P, n1e, -0.5, pairs, n_ao,
                                     out, 2, st);
             tr_end(tr, ev);
             TRY(rc);
@@ -575,7 +581,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
                 sa.n_ao = n_ao;
                 for (int g = 0; g < nseg; g++) {
                     const int m = part == 0 ? 3 : g;
-                    sa.p[g] = X + (m * n1 + i0) * nao2;
+                    sa.p[g] = X + (m * cnt + i0) * nao2;
                     sb.p[g] = part == 0 ? Hp : NHp + g * plane;
                 }
                 ev = tr_begin(tr, CAT_SPLIT);
@@ -594,7 +600,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
                 TRY(rc);
                 ev = tr_begin(tr, CAT_SPLIT);
                 rc = cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, part == 0 ? 1.0 : -0.5, pairs, n_ao, out,
-                                        part == 0 ? (i0 == 0 ? 0 : 1) : 2, st);
+                                        part == 0 ? (first ? 0 : 1) : 2, st);
                 tr_end(tr, ev);
                 TRY(rc);
             }
@@ -602,10 +608,12 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int 
         if (stats)
             stats->flops_exec += 2.0 * (double)P * (double)P * 4.0 * (double)n1e;
     }
-    ev = tr_begin(tr, CAT_SYMM);
-    rc = cutc_launch_trans_inplace(out, nao2, st);
-    tr_end(tr, ev);
-    TRY(rc);
+    if (symm) {
+        ev = tr_begin(tr, CAT_SYMM);
+        rc = cutc_launch_trans_inplace(out, nao2, st);
+        tr_end(tr, ev);
+        TRY(rc);
+    }
     TRYH(hipStreamSynchronize(st));
 done:
     if (rc != CUTC_OK)
@@ -697,7 +705,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             rc = cutc_launch_fill(out, (int64_t)n_ao * n_ao * n_ao * n_ao, 0.0, st);
             tr_end(&tr, ev);
         } else {
-            rc = stage2(opt, &pl, n1, n_ao, wr1, aos1, X, out, &tr, stats);
+            rc = stage2(opt, &pl, n1, 0, n1, n_ao, wr1, aos1, X, out, 0, 1, &tr, stats);
         }
     }
     tr_end(&tr, evt);
@@ -741,6 +749,132 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
     pthread_mutex_lock(&g_lock);
     int rc = device_path(opt, 3, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
                          c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats);
+    pthread_mutex_unlock(&g_lock);
+    cutc_set_last_error(rc);
+    return rc;
+}
+
+/* Streamed build: X is never stored whole.  The r1 points run in chunks of
+ * nc; each chunk's X (4 x nc x n_ao^2, one device allocation for the call)
+ * goes straight from stage 1 into stage 2, which adds it into out; out is
+ * symmetrised after the last chunk.  nc takes a quarter of the memory left
+ * over by the stages' fixed buffers (or opt->n1_chunk when smaller). */
+static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nuc, int size_bh,
+                         const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
+                         const double *aos1, const double *aos2, const double *c_bh, const int *m_bh,
+                         const int *n_bh, const int *o_bh, double *out, hipStream_t st, cutc_stats *stats)
+{
+    int rc = check_sizes(n1, n2, n_ao, n_nuc, size_bh);
+    if (rc)
+        return rc;
+    const cutc_options ropt = resolved_options(opt);
+    opt = &ropt;
+    if (!valid_precision(opt))
+        return CUTC_ERR_ARG;
+    if (n_ao > 0 && !out)
+        return CUTC_ERR_ARG;
+    if (n1 > 0 && n_ao > 0 &&
+        (!r1 || !wr1 || !aos1 ||
+         (n2 > 0 && (!r2 || !wr2 || !aos2 || (n_nuc > 0 && !rn) ||
+                     ((int64_t)n_nuc * size_bh > 0 && (!c_bh || !m_bh || !n_bh || !o_bh))))))
+        return CUTC_ERR_ARG;
+    if (stats)
+        memset(stats, 0, sizeof *stats);
+    if (n_ao == 0)
+        return CUTC_OK;
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    if (n1 == 0) {
+        rc = cutc_launch_fill(out, nao2 * nao2, 0.0, st);
+        if (rc == CUTC_OK)
+            rc = hip_rc(hipStreamSynchronize(st));
+        return rc;
+    }
+    cutc_jprog prog;
+    memset(&prog, 0, sizeof prog);
+    if (n2 > 0) {
+        rc = program_from_device(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, &prog);
+        if (rc)
+            return rc;
+    }
+    double *Xc = NULL;
+    tracer_t tr;
+    int have_tr = 0;
+    /* chunk of the stored X */
+    const size_t budget = device_budget(opt);
+    cutc_plan p1;
+    rc = cutc_make_plan(1, n2, n_ao, n_nuc, opt->precision, opt->slices, budget, 0, &p1);
+    if (rc)
+        goto done;
+    {
+        const size_t fixed = p1.bytes_stage1 > p1.bytes_stage2 ? p1.bytes_stage1 : p1.bytes_stage2;
+        const size_t xper = 32 * (size_t)nao2;
+        int64_t nc = budget > fixed ? (int64_t)((budget - fixed) / 4 / xper) : 0;
+        if (opt->n1_chunk > 0 && opt->n1_chunk < nc)
+            nc = opt->n1_chunk;
+        if (nc < 1) {
+            rc = CUTC_ERR_NOMEM;
+            goto done;
+        }
+        if (nc < n1) {
+            const int64_t nch = (n1 + nc - 1) / nc;
+            nc = (n1 + nch - 1) / nch;
+        } else {
+            nc = n1;
+        }
+        TRYH(hipMalloc((void **)&Xc, sizeof(double) * 4 * (size_t)nc * nao2));
+        cutc_plan pl;
+        TRY(cutc_make_plan(nc, n2, n_ao, n_nuc, opt->precision, opt->slices, device_budget(opt), opt->n1_chunk, &pl));
+        tr_init(&tr, st, stats != NULL || opt->verbose);
+        have_tr = 1;
+        int evt = tr_begin(&tr, CAT_TOTAL);
+        for (int64_t r0 = 0; r0 < n1 && rc == CUTC_OK; r0 += nc) {
+            const int64_t cnt = (n1 - r0 < nc) ? n1 - r0 : nc;
+            if (n2 == 0) {
+                int ev = tr_begin(&tr, CAT_GEMM1);
+                rc = cutc_launch_fill(Xc, 4 * cnt * nao2, 0.0, st);
+                tr_end(&tr, ev);
+            } else if (opt->precision == CUTC_PREC_INT8) {
+                rc = stage1_crt(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
+            } else {
+                rc = stage1(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
+            }
+            if (rc == CUTC_OK)
+                rc = stage2(opt, &pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, &tr, stats);
+        }
+        tr_end(&tr, evt);
+        if (rc == CUTC_OK)
+            rc = hip_rc(hipStreamSynchronize(st));
+        if (rc == CUTC_OK) {
+            fill_stats(stats, &tr);
+            if (stats) {
+                stats->n1_chunk = nc;
+                stats->n1_chunks = (int)((n1 + nc - 1) / nc);
+            }
+        }
+    }
+done:
+    hipStreamSynchronize(st);
+    if (have_tr)
+        tr_free(&tr);
+    dfree(Xc);
+    cutc_jast_program_free(&prog);
+    return rc;
+}
+
+int cutc_int_streamed_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
+                             const double *r1, const double *wr1, const double *r2, const double *wr2,
+                             const double *rn, const double *aos_data1, const double *aos_data2, const double *c_bh,
+                             const int *m_bh, const int *n_bh, const int *o_bh, double *int_2e_ao, void *stream,
+                             cutc_stats *stats)
+{
+    cutc_options def;
+    if (!opt) {
+        cutc_default_options(&def);
+        opt = &def;
+    }
+    pthread_mutex_lock(&g_lock);
+    int rc = streamed_path(opt, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
+                           c_bh, m_bh, n_bh, o_bh, int_2e_ao, (hipStream_t)stream, stats);
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     return rc;

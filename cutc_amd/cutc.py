@@ -71,7 +71,7 @@ HEADER_SYMBOLS = [
     "cutc_int", "get_int2_grad1_u12_ao", "get_int_2e_ao", "checkCudaErrors", "checkCublasErrors", "cutc_int_bh",
     "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
     "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
-    "cutc_release_workspace",
+    "cutc_release_workspace", "cutc_int_streamed_device",
 ]
 
 _vp = ctypes.c_void_p
@@ -88,6 +88,9 @@ def lib():
         L.cutc_int.argtypes = [_ci] * 11 + [_vp] * 13
         L.cutc_int_ex.restype = _ci
         L.cutc_int_ex.argtypes = [ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [ctypes.POINTER(Stats)]
+        L.cutc_int_streamed_device.restype = _ci
+        L.cutc_int_streamed_device.argtypes = ([ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 12 + [_vp]
+                                               + [ctypes.POINTER(Stats)])
         L.cutc_int_device.restype = _ci
         L.cutc_int_device.argtypes = ([ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [_vp]
                                       + [ctypes.POINTER(Stats)])
@@ -253,6 +256,21 @@ def cutc_int_device(ds, X, I, opt: Options | None = None, stream=None):
                                _tp(ds.aos2), _tp(ds.c_bh), _tp(ds.m_bh), _tp(ds.n_bh), _tp(ds.o_bh), _tp(X), _tp(I),
                                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
     _check(rc, "cutc_int_device")
+    return st.as_dict()
+
+
+def cutc_int_streamed_device(ds, I, opt: Options | None = None, stream=None):
+    """Streamed whole path: int_2e_ao only, X built and contracted r1 chunk by
+    r1 chunk (opt.n1_chunk, when set, caps the chunk).  Returns the stats dict."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    st = Stats()
+    rc = lib().cutc_int_streamed_device(ctypes.byref(opt or options()), ds.n1, ds.n2, ds.n_ao, ds.n_nuc, ds.size_bh,
+                                        _tp(ds.r1), _tp(ds.wr1), _tp(ds.r2), _tp(ds.wr2), _tp(ds.rn), _tp(ds.aos1),
+                                        _tp(ds.aos2), _tp(ds.c_bh), _tp(ds.m_bh), _tp(ds.n_bh), _tp(ds.o_bh),
+                                        _tp(I), ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+    _check(rc, "cutc_int_streamed_device")
     return st.as_dict()
 
 

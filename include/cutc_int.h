@@ -161,6 +161,17 @@ int cutc_int_ex(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int
                 const int *h_n_bh, const int *h_o_bh, double *h_int2_grad1_u12_ao, double *h_int_2e_ao,
                 cutc_stats *stats);
 
+/* Streamed whole path on device pointers: the same inputs and int_2e_ao as
+ * cutc_int_device, without int2_grad1_u12_ao.  X is built and contracted r1
+ * chunk by r1 chunk (one chunk buffer per call), so only int_2e_ao (n_ao^4
+ * doubles) has to fit beside the working set.  No reference counterpart: it
+ * serves workloads whose X (4 n_ao^2 n_grid1 doubles) does not fit in HBM. */
+int cutc_int_streamed_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
+                             const double *r1, const double *wr1, const double *r2, const double *wr2,
+                             const double *rn, const double *aos_data1, const double *aos_data2, const double *c_bh,
+                             const int *m_bh, const int *n_bh, const int *o_bh, double *int_2e_ao, void *stream,
+                             cutc_stats *stats);
+
 /* Memory plan (pure host function): r1 chunk for the given device budget.
  * Returns 0 or CUTC_ERR_NOMEM. */
 typedef struct cutc_plan {
