@@ -148,6 +148,8 @@ static int valid_precision(const cutc_options *o)
         return 1;
     if (o->precision == CUTC_PREC_INT8 && o->slices >= 6 && o->slices <= CUTC_CRT_MAX)
         return 1;
+    if ((o->precision == CUTC_PREC_BF16 || o->precision == CUTC_PREC_FP16) && o->slices >= 2 && o->slices <= 4)
+        return 1;
     return 0;
 }
 
@@ -157,6 +159,8 @@ static cutc_options resolved_options(const cutc_options *in)
     cutc_options o = *in;
     if (o.precision == CUTC_PREC_INT8 && o.slices == 0)
         o.slices = CUTC_INT8_DEFAULT_MODULI;
+    if ((o.precision == CUTC_PREC_BF16 || o.precision == CUTC_PREC_FP16) && o.slices == 0)
+        o.slices = 4;
     return o;
 }
 
@@ -398,12 +402,14 @@ done:
     return rc;
 }
 
-/* Stage 1 with the modular (CRT) contraction: A quantised on the fly from
- * w2 and chi, each G chunk quantised after the Jastrow block, one int8 GEMM
- * per modulus, CRT rebuild scattered into X. */
-static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
-                      const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
-                      const cutc_jprog *prog, double *X, tracer_t *tr, cutc_stats *stats)
+/* Stage 1 with a split-precision contraction.  INT8: the modular (CRT)
+ * scheme, pair residues quantised on the fly from w2 and chi, each G chunk
+ * quantised after the Jastrow block, one int8 GEMM per modulus, CRT rebuild
+ * scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
+ * pairs and of each G chunk, one slice GEMM into FP64, scattered into X. */
+static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
+                        const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
+                        const cutc_jprog *prog, double *X, tracer_t *tr, cutc_stats *stats)
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
@@ -414,10 +420,16 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
     int *rexpA = NULL, *rexpG = NULL;
     int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
+    const int slice = opt->precision != CUTC_PREC_INT8, f16 = opt->precision == CUTC_PREC_FP16;
+    const int emode = slice ? CUTC_ROWEXP_MAX : CUTC_ROWEXP_NORM;
     cutc_crt crt;
-    TRY(cutc_crt_setup(n2, nmod, &crt));
-    const size_t bA = (size_t)nmod * CUTC_PK_ROWS_B(P) * ldk, bG = (size_t)nmod * CUTC_PK_ROWS_A(4 * n1c) * ldk;
-    const size_t bC = (size_t)nmod * 4 * n1c * P;
+    if (!slice)
+        TRY(cutc_crt_setup(n2, nmod, &crt));
+    /* int8: residue planes and int8 results; slices: 16-bit digit planes and FP64 results */
+    const size_t bA = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * ldk : (size_t)nmod * CUTC_PK_ROWS_B(P) * ldk;
+    const size_t bG = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(4 * n1c) * ldk
+                            : (size_t)nmod * CUTC_PK_ROWS_A(4 * n1c) * ldk;
+    const size_t bC = slice ? sizeof(double) * 4 * n1c * P : (size_t)nmod * 4 * n1c * P;
 
     ws_t ws;
     {
@@ -444,9 +456,11 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_SPLIT);
-    rc = cutc_crt_rownorm_pairw(wr2, aos2, n2, n_ao, rexpA, st);
+    rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpA, st);
     if (rc == CUTC_OK)
-        rc = cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rexpA, Apl, st);
+        rc = slice ? cutc_oz_slice_pairw(nmod, f16, wr2, aos2, n2, n_ao, ldk, CUTC_OZ_ROWS(P), rexpA, (uint16_t *)Apl,
+                                         st)
+                   : cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rexpA, Apl, st);
     tr_end(tr, ev);
     TRY(rc);
 
@@ -462,17 +476,22 @@ static int stage1_crt(const cutc_options *opt, const cutc_plan *pl, int64_t n1, 
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
-        rc = cutc_crt_rownorm_k(G, ldk, 4 * n1e, n2, rexpG, st);
+        rc = cutc_rowexp_k(G, ldk, 4 * n1e, n2, emode, rexpG, st);
         if (rc == CUTC_OK)
-            rc = cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rexpG, Gpl, st);
+            rc = slice ? cutc_oz_slice_k(nmod, f16, G, ldk, 4 * n1e, n2, ldk, CUTC_OZ_ROWS(4 * n1e), rexpG,
+                                         (uint16_t *)Gpl, st)
+                       : cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rexpG, Gpl, st);
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_GEMM1);
-        rc = cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, ldk, Cpl, st);
+        rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
+                                  rexpA, (double *)Cpl, st)
+                   : cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, ldk, Cpl, st);
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
-        rc = cutc_crt_rebuild_x(&crt, Cpl, P, 4 * n1e, rexpA, rexpG, pairs, n_ao, X, n1, n1e, i0, st);
+        rc = slice ? cutc_oz_scatter_x((const double *)Cpl, P, 4 * n1e, pairs, n_ao, X, n1, n1e, i0, st)
+                   : cutc_crt_rebuild_x(&crt, Cpl, P, 4 * n1e, rexpA, rexpG, pairs, n_ao, X, n1, n1e, i0, st);
         tr_end(tr, ev);
         TRY(rc);
         if (stats)
@@ -510,19 +529,23 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     double *npart = NULL;
     int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
-    const int crt_mode = opt->precision == CUTC_PREC_INT8;
+    const int split = opt->precision != CUTC_PREC_FP64;
+    const int slice = split && opt->precision != CUTC_PREC_INT8, f16 = opt->precision == CUTC_PREC_FP16;
+    const int emode = slice ? CUTC_ROWEXP_MAX : CUTC_ROWEXP_NORM;
     const int nmod = opt->slices;
 
     const int64_t rows = (n1c + 15) / 16 * 16;
     const int64_t plane = rows * P;
     const int64_t kp_max = (n1c + CUTC_PK_KALIGN - 1) / CUTC_PK_KALIGN * CUTC_PK_KALIGN;
     ws_t ws;
-    const size_t bpl = (size_t)nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
-    const size_t bcp = (size_t)nmod * P * P;
+    /* operand planes of both sides and the contraction's results */
+    const size_t bpl = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * 3 * kp_max
+                             : (size_t)nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
+    const size_t bcp = slice ? sizeof(double) * P * P : (size_t)nmod * P * P;
     {
         size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
                       ws_round(sizeof(double) * 3 * plane);
-        if (crt_mode)
+        if (split)
             need += 2 * ws_round(sizeof(int) * P) + ws_round(sizeof(double) * CUTC_CRT_KSPLIT * P) + 2 * ws_round(bpl) +
                     ws_round(bcp);
         TRY(ws_begin(&ws, need));
@@ -532,7 +555,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     WS_TAKE(pairs, sizeof(int) * 2 * P);
     WS_TAKE(Hp, sizeof(double) * plane);
     WS_TAKE(NHp, sizeof(double) * 3 * plane);
-    if (crt_mode) {
+    if (split) {
         WS_TAKE(rexpA, sizeof(int) * P);
         WS_TAKE(rexpB, sizeof(int) * P);
         WS_TAKE(npart, sizeof(double) * CUTC_CRT_KSPLIT * P);
@@ -552,7 +575,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
         tr_end(tr, ev);
         TRY(rc);
-        if (!crt_mode) {
+        if (!split) {
             ev = tr_begin(tr, CAT_GEMM2);
             rc = cutc_gemm2_f64(X + (3 * cnt + i0) * nao2, nao2, Hp, P, P, n1e, 1.0, pairs, n_ao, out, first ? 0 : 1,
                                 st);
@@ -569,7 +592,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
             for (int part = 0; part < 2; part++) {
                 const int nseg = part == 0 ? 1 : 3;
                 cutc_crt crt;
-                TRY(cutc_crt_setup(nseg * n1e, nmod, &crt));
+                if (!slice)
+                    TRY(cutc_crt_setup(nseg * n1e, nmod, &crt));
                 cutc_colsrc sa, sb;
                 memset(&sa, 0, sizeof sa);
                 memset(&sb, 0, sizeof sb);
@@ -585,22 +609,28 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                     sb.p[g] = part == 0 ? Hp : NHp + g * plane;
                 }
                 ev = tr_begin(tr, CAT_SPLIT);
-                rc = cutc_crt_rownorm_colk(&sa, P, npart, rexpA, st);
+                rc = cutc_rowexp_colk(&sa, P, emode, npart, rexpA, st);
                 if (rc == CUTC_OK)
-                    rc = cutc_crt_quant_colk_pk(&crt, &sa, P, kp, CUTC_PK_ROWS_B(P), rexpA, Apl, st);
+                    rc = slice ? cutc_oz_slice_colk(nmod, f16, &sa, P, kp, CUTC_OZ_ROWS(P), rexpA, (uint16_t *)Apl, st)
+                               : cutc_crt_quant_colk_pk(&crt, &sa, P, kp, CUTC_PK_ROWS_B(P), rexpA, Apl, st);
                 if (rc == CUTC_OK)
-                    rc = cutc_crt_rownorm_colk(&sb, P, npart, rexpB, st);
+                    rc = cutc_rowexp_colk(&sb, P, emode, npart, rexpB, st);
                 if (rc == CUTC_OK)
-                    rc = cutc_crt_quant_colk_pk(&crt, &sb, P, kp, CUTC_PK_ROWS_A(P), rexpB, Bpl, st);
+                    rc = slice ? cutc_oz_slice_colk(nmod, f16, &sb, P, kp, CUTC_OZ_ROWS(P), rexpB, (uint16_t *)Bpl, st)
+                               : cutc_crt_quant_colk_pk(&crt, &sb, P, kp, CUTC_PK_ROWS_A(P), rexpB, Bpl, st);
                 tr_end(tr, ev);
                 TRY(rc);
                 ev = tr_begin(tr, CAT_GEMM2);
-                rc = cutc_crt_gemm_pk(&crt, Bpl, P, Apl, P, nseg * kp, Cpl, st);
+                rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Bpl, P, (const uint16_t *)Apl, P, nseg * kp, rexpB,
+                                          rexpA, (double *)Cpl, st)
+                           : cutc_crt_gemm_pk(&crt, Bpl, P, Apl, P, nseg * kp, Cpl, st);
                 tr_end(tr, ev);
                 TRY(rc);
                 ev = tr_begin(tr, CAT_SPLIT);
-                rc = cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, part == 0 ? 1.0 : -0.5, pairs, n_ao, out,
-                                        part == 0 ? (first ? 0 : 1) : 2, st);
+                const double alpha = part == 0 ? 1.0 : -0.5;
+                const int mode = part == 0 ? (first ? 0 : 1) : 2;
+                rc = slice ? cutc_oz_scatter_m((const double *)Cpl, P, alpha, pairs, n_ao, out, mode, st)
+                           : cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, alpha, pairs, n_ao, out, mode, st);
                 tr_end(tr, ev);
                 TRY(rc);
             }
@@ -693,8 +723,8 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             int ev = tr_begin(&tr, CAT_GEMM1);
             rc = cutc_launch_fill(X, 4 * (int64_t)n1 * n_ao * n_ao, 0.0, st);
             tr_end(&tr, ev);
-        } else if (opt->precision == CUTC_PREC_INT8) {
-            rc = stage1_crt(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
+        } else if (opt->precision != CUTC_PREC_FP64) {
+            rc = stage1_split(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
         } else {
             rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
         }
@@ -833,8 +863,8 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
                 int ev = tr_begin(&tr, CAT_GEMM1);
                 rc = cutc_launch_fill(Xc, 4 * cnt * nao2, 0.0, st);
                 tr_end(&tr, ev);
-            } else if (opt->precision == CUTC_PREC_INT8) {
-                rc = stage1_crt(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
+            } else if (opt->precision != CUTC_PREC_FP64) {
+                rc = stage1_split(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
             } else {
                 rc = stage1(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
             }

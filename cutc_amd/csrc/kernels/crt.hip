@@ -5,7 +5,8 @@
 // the rebuild formula in crt_host.c).
 //
 // Kernels:
-//   rownorm_*       row exponents e with ||x_row||_2 < 2^e (fixed-order sums)
+//   rowexp_*        row exponents e with ||x_row||_2 < 2^e or max|x_row| < 2^e
+//                   (fixed-order reductions)
 //   quant_*_pk      x -> q = rint(x 2^(bits - e_row)) -> residues q mod m_i,
 //                   int8 planes in the fragment-packed layout (launch.h)
 //   crt_gemm_pk     per modulus: 512 x 128 block tile, 4 waves of 128 x 128
@@ -17,6 +18,7 @@
 //                   epilogues)
 #include "common.h"
 #include "launch.h"
+#include "split_common.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -46,86 +48,99 @@ __device__ static inline int norm_exponent(double ss)
     return ilogb(sqrt(ss * (1.0 + 0x1p-29))) + 1;
 }
 
-// block-wide sum of v (256 threads) in a fixed order, returned to thread 0
-__device__ static inline double block_sum(double v)
+// Row exponent e with max |x| < 2^e (0 for a zero row)
+__device__ static inline int max_exponent(double mx) { return (mx > 0.0) ? ilogb(mx) + 1 : 0; }
+
+// Row statistics: MODE 0 sums squares (2-norm exponents), MODE 1 takes the
+// largest magnitude (max exponents); both reductions run in a fixed order.
+template <int MODE> __device__ static inline double rs_add(double v, double a)
+{
+    return MODE == 0 ? fma(a, a, v) : fmax(v, fabs(a));
+}
+template <int MODE> __device__ static inline double rs_join(double u, double v)
+{
+    return MODE == 0 ? u + v : fmax(u, v);
+}
+template <int MODE> __device__ static inline int rs_exponent(double v)
+{
+    return MODE == 0 ? norm_exponent(v) : max_exponent(v);
+}
+
+// block-wide reduction of v (256 threads) in a fixed order, returned to thread 0
+template <int MODE> __device__ static inline double block_reduce(double v)
 {
     __shared__ double red[4];
     for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
+        v = rs_join<MODE>(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0)
         red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0)
-        v = (red[0] + red[1]) + (red[2] + red[3]);
+        v = rs_join<MODE>(rs_join<MODE>(red[0], red[1]), rs_join<MODE>(red[2], red[3]));
     return v;
 }
 
-// ---------------------------------------------------------------- row norms
+// ---------------------------------------------------------------- row exponents
 
-__global__ __launch_bounds__(256) void rownorm_k_kernel(const double *__restrict__ X, int64_t ld, int64_t K,
-                                                        int *rexp)
+template <int MODE>
+__global__ __launch_bounds__(256) void rowexp_k_kernel(const double *__restrict__ X, int64_t ld, int64_t K, int *rexp)
 {
     const double *row = X + (int64_t)blockIdx.x * ld;
     double v = 0.0;
     for (int64_t k = threadIdx.x; k < K; k += 256)
-        v = fma(row[k], row[k], v);
-    v = block_sum(v);
+        v = rs_add<MODE>(v, row[k]);
+    v = block_reduce<MODE>(v);
     if (threadIdx.x == 0)
-        rexp[blockIdx.x] = norm_exponent(v);
+        rexp[blockIdx.x] = rs_exponent<MODE>(v);
 }
 
-extern "C" int cutc_crt_rownorm_k(const double *X, int64_t ld, int64_t rows, int64_t K, int *rexp, hipStream_t st)
+extern "C" int cutc_rowexp_k(const double *X, int64_t ld, int64_t rows, int64_t K, int mode, int *rexp,
+                             hipStream_t st)
 {
     if (rows <= 0)
         return 0;
     if (rows > 2147483647)
         return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(rownorm_k_kernel, dim3((unsigned)rows), dim3(256), 0, st, X, ld, K, rexp);
+    if (mode == CUTC_ROWEXP_MAX)
+        hipLaunchKernelGGL(rowexp_k_kernel<1>, dim3((unsigned)rows), dim3(256), 0, st, X, ld, K, rexp);
+    else
+        hipLaunchKernelGGL(rowexp_k_kernel<0>, dim3((unsigned)rows), dim3(256), 0, st, X, ld, K, rexp);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
 // row p = (k, l) of A[p][r2] = (w chi_l) chi_k
-__global__ __launch_bounds__(256) void rownorm_pairw_kernel(const double *__restrict__ wr2,
-                                                            const double *__restrict__ aos2, int64_t n2, int n_ao,
-                                                            int *rexp)
+template <int MODE>
+__global__ __launch_bounds__(256) void rowexp_pairw_kernel(const double *__restrict__ wr2,
+                                                           const double *__restrict__ aos2, int64_t n2, int *rexp)
 {
     const int64_t p = blockIdx.x;
-    int l = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(l + 1) * (l + 2) / 2 <= p)
-        l++;
-    while ((int64_t)l * (l + 1) / 2 > p)
-        l--;
-    const int k = (int)(p - (int64_t)l * (l + 1) / 2);
+    int k, l;
+    pair_of(p, k, l);
     double v = 0.0;
-    for (int64_t i = threadIdx.x; i < n2; i += 256) {
-        const double a = wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k];
-        v = fma(a, a, v);
-    }
-    v = block_sum(v);
+    for (int64_t i = threadIdx.x; i < n2; i += 256)
+        v = rs_add<MODE>(v, wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k]);
+    v = block_reduce<MODE>(v);
     if (threadIdx.x == 0)
-        rexp[p] = norm_exponent(v);
+        rexp[p] = rs_exponent<MODE>(v);
 }
 
-extern "C" int cutc_crt_rownorm_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int *rexp,
-                                      hipStream_t st)
+extern "C" int cutc_rowexp_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int mode, int *rexp,
+                                 hipStream_t st)
 {
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     if (P <= 0)
         return 0;
-    hipLaunchKernelGGL(rownorm_pairw_kernel, dim3((unsigned)P), dim3(256), 0, st, wr2, aos2, n2, n_ao, rexp);
+    if (mode == CUTC_ROWEXP_MAX)
+        hipLaunchKernelGGL(rowexp_pairw_kernel<1>, dim3((unsigned)P), dim3(256), 0, st, wr2, aos2, n2, rexp);
+    else
+        hipLaunchKernelGGL(rowexp_pairw_kernel<0>, dim3((unsigned)P), dim3(256), 0, st, wr2, aos2, n2, rexp);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-__device__ static inline int64_t colsrc_col(const cutc_colsrc &s, int64_t r)
-{
-    if (s.pairs)
-        return s.pairs[2 * r] + (int64_t)s.n_ao * s.pairs[2 * r + 1];
-    return r;
-}
-
-// grid (ceil(rows/256), ks): block y sums the squares of its K slice of every
-// segment into partial[y][r]; the slices are then added in order
-__global__ __launch_bounds__(256) void rownorm_colk_kernel(cutc_colsrc s, int64_t rows, double *partial)
+// grid (ceil(rows/256), ks): block y reduces its K slice of every segment
+// into partial[y][r]; the slices are then joined in order
+template <int MODE>
+__global__ __launch_bounds__(256) void rowexp_colk_kernel(cutc_colsrc s, int64_t rows, double *partial)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= rows)
@@ -136,28 +151,27 @@ __global__ __launch_bounds__(256) void rownorm_colk_kernel(cutc_colsrc s, int64_
     double v = 0.0;
     for (int seg = 0; seg < s.nsrc; seg++) {
         const double *base = s.p[seg] + col;
-        for (int64_t k = k0; k < k1; k++) {
-            const double a = base[k * s.ld];
-            v = fma(a, a, v);
-        }
+        for (int64_t k = k0; k < k1; k++)
+            v = rs_add<MODE>(v, base[k * s.ld]);
     }
     partial[(int64_t)blockIdx.y * rows + r] = v;
 }
 
-__global__ __launch_bounds__(256) void rownorm_finish_kernel(const double *__restrict__ partial, int ks, int64_t rows,
-                                                             int *rexp)
+template <int MODE>
+__global__ __launch_bounds__(256) void rowexp_finish_kernel(const double *__restrict__ partial, int ks, int64_t rows,
+                                                            int *rexp)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= rows)
         return;
     double v = 0.0;
     for (int y = 0; y < ks; y++)
-        v += partial[(int64_t)y * rows + r];
-    rexp[r] = norm_exponent(v);
+        v = rs_join<MODE>(v, partial[(int64_t)y * rows + r]);
+    rexp[r] = rs_exponent<MODE>(v);
 }
 
-extern "C" int cutc_crt_rownorm_colk(const cutc_colsrc *src, int64_t rows, double *partial, int *rexp,
-                                     hipStream_t st)
+extern "C" int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, double *partial, int *rexp,
+                                hipStream_t st)
 {
     if (rows <= 0)
         return 0;
@@ -166,10 +180,14 @@ extern "C" int cutc_crt_rownorm_colk(const cutc_colsrc *src, int64_t rows, doubl
         ks = CUTC_CRT_KSPLIT;
     if (ks < 1)
         ks = 1;
-    hipLaunchKernelGGL(rownorm_colk_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)ks), dim3(256), 0, st,
-                       *src, rows, partial);
-    hipLaunchKernelGGL(rownorm_finish_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, partial, (int)ks,
-                       rows, rexp);
+    const dim3 g1((unsigned)((rows + 255) / 256), (unsigned)ks), g2((unsigned)((rows + 255) / 256));
+    if (mode == CUTC_ROWEXP_MAX) {
+        hipLaunchKernelGGL(rowexp_colk_kernel<1>, g1, dim3(256), 0, st, *src, rows, partial);
+        hipLaunchKernelGGL(rowexp_finish_kernel<1>, g2, dim3(256), 0, st, partial, (int)ks, rows, rexp);
+    } else {
+        hipLaunchKernelGGL(rowexp_colk_kernel<0>, g1, dim3(256), 0, st, *src, rows, partial);
+        hipLaunchKernelGGL(rowexp_finish_kernel<0>, g2, dim3(256), 0, st, partial, (int)ks, rows, rexp);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -257,12 +275,8 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     double q[16];
     if (p < P) {
-        int l = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-        while ((int64_t)(l + 1) * (l + 2) / 2 <= p)
-            l++;
-        while ((int64_t)l * (l + 1) / 2 > p)
-            l--;
-        const int k = (int)(p - (int64_t)l * (l + 1) / 2);
+        int k, l;
+        pair_of(p, k, l);
         const int sh = c.bits - rexp[p];
 #pragma unroll
         for (int e = 0; e < 16; e++) {

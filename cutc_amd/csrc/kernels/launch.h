@@ -107,11 +107,16 @@ typedef struct cutc_colsrc {
     int n_ao;
 } cutc_colsrc;
 
-int cutc_crt_rownorm_k(const double *X, int64_t ld, int64_t rows, int64_t K, int *rexp, hipStream_t st);
-int cutc_crt_rownorm_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int *rexp, hipStream_t st);
+/* Row exponents: CUTC_ROWEXP_NORM gives e with ||x_row||_2 < 2^e (the
+ * modular path), CUTC_ROWEXP_MAX e with max |x_row| < 2^e (the slice path). */
+#define CUTC_ROWEXP_NORM 0
+#define CUTC_ROWEXP_MAX 1
+int cutc_rowexp_k(const double *X, int64_t ld, int64_t rows, int64_t K, int mode, int *rexp, hipStream_t st);
+int cutc_rowexp_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int mode, int *rexp,
+                      hipStream_t st);
 /* partial: CUTC_CRT_KSPLIT x rows doubles of scratch */
 #define CUTC_CRT_KSPLIT 64
-int cutc_crt_rownorm_colk(const cutc_colsrc *src, int64_t rows, double *partial, int *rexp, hipStream_t st);
+int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, double *partial, int *rexp, hipStream_t st);
 /* Fragment-packed residue planes of the int8 contractions.  Rows are
  * grouped in blocks of 32 and K in steps of 64 bytes; the 2048-byte tile of
  * (row block rb, step kt) sits at (rb * KT + kt) * 2048 (KT = kpad / 64) and
@@ -140,6 +145,25 @@ int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t ro
  * A side is contiguous over p). */
 int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
                      int8_t *C, hipStream_t st);
+/* Slice (Ozaki scheme I) contractions, ozaki1.hip: S = 2..4 digit planes of
+ * 16-bit elements (bf16 or fp16 bit patterns, f16 != 0), rows padded to
+ * CUTC_OZ_ROWS, kpad a multiple of CUTC_PK_KALIGN.  Row exponents from
+ * cutc_rowexp_* with CUTC_ROWEXP_MAX. */
+#define CUTC_OZ_ROWS(rows) ((((int64_t)(rows)) + 127) / 128 * 128)
+int cutc_oz_slice_k(int S, int f16, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
+                    int64_t plane_rows, const int *rexp, uint16_t *planes, hipStream_t st);
+int cutc_oz_slice_pairw(int S, int f16, const double *wr2, const double *aos2, int64_t n2, int n_ao, int64_t kpad,
+                        int64_t plane_rows, const int *rexp, uint16_t *planes, hipStream_t st);
+int cutc_oz_slice_colk(int S, int f16, const cutc_colsrc *src, int64_t rows, int64_t kp, int64_t plane_rows,
+                       const int *rexp, uint16_t *planes, hipStream_t st);
+/* C[r][c] (FP64, row-major, ld N) = sum_k A[r][k] B[c][k] from the digit planes */
+int cutc_oz_gemm(int S, int f16, const uint16_t *A, int64_t M, const uint16_t *B, int64_t N, int64_t kpad,
+                 const int *eA, const int *eB, double *C, hipStream_t st);
+/* scatter of FP64 contraction results, as cutc_crt_rebuild_x / _m */
+int cutc_oz_scatter_x(const double *C, int64_t P, int64_t N, const int *pairs, int n_ao, double *X, int64_t n1,
+                      int64_t n1c, int64_t i0, hipStream_t st);
+int cutc_oz_scatter_m(const double *C, int64_t P, double alpha, const int *pairs, int n_ao, double *M, int mode,
+                      hipStream_t st);
 /* Rebuild C = V 2^(eA + eB - 2 bits) and scatter into X (4, n1, n_ao, n_ao):
  * column j = m n1c + i1 -> X[m][i0 + i1][(k,l) and (l,k)]. */
 int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N, const int *rexpA,

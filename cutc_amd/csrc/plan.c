@@ -40,9 +40,10 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     if (n1 < 0 || n2 < 0 || n_ao < 0 || n_nuc < 0 || !pl)
         return CUTC_ERR_ARG;
     const int crt = precision == CUTC_PREC_INT8;
-    const int64_t nmod = crt ? slices : 0;
+    const int slice = precision == CUTC_PREC_BF16 || precision == CUTC_PREC_FP16;
+    const int64_t nmod = (crt || slice) ? slices : 0;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
-    const int64_t ldk = rup(n2, crt ? CUTC_PK_KALIGN : 64);
+    const int64_t ldk = rup(n2, (crt || slice) ? CUTC_PK_KALIGN : 64);
     pl->n_pairs = P;
     pl->ldk = ldk;
 
@@ -51,6 +52,9 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     if (crt) {
         fixed1 = 16 * P + nmod * CUTC_PK_ROWS_B(P) * ldk + nmod * 511 * ldk + 4096;
         per_r1 = 32 * ldk + 32 + 4 * nmod * ldk + nmod * 4 * P;
+    } else if (slice) { /* digit planes of 2-byte elements, FP64 results */
+        fixed1 = 16 * P + 2 * nmod * CUTC_OZ_ROWS(P) * ldk + 2 * nmod * 127 * ldk + 4096;
+        per_r1 = 32 * ldk + 32 + 8 * nmod * ldk + 32 * P;
     } else {
         fixed1 = 8 * P + 8 * P * ldk + 4096;
         per_r1 = 32 * ldk;
@@ -77,6 +81,10 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     if (crt) {
         fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + nmod * P * P + 2 * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1);
         per_r1b += 2 * nmod * 3 * CUTC_PK_ROWS_A(P);
+    } else if (slice) {
+        fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + 8 * P * P +
+                  2 * 2 * nmod * CUTC_OZ_ROWS(P) * 3 * (CUTC_PK_KALIGN - 1);
+        per_r1b += 2 * 2 * nmod * 3 * CUTC_OZ_ROWS(P);
     }
     int64_t n1c2 = 0;
     if (n1 > 0 && P > 0) {
