@@ -16,10 +16,12 @@
  * errors are returned instead of exit(0); kernels run back to back on one
  * stream with timing events read once at the end.
  */
+#define _POSIX_C_SOURCE 200809L /* clock_gettime */
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <hip/hip_runtime_api.h>
 
@@ -34,6 +36,7 @@ enum { CAT_JAST = 0, CAT_INTEG, CAT_GEMM1, CAT_GEMM2, CAT_SYMM, CAT_SPLIT, CAT_T
 typedef struct {
     hipStream_t st;
     int timing;
+    int capture; /* recording into a graph: no stream syncs, no timing events */
     int nev;
     int cap;
     hipEvent_t *ev; /* pairs (start, stop) */
@@ -199,6 +202,13 @@ static int program_from_device(int n_nuc, int size_bh, const double *dc, const i
     return rc;
 }
 
+/* A Jastrow program with its device copies (monomials and classes). */
+typedef struct {
+    cutc_jprog h;
+    cutc_jmono *dmono;
+    int *dcls;
+} jprog_t;
+
 /* Device copies of a program's monomials and classes. */
 static int program_upload(const cutc_jprog *prog, cutc_jmono **dmono, int **dcls)
 {
@@ -212,6 +222,30 @@ static int program_upload(const cutc_jprog *prog, cutc_jmono **dmono, int **dcls
     if (rc == CUTC_OK && prog->n_cls > 0)
         rc = hip_rc(hipMemcpy(*dcls, prog->cls, sizeof(int) * 2 * prog->n_cls, hipMemcpyHostToDevice));
     return rc;
+}
+
+/* Program of a system: decoded from the device tables and uploaded (need = 0:
+ * nothing, for calls that run no Jastrow). */
+static int jprog_load(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh,
+                      int need, jprog_t *jp)
+{
+    memset(jp, 0, sizeof *jp);
+    if (!need)
+        return CUTC_OK;
+    int rc = program_from_device(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, &jp->h);
+    if (rc == CUTC_OK)
+        rc = program_upload(&jp->h, &jp->dmono, &jp->dcls);
+    return rc;
+}
+
+static void dfree(void *p);
+
+static void jprog_free(jprog_t *jp)
+{
+    dfree(jp->dmono);
+    dfree(jp->dcls);
+    cutc_jast_program_free(&jp->h);
+    memset(jp, 0, sizeof *jp);
 }
 
 static size_t arena_size_current(void);
@@ -264,12 +298,17 @@ static size_t arena_size_current(void)
     return g_arena[dev].size;
 }
 
+/* set while a graph is being recorded: the arena may not be reallocated */
+static int g_ws_frozen = 0;
+
 static int ws_begin(ws_t *w, size_t need)
 {
     int dev = 0;
     memset(w, 0, sizeof *w);
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
         return CUTC_ERR_HIP;
+    if (g_arena[dev].size < need && g_ws_frozen)
+        return CUTC_ERR_NOMEM;
     if (g_arena[dev].size < need) {
         if (g_arena[dev].base) {
             hipDeviceSynchronize();
@@ -335,12 +374,14 @@ void cutc_release_workspace(void)
 
 static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                   const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
-                  const cutc_jprog *prog, double *X, tracer_t *tr, cutc_stats *stats)
+                  const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats)
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
-    int *pairs = NULL, *dcls = NULL;
-    cutc_jmono *dmono = NULL;
+    const cutc_jprog *prog = &jp->h;
+    const cutc_jmono *dmono = jp->dmono;
+    const int *dcls = jp->dcls;
+    int *pairs = NULL;
     double *Apk = NULL, *G = NULL, *f2tab = NULL, *tab = NULL;
     int rc = CUTC_OK, ev;
     (void)opt;
@@ -361,7 +402,6 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     if (n_nuc > 32)
         WS_TAKE(f2tab, sizeof(double) * n_nuc * ldk);
     WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
-    TRY(program_upload(prog, &dmono, &dcls));
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_INTEG);
@@ -393,12 +433,11 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         stats->n1_chunk = n1c;
         stats->n1_chunks = nchunks;
     }
-    TRYH(hipStreamSynchronize(st));
+    if (!tr->capture)
+        TRYH(hipStreamSynchronize(st));
 done:
-    if (rc != CUTC_OK)
+    if (rc != CUTC_OK && !tr->capture)
         hipStreamSynchronize(st);
-    dfree(dmono);
-    dfree(dcls);
     return rc;
 }
 
@@ -409,13 +448,15 @@ done:
  * pairs and of each G chunk, one slice GEMM into FP64, scattered into X. */
 static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                         const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
-                        const cutc_jprog *prog, double *X, tracer_t *tr, cutc_stats *stats)
+                        const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats)
 {
+    const cutc_jprog *prog = &jp->h;
+    const cutc_jmono *dmono = jp->dmono;
+    const int *dcls = jp->dcls;
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
     const int nmod = opt->slices;
-    int *pairs = NULL, *dcls = NULL;
-    cutc_jmono *dmono = NULL;
+    int *pairs = NULL;
     double *G = NULL, *f2tab = NULL, *tab = NULL;
     int *rexpA = NULL, *rexpG = NULL;
     int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
@@ -452,7 +493,6 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     if (n_nuc > 32)
         WS_TAKE(f2tab, sizeof(double) * n_nuc * ldk);
     WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
-    TRY(program_upload(prog, &dmono, &dcls));
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_SPLIT);
@@ -501,12 +541,11 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         stats->n1_chunk = n1c;
         stats->n1_chunks = nchunks;
     }
-    TRYH(hipStreamSynchronize(st));
+    if (!tr->capture)
+        TRYH(hipStreamSynchronize(st));
 done:
-    if (rc != CUTC_OK)
+    if (rc != CUTC_OK && !tr->capture)
         hipStreamSynchronize(st);
-    dfree(dmono);
-    dfree(dcls);
     return rc;
 }
 
@@ -644,14 +683,22 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         tr_end(tr, ev);
         TRY(rc);
     }
-    TRYH(hipStreamSynchronize(st));
+    if (!tr->capture)
+        TRYH(hipStreamSynchronize(st));
 done:
-    if (rc != CUTC_OK)
+    if (rc != CUTC_OK && !tr->capture)
         hipStreamSynchronize(st);
     return rc;
 }
 
 /* --------------------------------------------------------- entry points */
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
 
 static int check_sizes(int n1, int n2, int n_ao, int n_nuc, int size_bh)
 {
@@ -700,17 +747,16 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         return CUTC_ERR_ARG;
     if ((stages & 2) && n1 > 0 && n_ao > 0 && (!wr1 || !aos1))
         return CUTC_ERR_ARG;
-    cutc_jprog prog;
-    memset(&prog, 0, sizeof prog);
-    if ((stages & 1) && n1 > 0 && n2 > 0 && n_ao > 0) {
-        rc = program_from_device(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, &prog);
-        if (rc)
-            return rc;
+    jprog_t prog;
+    rc = jprog_load(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, (stages & 1) && n1 > 0 && n2 > 0 && n_ao > 0, &prog);
+    if (rc) {
+        jprog_free(&prog);
+        return rc;
     }
     cutc_plan pl;
     rc = cutc_make_plan(n1, n2, n_ao, n_nuc, opt->precision, opt->slices, device_budget(opt), opt->n1_chunk, &pl);
     if (rc) {
-        cutc_jast_program_free(&prog);
+        jprog_free(&prog);
         return rc;
     }
     if (stats)
@@ -761,7 +807,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         }
     }
     tr_free(&tr);
-    cutc_jast_program_free(&prog);
+    jprog_free(&prog);
     return rc;
 }
 
@@ -781,6 +827,33 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
                          c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats);
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
+    return rc;
+}
+
+/* The r1 chunks of a streamed build (n1 > 0, n_ao > 0): stage 1 of each chunk
+ * into the chunk buffer Xc (nc points), then stage 2 adds it into out, which
+ * is symmetrised after the last chunk. */
+static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
+                           const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
+                           const double *aos1, const double *aos2, const jprog_t *jp, double *out, double *Xc,
+                           int64_t nc, tracer_t *tr, cutc_stats *stats)
+{
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    int rc = CUTC_OK;
+    for (int64_t r0 = 0; r0 < n1 && rc == CUTC_OK; r0 += nc) {
+        const int64_t cnt = (n1 - r0 < nc) ? n1 - r0 : nc;
+        if (n2 == 0) {
+            int ev = tr_begin(tr, CAT_GEMM1);
+            rc = cutc_launch_fill(Xc, 4 * cnt * nao2, 0.0, tr->st);
+            tr_end(tr, ev);
+        } else if (opt->precision != CUTC_PREC_FP64) {
+            rc = stage1_split(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats);
+        } else {
+            rc = stage1(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats);
+        }
+        if (rc == CUTC_OK)
+            rc = stage2(opt, pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, tr, stats);
+    }
     return rc;
 }
 
@@ -819,12 +892,11 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
             rc = hip_rc(hipStreamSynchronize(st));
         return rc;
     }
-    cutc_jprog prog;
-    memset(&prog, 0, sizeof prog);
-    if (n2 > 0) {
-        rc = program_from_device(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, &prog);
-        if (rc)
-            return rc;
+    jprog_t prog;
+    rc = jprog_load(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, n2 > 0, &prog);
+    if (rc) {
+        jprog_free(&prog);
+        return rc;
     }
     double *Xc = NULL;
     tracer_t tr;
@@ -857,20 +929,8 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
         tr_init(&tr, st, stats != NULL || opt->verbose);
         have_tr = 1;
         int evt = tr_begin(&tr, CAT_TOTAL);
-        for (int64_t r0 = 0; r0 < n1 && rc == CUTC_OK; r0 += nc) {
-            const int64_t cnt = (n1 - r0 < nc) ? n1 - r0 : nc;
-            if (n2 == 0) {
-                int ev = tr_begin(&tr, CAT_GEMM1);
-                rc = cutc_launch_fill(Xc, 4 * cnt * nao2, 0.0, st);
-                tr_end(&tr, ev);
-            } else if (opt->precision != CUTC_PREC_FP64) {
-                rc = stage1_split(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
-            } else {
-                rc = stage1(opt, &pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, &prog, Xc, &tr, stats);
-            }
-            if (rc == CUTC_OK)
-                rc = stage2(opt, &pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, &tr, stats);
-        }
+        rc = streamed_chunks(opt, &pl, n1, n2, n_ao, n_nuc, r1, wr1, r2, wr2, rn, aos1, aos2, &prog, out, Xc, nc, &tr,
+                             stats);
         tr_end(&tr, evt);
         if (rc == CUTC_OK)
             rc = hip_rc(hipStreamSynchronize(st));
@@ -887,7 +947,7 @@ done:
     if (have_tr)
         tr_free(&tr);
     dfree(Xc);
-    cutc_jast_program_free(&prog);
+    jprog_free(&prog);
     return rc;
 }
 
@@ -905,6 +965,174 @@ int cutc_int_streamed_device(const cutc_options *opt, int n_grid1, int n_grid2, 
     pthread_mutex_lock(&g_lock);
     int rc = streamed_path(opt, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
                            c_bh, m_bh, n_bh, o_bh, int_2e_ao, (hipStream_t)stream, stats);
+    pthread_mutex_unlock(&g_lock);
+    cutc_set_last_error(rc);
+    return rc;
+}
+
+/* Batched build: every system of the batch is built streamed (its X in r1
+ * chunks of at most opt->n1_chunk points, default 4096, in one chunk buffer
+ * shared by the batch) into its own int_2e_ao.  All programs, plans, the
+ * workspace and the chunk buffer are set up first; then the launches of all
+ * systems are recorded into one hipGraph, which is instantiated and launched
+ * once on the stream. */
+#define CUTC_BATCH_CHUNK 4096
+
+static int check_system(const cutc_system *y)
+{
+    if (check_sizes(y->n_grid1, y->n_grid2, y->n_ao, y->n_nuc, y->size_bh))
+        return CUTC_ERR_ARG;
+    if (y->n_ao > 0 && !y->int_2e_ao)
+        return CUTC_ERR_ARG;
+    if (y->n_grid1 > 0 && y->n_ao > 0 &&
+        (!y->r1 || !y->wr1 || !y->aos_data1 ||
+         (y->n_grid2 > 0 &&
+          (!y->r2 || !y->wr2 || !y->aos_data2 || (y->n_nuc > 0 && !y->rn) ||
+           ((int64_t)y->n_nuc * y->size_bh > 0 && (!y->c_bh || !y->m_bh || !y->n_bh || !y->o_bh))))))
+        return CUTC_ERR_ARG;
+    return CUTC_OK;
+}
+
+static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys, hipStream_t st,
+                      cutc_batch_stats *bst)
+{
+    const cutc_options ropt = resolved_options(opt);
+    opt = &ropt;
+    if (!valid_precision(opt) || nsys < 0 || (nsys > 0 && !sys))
+        return CUTC_ERR_ARG;
+    for (int i = 0; i < nsys; i++)
+        if (check_system(&sys[i]))
+            return CUTC_ERR_ARG;
+    if (bst)
+        memset(bst, 0, sizeof *bst);
+    if (nsys == 0)
+        return CUTC_OK;
+    int rc = CUTC_OK;
+    jprog_t *jp = (jprog_t *)calloc((size_t)nsys, sizeof(jprog_t));
+    cutc_plan *pl = (cutc_plan *)calloc((size_t)nsys, sizeof(cutc_plan));
+    int64_t *nc = (int64_t *)calloc((size_t)nsys, sizeof(int64_t));
+    double *Xc = NULL;
+    hipGraph_t graph = NULL;
+    hipGraphExec_t exec = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    hipStream_t cs = NULL;
+    if (!jp || !pl || !nc) {
+        rc = CUTC_ERR_NOMEM;
+        goto done;
+    }
+    {
+        const int64_t cmax = opt->n1_chunk > 0 ? opt->n1_chunk : CUTC_BATCH_CHUNK;
+        size_t xbytes = 0;
+        for (int i = 0; i < nsys; i++) {
+            const int64_t n1 = sys[i].n_grid1, nch = (n1 + cmax - 1) / cmax;
+            nc[i] = nch > 0 ? (n1 + nch - 1) / nch : 0; /* balanced, as the streamed call */
+            const size_t b = sizeof(double) * 4 * (size_t)nc[i] * sys[i].n_ao * sys[i].n_ao;
+            if (b > xbytes)
+                xbytes = b;
+        }
+        if (xbytes)
+            TRYH(hipMalloc((void **)&Xc, xbytes));
+        const double t0 = now_ms();
+        /* plans against the memory left beside the chunk buffer; the workspace
+         * is grown once to the largest stage of the batch */
+        const size_t budget = device_budget(opt);
+        size_t need = 0;
+        for (int i = 0; i < nsys; i++) {
+            const cutc_system *y = &sys[i];
+            if (y->n_grid1 == 0 || y->n_ao == 0)
+                continue;
+            TRY(cutc_make_plan(nc[i], y->n_grid2, y->n_ao, y->n_nuc, opt->precision, opt->slices, budget, nc[i],
+                               &pl[i]));
+            TRY(jprog_load(y->n_nuc, y->size_bh, y->c_bh, y->m_bh, y->n_bh, y->o_bh, y->n_grid2 > 0, &jp[i]));
+            /* stage 1 also holds the program's per-r1 table */
+            const size_t b1 = pl[i].bytes_stage1 + sizeof(double) * ((size_t)jp[i].h.ts * pl[i].n1_chunk + 1) + 4096;
+            const size_t b = b1 > pl[i].bytes_stage2 ? b1 : pl[i].bytes_stage2;
+            if (b > need)
+                need = b;
+        }
+        ws_t ws;
+        TRY(ws_begin(&ws, need + need / 64 + ((size_t)16 << 20)));
+        /* record on a private stream (the legacy default stream cannot be
+         * captured), ordered after the caller's stream */
+        TRYH(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        TRYH(hipEventCreate(&e0));
+        TRYH(hipEventCreate(&e1));
+        TRYH(hipEventRecord(e0, st));
+        TRYH(hipStreamWaitEvent(cs, e0, 0));
+        tracer_t tr;
+        tr_init(&tr, cs, 0);
+        tr.capture = 1;
+        g_ws_frozen = 1;
+        TRYH(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < nsys && rc == CUTC_OK; i++) {
+            const cutc_system *y = &sys[i];
+            const int64_t nao2 = (int64_t)y->n_ao * y->n_ao;
+            if (y->n_ao == 0)
+                continue;
+            if (y->n_grid1 == 0)
+                rc = cutc_launch_fill(y->int_2e_ao, nao2 * nao2, 0.0, cs);
+            else
+                rc = streamed_chunks(opt, &pl[i], y->n_grid1, y->n_grid2, y->n_ao, y->n_nuc, y->r1, y->wr1, y->r2,
+                                     y->wr2, y->rn, y->aos_data1, y->aos_data2, &jp[i], y->int_2e_ao, Xc, nc[i], &tr,
+                                     NULL);
+        }
+        const int rc_end = hip_rc(hipStreamEndCapture(cs, &graph));
+        g_ws_frozen = 0;
+        tr_free(&tr);
+        TRY(rc);
+        TRY(rc_end);
+        TRYH(hipGraphInstantiate(&exec, graph, NULL, NULL, 0));
+        if (bst) {
+            bst->ms_setup = now_ms() - t0;
+            size_t nn = 0;
+            if (hipGraphGetNodes(graph, NULL, &nn) == hipSuccess)
+                bst->n_nodes = (int64_t)nn;
+        }
+        TRYH(hipEventRecord(e0, cs));
+        TRYH(hipGraphLaunch(exec, cs));
+        TRYH(hipEventRecord(e1, cs));
+        TRYH(hipStreamSynchronize(cs));
+        float ms = 0.0f;
+        TRYH(hipEventElapsedTime(&ms, e0, e1));
+        if (bst) {
+            bst->ms_launch = ms;
+            bst->n_systems = nsys;
+        }
+    }
+done:
+    g_ws_frozen = 0;
+    if (cs) {
+        hipStreamSynchronize(cs);
+        hipStreamDestroy(cs);
+    }
+    if (exec)
+        hipGraphExecDestroy(exec);
+    if (graph)
+        hipGraphDestroy(graph);
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    dfree(Xc);
+    if (jp)
+        for (int i = 0; i < nsys; i++)
+            jprog_free(&jp[i]);
+    free(jp);
+    free(pl);
+    free(nc);
+    return rc;
+}
+
+int cutc_int_batch_device(const cutc_options *opt, int n_systems, const cutc_system *systems, void *stream,
+                          cutc_batch_stats *stats)
+{
+    cutc_options def;
+    if (!opt) {
+        cutc_default_options(&def);
+        opt = &def;
+    }
+    pthread_mutex_lock(&g_lock);
+    int rc = batch_path(opt, n_systems, systems, (hipStream_t)stream, stats);
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     return rc;

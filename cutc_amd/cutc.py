@@ -60,6 +60,22 @@ class Stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class System(ctypes.Structure):
+    """one system of a batch (cutc_system): device pointers"""
+    _fields_ = [("n_grid1", ctypes.c_int), ("n_grid2", ctypes.c_int), ("n_ao", ctypes.c_int), ("n_nuc", ctypes.c_int),
+                ("size_bh", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in (
+                    "r1", "wr1", "r2", "wr2", "rn", "aos_data1", "aos_data2", "c_bh", "m_bh", "n_bh", "o_bh",
+                    "int_2e_ao")]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ms_setup", ctypes.c_double), ("ms_launch", ctypes.c_double), ("n_nodes", ctypes.c_int64),
+                ("n_systems", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Plan(ctypes.Structure):
     _fields_ = [("n1_chunk", ctypes.c_int64), ("ldk", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
                 ("n1_chunk_2e", ctypes.c_int64), ("bytes_stage1", ctypes.c_size_t),
@@ -71,7 +87,7 @@ HEADER_SYMBOLS = [
     "cutc_int", "get_int2_grad1_u12_ao", "get_int_2e_ao", "checkCudaErrors", "checkCublasErrors", "cutc_int_bh",
     "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
     "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
-    "cutc_release_workspace", "cutc_int_streamed_device",
+    "cutc_release_workspace", "cutc_int_streamed_device", "cutc_int_batch_device",
 ]
 
 _vp = ctypes.c_void_p
@@ -88,6 +104,9 @@ def lib():
         L.cutc_int.argtypes = [_ci] * 11 + [_vp] * 13
         L.cutc_int_ex.restype = _ci
         L.cutc_int_ex.argtypes = [ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [ctypes.POINTER(Stats)]
+        L.cutc_int_batch_device.restype = _ci
+        L.cutc_int_batch_device.argtypes = [ctypes.POINTER(Options), _ci, ctypes.POINTER(System), _vp,
+                                            ctypes.POINTER(BatchStats)]
         L.cutc_int_streamed_device.restype = _ci
         L.cutc_int_streamed_device.argtypes = ([ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 12 + [_vp]
                                                + [ctypes.POINTER(Stats)])
@@ -271,6 +290,27 @@ def cutc_int_streamed_device(ds, I, opt: Options | None = None, stream=None):
                                         _tp(ds.aos2), _tp(ds.c_bh), _tp(ds.m_bh), _tp(ds.n_bh), _tp(ds.o_bh),
                                         _tp(I), ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
     _check(rc, "cutc_int_streamed_device")
+    return st.as_dict()
+
+
+def cutc_int_batch_device(dss, Is, opt: Options | None = None, stream=None):
+    """Batched build of the DeviceSystems ``dss`` into the int_2e_ao tensors
+    ``Is``: one hipGraph launch for the whole batch.  Returns the batch
+    statistics dict."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    arr = (System * max(1, len(dss)))()
+    for y, ds, I in zip(arr, dss, Is):
+        y.n_grid1, y.n_grid2, y.n_ao, y.n_nuc, y.size_bh = ds.n1, ds.n2, ds.n_ao, ds.n_nuc, ds.size_bh
+        for f, t in (("r1", ds.r1), ("wr1", ds.wr1), ("r2", ds.r2), ("wr2", ds.wr2), ("rn", ds.rn),
+                     ("aos_data1", ds.aos1), ("aos_data2", ds.aos2), ("c_bh", ds.c_bh), ("m_bh", ds.m_bh),
+                     ("n_bh", ds.n_bh), ("o_bh", ds.o_bh), ("int_2e_ao", I)):
+            setattr(y, f, None if t is None else t.data_ptr())
+    st = BatchStats()
+    rc = lib().cutc_int_batch_device(ctypes.byref(opt or options()), len(dss), arr,
+                                     ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+    _check(rc, "cutc_int_batch_device")
     return st.as_dict()
 
 

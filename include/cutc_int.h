@@ -172,6 +172,30 @@ int cutc_int_streamed_device(const cutc_options *opt, int n_grid1, int n_grid2, 
                              const int *m_bh, const int *n_bh, const int *o_bh, double *int_2e_ao, void *stream,
                              cutc_stats *stats);
 
+/* One system of a batch: the inputs of cutc_int_device on the device and
+ * the system's own int_2e_ao (n_ao^4 doubles). */
+typedef struct cutc_system {
+    int n_grid1, n_grid2, n_ao, n_nuc, size_bh;
+    const double *r1, *wr1, *r2, *wr2, *rn, *aos_data1, *aos_data2, *c_bh;
+    const int *m_bh, *n_bh, *o_bh;
+    double *int_2e_ao;
+} cutc_system;
+
+typedef struct cutc_batch_stats {
+    double ms_setup;  /* plans, programs, workspace, graph recording and instantiation */
+    double ms_launch; /* the one graph launch, stream-ordered hipEvents */
+    int64_t n_nodes;  /* kernel launches recorded in the graph */
+    int n_systems;
+} cutc_batch_stats;
+
+/* Batched build of independent systems of any sizes: each is built streamed
+ * (X in r1 chunks of opt->n1_chunk points, default 4096) into its own
+ * int_2e_ao; the launches of the whole batch are recorded into one hipGraph
+ * and launched once on `stream`.  No reference counterpart (the reference
+ * builds one system per call). */
+int cutc_int_batch_device(const cutc_options *opt, int n_systems, const cutc_system *systems, void *stream,
+                          cutc_batch_stats *stats);
+
 /* Memory plan (pure host function): r1 chunk for the given device budget.
  * Returns 0 or CUTC_ERR_NOMEM. */
 typedef struct cutc_plan {
