@@ -13,6 +13,10 @@ split results are compared with the FP64 ones: relative Frobenius and
 max-abs error of every X component and of int_2e_ao, and of
 sum_{kl,ab} D_kl D_ab (kl|ab) for a seeded rank-10 PSD density D.
 
+--workload batch64 times 64 independent systems of uneven sizes: 64
+sequential FP64 calls against one batched hipGraph launch in FP64, bf16
+slices and int8 (errors against the sequential FP64 results).
+
 --workload cube40 runs the large streamed case instead (40 nuclei, n_ao = 300,
 n1 = 80,000, n2 = 300,000, cutc_int_streamed_device: X is never stored, each
 r1 chunk goes straight into the int_2e_ao accumulator); errors are reported
@@ -60,9 +64,11 @@ def parse():
     ap.add_argument("--slices", type=int, default=None,
                     help="int8: moduli (default 11); bf16 / fp16: digit slices per operand (default 4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
-    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40"],
+    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40", "batch64"],
                     help="cube20: 20 nuclei, n_ao=200, X stored (the metric's workload); cube40: 40 nuclei, "
-                         "n_ao=300, n1=80000, n2=300000, streamed (X never stored)")
+                         "n_ao=300, n1=80000, n2=300000, streamed (X never stored); batch64: 64 independent "
+                         "systems, sequential FP64 calls vs one batched graph launch")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="batch64: grid sizes times this factor")
     ap.add_argument("--n1", type=int, default=None)
     ap.add_argument("--n2", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=32, help="r1 points of the CPU-baseline sample (0: skip)")
@@ -122,6 +128,65 @@ def density(n_ao, seed=20261205):
     return C @ C.T
 
 
+def batch_bench(args, torch, cutc, synth):
+    """64 independent systems (n_nuc U{3..12}, n_ao U{30..150}, n1 U{5k..30k},
+    n2 U{20k..100k}): 64 sequential FP64 streamed calls against one batched
+    graph launch (cutc_int_batch_device) in FP64, bf16 slices and int8."""
+    t0 = time.perf_counter()
+    sys_ = synth.batch_systems(64, scale=args.batch_scale)
+    log(args, f"generated 64 systems in {time.perf_counter() - t0:.1f} s")
+    dss = [cutc.DeviceSystem.upload(s) for s in sys_]
+    W = sum(w_ref(s.n_ao, s.n1, s.n2) for s in sys_)
+    stream = torch.cuda.current_stream()
+    Iseq = [torch.empty((s.n_ao,) * 4, dtype=torch.float64, device="cuda") for s in sys_]
+    Ib = [torch.empty_like(I) for I in Iseq]
+    # warm-up: load the kernels of every path on a small system
+    small = synth.make_system(3, 24, 300, 2000, 1, name="warm")
+    dsm = cutc.DeviceSystem.upload(small)
+    Ism = torch.empty((24,) * 4, dtype=torch.float64, device="cuda")
+    for prec in ("fp64", "bf16", "int8"):
+        cutc.cutc_int_streamed_device(dsm, Ism, cutc.options(prec), stream)
+        cutc.cutc_int_batch_device([dsm], [Ism], cutc.options(prec), stream)
+    # sequential FP64 calls
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ds, I in zip(dss, Iseq):
+        cutc.cutc_int_streamed_device(ds, I, cutc.options("fp64"), stream)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    log(args, f"64 sequential fp64 calls: {seq_s:.2f} s")
+    cutc.release_workspace()
+    out = {"metric": "batch64: int_2e_ao build s + eff TFLOP/s, 64 systems, one graph launch vs 64 FP64 calls",
+           "unit": "TFLOP/s", "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": f"batch64: 64 systems, n_nuc U{{3..12}}, n_ao U{{30..150}}, "
+                                  f"n1 U{{5k..30k}} x {args.batch_scale}, n2 U{{20k..100k}} x {args.batch_scale}",
+                      "n_ao_range": [min(s.n_ao for s in sys_), max(s.n_ao for s in sys_)]},
+           "w_ref_flops": W, "sequential_fp64_s": seq_s, "sequential_fp64_eff_tflops": W / seq_s / 1e12,
+           "batch": {}}
+    for prec in ("fp64", "bf16", "int8"):
+        opt = cutc.options(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = cutc.cutc_int_batch_device(dss, Ib, opt, stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        errs = []
+        for I, Ir in zip(Ib, Iseq):
+            d = float(torch.linalg.norm(I - Ir) / torch.linalg.norm(Ir))
+            errs.append(d)
+        cutc.release_workspace()
+        res = {"graph_launch_s": st["ms_launch"] / 1e3, "call_wall_s": wall, "setup_ms": st["ms_setup"],
+               "autotune_ms": st["ms_tune"], "graph_nodes": st["n_nodes"], "systems_split_k": st["n_split"],
+               "eff_tflops": W / (st["ms_launch"] / 1e3) / 1e12, "speedup_vs_sequential_fp64": seq_s / wall,
+               "max_rel_frob_vs_sequential_fp64": max(errs)}
+        log(args, f"batch {prec}: {res}")
+        out["batch"][f"{prec}x{opt.slices}" if prec != "fp64" else prec] = res
+    best = out["batch"]["fp64"]
+    out.update({"value": best["eff_tflops"], "ms_per_step": best["graph_launch_s"] * 1e3, "dtype": "f64"})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -129,6 +194,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.workload == "batch64":
+        from cutc_amd import cutc, synth
+        batch_bench(args, torch, cutc, synth)
+        return
 
     from cutc_amd import cutc, synth
 

@@ -28,6 +28,13 @@
 #include "cutc_int.h"
 #include "kernels/launch.h"
 
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
 void cutc_set_last_error(int code);
 
 #define NCAT 7
@@ -551,13 +558,121 @@ done:
 
 /* ------------------------------------------------------------ stage 2 */
 
+/* K splits of the FP64 step-2 contraction for n_pairs P and K = n1 chunk:
+ * split only when the P x P grid of 128 x 128 tiles is small (< 256 blocks),
+ * into at most 16 slices of at least 1024 K rows, keeping the partial sums
+ * within a sixteenth of the budget. */
+static int split2_heuristic(int64_t P, int64_t K, size_t budget)
+{
+    const int64_t tiles = ((P + 127) / 128) * ((P + 127) / 128);
+    if (tiles >= 256 || K < 2048)
+        return 1;
+    int s = 1;
+    while (s < 16 && tiles * s < 512 && K / (2 * s) >= 1024 && (size_t)(2 * s) * P * P * 8 <= budget / 16)
+        s *= 2;
+    return s;
+}
+
+/* Autotuner of the split: times the FP64 step-2 contraction and its reduce
+ * for the candidate splits 1, 2, 4, 8, 16 (up to the heuristic's memory and
+ * K limits) on scratch operands of the real shape (P pairs of n_ao orbitals,
+ * K rows) and keeps the fastest; results are cached per (P, K). */
+#define TUNE_CACHE 64
+static struct {
+    int64_t P, K;
+    int split;
+} g_tune[TUNE_CACHE];
+static int g_ntune = 0;
+
+static int tune_split2(int64_t P, int n_ao, int64_t K, size_t budget, hipStream_t st, double *ms_tune)
+{
+    for (int i = 0; i < g_ntune; i++)
+        if (g_tune[i].P == P && g_tune[i].K == K)
+            return g_tune[i].split;
+    int smax = split2_heuristic(P, K, budget);
+    if (smax <= 1)
+        return 1;
+    while (smax < 16 && K / (2 * smax) >= 1024 && (size_t)(2 * smax) * P * P * 8 <= budget / 16)
+        smax *= 2;
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    double *X = NULL, *B = NULL, *M = NULL, *work = NULL;
+    int *pairs = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    int best = 1;
+    float best_ms = 0.0f;
+    const double t0 = now_ms();
+    int rc = hip_rc(hipMalloc((void **)&X, sizeof(double) * K * nao2));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipMalloc((void **)&B, sizeof(double) * ((K + 15) / 16 * 16) * P));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipMalloc((void **)&M, sizeof(double) * nao2 * nao2));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipMalloc((void **)&work, sizeof(double) * smax * P * P));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipMalloc((void **)&pairs, sizeof(int) * 2 * P));
+    if (rc == CUTC_OK)
+        rc = cutc_launch_fill(X, K * nao2, 1.0, st);
+    if (rc == CUTC_OK)
+        rc = cutc_launch_fill(B, (K + 15) / 16 * 16 * P, 0.5, st);
+    if (rc == CUTC_OK)
+        rc = cutc_launch_pair_table(n_ao, pairs, st);
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipEventCreate(&e0));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipEventCreate(&e1));
+    for (int s = 1; s <= smax && rc == CUTC_OK; s *= 2) {
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && rc == CUTC_OK; rep++) { /* the first run warms up */
+            rc = hip_rc(hipEventRecord(e0, st));
+            if (rc == CUTC_OK && s == 1)
+                rc = cutc_gemm2_f64(X, nao2, B, P, P, K, 1.0, pairs, n_ao, M, 1, st);
+            if (rc == CUTC_OK && s > 1)
+                rc = cutc_gemm2_f64_split(X, nao2, B, P, P, K, pairs, n_ao, s, work, st);
+            if (rc == CUTC_OK && s > 1)
+                rc = cutc_scatter_m(work, cutc_gemm2_split_count(K, s), P, 1.0, pairs, n_ao, M, 1, st);
+            if (rc == CUTC_OK)
+                rc = hip_rc(hipEventRecord(e1, st));
+            if (rc == CUTC_OK)
+                rc = hip_rc(hipEventSynchronize(e1));
+            if (rc == CUTC_OK)
+                rc = hip_rc(hipEventElapsedTime(&ms, e0, e1));
+        }
+        if (rc == CUTC_OK && (s == 1 || ms < best_ms)) {
+            best = s;
+            best_ms = ms;
+        }
+    }
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    dfree(X);
+    dfree(B);
+    dfree(M);
+    dfree(work);
+    dfree(pairs);
+    if (ms_tune)
+        *ms_tune += now_ms() - t0;
+    if (rc != CUTC_OK)
+        return split2_heuristic(P, K, budget);
+    if (g_ntune < TUNE_CACHE) {
+        g_tune[g_ntune].P = P;
+        g_tune[g_ntune].K = K;
+        g_tune[g_ntune].split = best;
+        g_ntune++;
+    }
+    return best;
+}
+
 /* Stage 2 over the r1 points r0 .. r0 + cnt - 1: wr1 / aos1 are the caller's
  * full arrays (leading dimension n1), X holds the cnt points' four components
  * with component stride cnt.  accumulate = 0: the first contraction stores
- * into out, else everything adds; symm = 1: out = M + M^T at the end. */
+ * into out, else everything adds; symm = 1: out = M + M^T at the end.
+ * nsplit2 > 1: the FP64 contractions run split-K (for small n_pairs, whose
+ * P x P tile grid leaves most of the GPU idle). */
 static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t r0, int64_t cnt, int n_ao,
                   const double *wr1, const double *aos1, const double *X, double *out, int accumulate, int symm,
-                  tracer_t *tr, cutc_stats *stats)
+                  int nsplit2, tracer_t *tr, cutc_stats *stats)
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, n1c = pl->n1_chunk_2e;
@@ -565,7 +680,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     int *pairs = NULL;
     double *Hp = NULL, *NHp = NULL;
     int *rexpA = NULL, *rexpB = NULL;
-    double *npart = NULL;
+    double *npart = NULL, *work = NULL;
     int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
     const int split = opt->precision != CUTC_PREC_FP64;
@@ -584,6 +699,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     {
         size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
                       ws_round(sizeof(double) * 3 * plane);
+        if (!split && nsplit2 > 1)
+            need += ws_round(sizeof(double) * nsplit2 * P * P);
         if (split)
             need += 2 * ws_round(sizeof(int) * P) + ws_round(sizeof(double) * CUTC_CRT_KSPLIT * P) + 2 * ws_round(bpl) +
                     ws_round(bcp);
@@ -594,6 +711,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     WS_TAKE(pairs, sizeof(int) * 2 * P);
     WS_TAKE(Hp, sizeof(double) * plane);
     WS_TAKE(NHp, sizeof(double) * 3 * plane);
+    if (!split && nsplit2 > 1)
+        WS_TAKE(work, sizeof(double) * nsplit2 * P * P);
     if (split) {
         WS_TAKE(rexpA, sizeof(int) * P);
         WS_TAKE(rexpB, sizeof(int) * P);
@@ -614,7 +733,19 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
         tr_end(tr, ev);
         TRY(rc);
-        if (!split) {
+        if (!split && nsplit2 > 1) {
+            const int ns = cutc_gemm2_split_count(n1e, nsplit2);
+            ev = tr_begin(tr, CAT_GEMM2);
+            for (int m = 3; m >= 0 && rc == CUTC_OK; m--) {
+                rc = cutc_gemm2_f64_split(X + (m * cnt + i0) * nao2, nao2, m == 3 ? Hp : NHp + m * plane, P, P, n1e,
+                                          pairs, n_ao, nsplit2, work, st);
+                if (rc == CUTC_OK)
+                    rc = cutc_scatter_m(work, ns, P, m == 3 ? 1.0 : -0.5, pairs, n_ao, out,
+                                        m == 3 ? (first ? 0 : 1) : 2, st);
+            }
+            tr_end(tr, ev);
+            TRY(rc);
+        } else if (!split) {
             ev = tr_begin(tr, CAT_GEMM2);
             rc = cutc_gemm2_f64(X + (3 * cnt + i0) * nao2, nao2, Hp, P, P, n1e, 1.0, pairs, n_ao, out, first ? 0 : 1,
                                 st);
@@ -668,7 +799,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 ev = tr_begin(tr, CAT_SPLIT);
                 const double alpha = part == 0 ? 1.0 : -0.5;
                 const int mode = part == 0 ? (first ? 0 : 1) : 2;
-                rc = slice ? cutc_oz_scatter_m((const double *)Cpl, P, alpha, pairs, n_ao, out, mode, st)
+                rc = slice ? cutc_scatter_m((const double *)Cpl, 1, P, alpha, pairs, n_ao, out, mode, st)
                            : cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, alpha, pairs, n_ao, out, mode, st);
                 tr_end(tr, ev);
                 TRY(rc);
@@ -693,12 +824,6 @@ done:
 
 /* --------------------------------------------------------- entry points */
 
-static double now_ms(void)
-{
-    struct timespec ts;
-    clock_gettime(CLOCK_MONOTONIC, &ts);
-    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
-}
 
 static int check_sizes(int n1, int n2, int n_ao, int n_nuc, int size_bh)
 {
@@ -781,7 +906,8 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             rc = cutc_launch_fill(out, (int64_t)n_ao * n_ao * n_ao * n_ao, 0.0, st);
             tr_end(&tr, ev);
         } else {
-            rc = stage2(opt, &pl, n1, 0, n1, n_ao, wr1, aos1, X, out, 0, 1, &tr, stats);
+            rc = stage2(opt, &pl, n1, 0, n1, n_ao, wr1, aos1, X, out, 0, 1,
+                        split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
         }
     }
     tr_end(&tr, evt);
@@ -836,7 +962,7 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
 static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                            const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
                            const double *aos1, const double *aos2, const jprog_t *jp, double *out, double *Xc,
-                           int64_t nc, tracer_t *tr, cutc_stats *stats)
+                           int64_t nc, int nsplit2, tracer_t *tr, cutc_stats *stats)
 {
     const int64_t nao2 = (int64_t)n_ao * n_ao;
     int rc = CUTC_OK;
@@ -852,7 +978,7 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
             rc = stage1(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats);
         }
         if (rc == CUTC_OK)
-            rc = stage2(opt, pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, tr, stats);
+            rc = stage2(opt, pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, nsplit2, tr, stats);
     }
     return rc;
 }
@@ -929,8 +1055,8 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
         tr_init(&tr, st, stats != NULL || opt->verbose);
         have_tr = 1;
         int evt = tr_begin(&tr, CAT_TOTAL);
-        rc = streamed_chunks(opt, &pl, n1, n2, n_ao, n_nuc, r1, wr1, r2, wr2, rn, aos1, aos2, &prog, out, Xc, nc, &tr,
-                             stats);
+        rc = streamed_chunks(opt, &pl, n1, n2, n_ao, n_nuc, r1, wr1, r2, wr2, rn, aos1, aos2, &prog, out, Xc, nc,
+                             split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
         tr_end(&tr, evt);
         if (rc == CUTC_OK)
             rc = hip_rc(hipStreamSynchronize(st));
@@ -1011,12 +1137,13 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
     jprog_t *jp = (jprog_t *)calloc((size_t)nsys, sizeof(jprog_t));
     cutc_plan *pl = (cutc_plan *)calloc((size_t)nsys, sizeof(cutc_plan));
     int64_t *nc = (int64_t *)calloc((size_t)nsys, sizeof(int64_t));
+    int *split2 = (int *)calloc((size_t)nsys, sizeof(int));
     double *Xc = NULL;
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
     hipEvent_t e0 = NULL, e1 = NULL;
     hipStream_t cs = NULL;
-    if (!jp || !pl || !nc) {
+    if (!jp || !pl || !nc || !split2) {
         rc = CUTC_ERR_NOMEM;
         goto done;
     }
@@ -1044,9 +1171,16 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             TRY(cutc_make_plan(nc[i], y->n_grid2, y->n_ao, y->n_nuc, opt->precision, opt->slices, budget, nc[i],
                                &pl[i]));
             TRY(jprog_load(y->n_nuc, y->size_bh, y->c_bh, y->m_bh, y->n_bh, y->o_bh, y->n_grid2 > 0, &jp[i]));
-            /* stage 1 also holds the program's per-r1 table */
+            /* the autotuner picks the K split of the FP64 step-2 contraction */
+            split2[i] = opt->precision == CUTC_PREC_FP64
+                            ? tune_split2(pl[i].n_pairs, y->n_ao, pl[i].n1_chunk_2e, budget, st,
+                                          bst ? &bst->ms_tune : NULL)
+                            : 1;
+            /* stage 1 also holds the program's per-r1 table, stage 2 the split partial sums */
             const size_t b1 = pl[i].bytes_stage1 + sizeof(double) * ((size_t)jp[i].h.ts * pl[i].n1_chunk + 1) + 4096;
-            const size_t b = b1 > pl[i].bytes_stage2 ? b1 : pl[i].bytes_stage2;
+            const size_t b2 = pl[i].bytes_stage2 +
+                              (split2[i] > 1 ? sizeof(double) * split2[i] * pl[i].n_pairs * pl[i].n_pairs + 256 : 0);
+            const size_t b = b1 > b2 ? b1 : b2;
             if (b > need)
                 need = b;
         }
@@ -1073,8 +1207,8 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
                 rc = cutc_launch_fill(y->int_2e_ao, nao2 * nao2, 0.0, cs);
             else
                 rc = streamed_chunks(opt, &pl[i], y->n_grid1, y->n_grid2, y->n_ao, y->n_nuc, y->r1, y->wr1, y->r2,
-                                     y->wr2, y->rn, y->aos_data1, y->aos_data2, &jp[i], y->int_2e_ao, Xc, nc[i], &tr,
-                                     NULL);
+                                     y->wr2, y->rn, y->aos_data1, y->aos_data2, &jp[i], y->int_2e_ao, Xc, nc[i],
+                                     split2[i], &tr, NULL);
         }
         const int rc_end = hip_rc(hipStreamEndCapture(cs, &graph));
         g_ws_frozen = 0;
@@ -1097,6 +1231,8 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
         if (bst) {
             bst->ms_launch = ms;
             bst->n_systems = nsys;
+            for (int i = 0; i < nsys; i++)
+                bst->n_split += split2[i] > 1;
         }
     }
 done:
@@ -1120,6 +1256,7 @@ done:
     free(jp);
     free(pl);
     free(nc);
+    free(split2);
     return rc;
 }
 

@@ -26,7 +26,7 @@
 
 enum { OPA_ROWK = 0, OPA_COLK_GATHER = 1 };
 enum { OPB_ROWK = 0, OPB_COLK = 1 };
-enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4 };
+enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4, EPI_RAW = 5 };
 
 struct GemmArgs {
     const double *A;
@@ -41,6 +41,7 @@ struct GemmArgs {
     int64_t n1c, n1, i0;
     double alpha;
     int tiles_m, tiles_n;
+    int64_t kslice; // split K (gridDim.y > 1): block y takes K rows [y kslice, (y + 1) kslice)
 };
 
 // Fetch this thread's 8 operand values of one K step into registers.
@@ -94,6 +95,15 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
     int tm, tn;
     cutc_group_tile<8>(t, g.tiles_m, g.tiles_n, tm, tn);
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+    if (gridDim.y > 1) {
+        // this block's K slice of the (K-row) operands; slices start at
+        // multiples of BK below K = roundup(Ka, BK), so Ka - start >= 1
+        const int64_t ks = (int64_t)blockIdx.y * g.kslice;
+        g.A += ks * g.lda;
+        g.B += ks * g.ldb;
+        g.K = (g.K - ks < g.kslice) ? g.K - ks : g.kslice;
+        g.Ka = (g.Ka - ks < g.kslice) ? g.Ka - ks : g.kslice;
+    }
 
     // per-thread operand row (fixed over the K loop)
     int64_t aidx = 0, bidx = 0;
@@ -174,6 +184,11 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
                 const int64_t row = row0 + wm + 16 * i + 4 * r + lg;
                 if (row >= g.M)
                     continue;
+                if constexpr (EPI == EPI_RAW) {
+                    // unscaled partial sum of this K slice: out[y][col][row]
+                    g.out[((int64_t)blockIdx.y * g.N + col) * g.M + row] = acc[i][j][r];
+                    continue;
+                }
                 const double v = g.alpha * acc[i][j][r];
                 const int k = g.pairs[2 * row], l = g.pairs[2 * row + 1];
                 const int64_t kl = k + (int64_t)g.n_ao * l, lk = l + (int64_t)g.n_ao * k;
@@ -214,18 +229,27 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
     }
 }
 
+// nsplit > 1: K slices of g.kslice rows (a multiple of BK, set by the
+// caller with nsplit = ceil(K / kslice), so every slice starts below K)
 template <int AM, int BMODE, int EPI>
-static int launch_gemm(const GemmArgs &g0, hipStream_t st)
+static int launch_gemm(const GemmArgs &g0, hipStream_t st, int nsplit = 1)
 {
     GemmArgs g = g0;
     if (g.M <= 0 || g.N <= 0)
         return 0;
     int64_t tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
-    if (tm * tn > 2147483647)
+    if (tm * tn > 2147483647 || nsplit < 1 || nsplit > 65535)
+        return CUTC_ERR_ARG;
+    if (nsplit == 1)
+        g.kslice = g.K;
+    else if (g.kslice <= 0 || g.kslice % BK || (nsplit - 1) * g.kslice >= g.K || nsplit * g.kslice < g.K)
+        return CUTC_ERR_ARG;
+    if (AM == OPA_COLK_GATHER && !g.pairs)
         return CUTC_ERR_ARG;
     g.tiles_m = (int)tm;
     g.tiles_n = (int)tn;
-    hipLaunchKernelGGL((gemm_f64_kernel<AM, BMODE, EPI>), dim3((unsigned)(tm * tn)), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((gemm_f64_kernel<AM, BMODE, EPI>), dim3((unsigned)(tm * tn), (unsigned)nsplit), dim3(256), 0,
+                       st, g);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -281,4 +305,40 @@ extern "C" int cutc_gemm2_f64(const double *X, int64_t xld, const double *B, int
     case 2: return launch_gemm<OPA_COLK_GATHER, OPB_COLK, EPI_M_ADDSUB>(g, st);
     }
     return CUTC_ERR_ARG;
+}
+
+/* Split-K form of the step-2 contraction: the K rows run in nsplit slices
+ * (blocks y = 0 .. nsplit - 1), each writing its unscaled partial sums
+ * work[y][q][p] (q: B row, p: A row); cutc_scatter_m adds the slices in
+ * order and scatters them.  nsplit is reduced so that every slice has rows. */
+extern "C" int cutc_gemm2_f64_split(const double *X, int64_t xld, const double *B, int64_t ldb, int64_t P, int64_t K,
+                                    const int *pairs, int n_ao, int nsplit, double *work, hipStream_t st)
+{
+    if (K <= 0 || nsplit < 1 || !pairs)
+        return CUTC_ERR_ARG;
+    GemmArgs g = {};
+    g.A = X;
+    g.lda = xld;
+    g.B = B;
+    g.ldb = ldb;
+    g.M = P;
+    g.N = P;
+    g.K = (K + BK - 1) / BK * BK;
+    g.Ka = K;
+    g.pairs = pairs; /* the X operand is gathered through the pair table */
+    g.n_ao = n_ao;
+    g.out = work;
+    g.alpha = 1.0;
+    g.kslice = ((g.K + nsplit - 1) / nsplit + BK - 1) / BK * BK;
+    const int ns = (int)((g.K + g.kslice - 1) / g.kslice);
+    return launch_gemm<OPA_COLK_GATHER, OPB_COLK, EPI_RAW>(g, st, ns);
+}
+
+extern "C" int cutc_gemm2_split_count(int64_t K, int nsplit)
+{
+    if (K <= 0 || nsplit < 1)
+        return 1;
+    const int64_t Kp = (K + BK - 1) / BK * BK;
+    const int64_t kslice = ((Kp + nsplit - 1) / nsplit + BK - 1) / BK * BK;
+    return (int)((Kp + kslice - 1) / kslice);
 }

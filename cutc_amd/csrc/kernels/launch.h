@@ -74,6 +74,11 @@ int cutc_gemm1_f64(const double *Apk, int64_t lda, const double *G, int64_t ldg,
  * X has K valid rows; B must hold roundup(K, 16) rows, zero past row K.
  * mode 0: store C at (kl,ab) (lk,ab) (kl,ba) (lk,ba); 1: add there;
  * 2: add at (*,ab) and subtract at (*,ba) for a < b. */
+/* Split-K step 2: unscaled partial sums of cutc_gemm2_split_count(K, nsplit)
+ * K slices into work[y][q][p] (P x P each), reduced by cutc_scatter_m. */
+int cutc_gemm2_f64_split(const double *X, int64_t xld, const double *B, int64_t ldb, int64_t P, int64_t K,
+                         const int *pairs, int n_ao, int nsplit, double *work, hipStream_t st);
+int cutc_gemm2_split_count(int64_t K, int nsplit);
 int cutc_gemm2_f64(const double *X, int64_t xld, const double *B, int64_t ldb, int64_t P, int64_t K, double alpha,
                    const int *pairs, int n_ao, double *M, int mode, hipStream_t st);
 
@@ -162,8 +167,9 @@ int cutc_oz_gemm(int S, int f16, const uint16_t *A, int64_t M, const uint16_t *B
 /* scatter of FP64 contraction results, as cutc_crt_rebuild_x / _m */
 int cutc_oz_scatter_x(const double *C, int64_t P, int64_t N, const int *pairs, int n_ao, double *X, int64_t n1,
                       int64_t n1c, int64_t i0, hipStream_t st);
-int cutc_oz_scatter_m(const double *C, int64_t P, double alpha, const int *pairs, int n_ao, double *M, int mode,
-                      hipStream_t st);
+/* alpha sum_{y < nsplit} C[y][q][p] into M (C slices of P x P) */
+int cutc_scatter_m(const double *C, int nsplit, int64_t P, double alpha, const int *pairs, int n_ao, double *M,
+                   int mode, hipStream_t st);
 /* Rebuild C = V 2^(eA + eB - 2 bits) and scatter into X (4, n1, n_ao, n_ao):
  * column j = m n1c + i1 -> X[m][i0 + i1][(k,l) and (l,k)]. */
 int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N, const int *rexpA,

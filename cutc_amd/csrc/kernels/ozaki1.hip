@@ -17,7 +17,8 @@
 //   oz_gemm         per block 64 x 128, 4 waves of 32 x 64 (1 x 2 MFMA
 //                   32x32x16), S level accumulators in FP32, FP64 sums; both
 //                   operands through a 2-stage LDS ring fed by LDS DMA
-//   oz_scatter_*    the FP64 result scattered to the reference's layouts
+//   oz_scatter_x, scatter_m   FP64 results scattered to the reference's
+//                   layouts (scatter_m also serves the split-K FP64 path)
 //
 // Slice-packed planes (16-bit elements): rows in blocks of 32, K in steps of
 // 32; the 2 KB tile (row block rb, step kt) sits at element (rb KT + kt) 1024
@@ -451,11 +452,12 @@ extern "C" int cutc_oz_scatter_x(const double *C, int64_t P, int64_t N, const in
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// alpha C[q][p] into M; modes as the FP64 step-2 epilogue (0 store, 1 add,
-// 2 add at (kl, ab) and subtract at (kl, ba))
-__global__ __launch_bounds__(256) void oz_scatter_m_kernel(const double *__restrict__ C, int64_t P, double alpha,
-                                                           const int *__restrict__ pairs, int n_ao, double *M,
-                                                           int mode)
+// alpha sum_y C[y][q][p] (y < nsplit, added in order) into M; modes as the
+// FP64 step-2 epilogue (0 store, 1 add, 2 add at (kl, ab) and subtract at
+// (kl, ba))
+__global__ __launch_bounds__(256) void scatter_m_kernel(const double *__restrict__ C, int nsplit, int64_t P,
+                                                        double alpha, const int *__restrict__ pairs, int n_ao,
+                                                        double *M, int mode)
 {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= P)
@@ -464,7 +466,10 @@ __global__ __launch_bounds__(256) void oz_scatter_m_kernel(const double *__restr
     const int64_t n2 = (int64_t)n_ao * n_ao;
     const int64_t kl = k + (int64_t)n_ao * l, lk = l + (int64_t)n_ao * k;
     for (int64_t q = blockIdx.y; q < P; q += gridDim.y) {
-        const double v = alpha * C[q * P + p];
+        double c = C[q * P + p];
+        for (int y = 1; y < nsplit; y++)
+            c += C[(int64_t)y * P * P + q * P + p];
+        const double v = alpha * c;
         const int a = pairs[2 * q], b = pairs[2 * q + 1];
         double *o1 = M + n2 * (a + (int64_t)n_ao * b);
         double *o2 = M + n2 * (b + (int64_t)n_ao * a);
@@ -487,13 +492,15 @@ __global__ __launch_bounds__(256) void oz_scatter_m_kernel(const double *__restr
     }
 }
 
-extern "C" int cutc_oz_scatter_m(const double *C, int64_t P, double alpha, const int *pairs, int n_ao, double *M,
-                                 int mode, hipStream_t st)
+extern "C" int cutc_scatter_m(const double *C, int nsplit, int64_t P, double alpha, const int *pairs, int n_ao,
+                              double *M, int mode, hipStream_t st)
 {
     if (P <= 0)
         return 0;
+    if (nsplit < 1)
+        return CUTC_ERR_ARG;
     const int64_t gy = P < 65535 ? P : 65535;
-    hipLaunchKernelGGL(oz_scatter_m_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, C, P,
-                       alpha, pairs, n_ao, M, mode);
+    hipLaunchKernelGGL(scatter_m_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, C, nsplit,
+                       P, alpha, pairs, n_ao, M, mode);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -69,8 +69,8 @@ class System(ctypes.Structure):
 
 
 class BatchStats(ctypes.Structure):
-    _fields_ = [("ms_setup", ctypes.c_double), ("ms_launch", ctypes.c_double), ("n_nodes", ctypes.c_int64),
-                ("n_systems", ctypes.c_int)]
+    _fields_ = [("ms_setup", ctypes.c_double), ("ms_tune", ctypes.c_double), ("ms_launch", ctypes.c_double),
+                ("n_nodes", ctypes.c_int64), ("n_systems", ctypes.c_int), ("n_split", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -182,16 +182,20 @@ typedef struct cutc_system {
 } cutc_system;
 
 typedef struct cutc_batch_stats {
-    double ms_setup;  /* plans, programs, workspace, graph recording and instantiation */
+    double ms_setup;  /* plans, programs, autotuning, workspace, graph recording and instantiation */
+    double ms_tune;   /* of which the autotuner (timed trial contractions) */
     double ms_launch; /* the one graph launch, stream-ordered hipEvents */
-    int64_t n_nodes;  /* kernel launches recorded in the graph */
+    int64_t n_nodes;  /* nodes recorded in the graph */
     int n_systems;
+    int n_split;      /* systems whose FP64 step-2 contraction the autotuner split along K */
 } cutc_batch_stats;
 
 /* Batched build of independent systems of any sizes: each is built streamed
  * (X in r1 chunks of opt->n1_chunk points, default 4096) into its own
  * int_2e_ao; the launches of the whole batch are recorded into one hipGraph
- * and launched once on `stream`.  No reference counterpart (the reference
+ * and launched once on `stream`.  Before recording, an autotuner times the
+ * K splits of the FP64 step-2 contraction for every small pair count (cached
+ * per shape for the process).  No reference counterpart (the reference
  * builds one system per call). */
 int cutc_int_batch_device(const cutc_options *opt, int n_systems, const cutc_system *systems, void *stream,
                           cutc_batch_stats *stats);

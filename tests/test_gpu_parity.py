@@ -182,6 +182,19 @@ def test_empty_and_error_paths(torch):
     assert e.value.code == cutc.CUTC_ERR_NOMEM
 
 
+def test_split_k_step2_matches_oracle(torch):
+    """A small pair count with a long r1 range: the FP64 step-2 contraction
+    runs split along K (8 slices here), reduced in order."""
+    s = synth.make_system(4, 30, 12000, 1500, 1201, name="splitk")
+    print(f"split-K step 2: {s.shape_str()}", flush=True)
+    X, I, st = run_device(torch, s)
+    Xo = oracle.int2_grad1_u12_ao(s)
+    Io = oracle.int_2e_ao(s, Xo)
+    e = relerr(I, Io)
+    print(f"  int_2e_ao vs oracle {e:.2e}", flush=True)
+    assert e <= 1e-12 and relerr(X, Xo) <= 1e-12
+
+
 def test_cube20_full_basis_subset(torch):
     """20 nuclei, n_ao = 200, all n2 = 200000 points, a 24-point r1 subset."""
     s = synth.cube20(n1=24, n2=200000)
