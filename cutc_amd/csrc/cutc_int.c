@@ -1097,8 +1097,9 @@ int cutc_int_streamed_device(const cutc_options *opt, int n_grid1, int n_grid2, 
 }
 
 /* Batched build: every system of the batch is built streamed (its X in r1
- * chunks of at most opt->n1_chunk points, default 4096, in one chunk buffer
- * shared by the batch) into its own int_2e_ao.  All programs, plans, the
+ * chunks of at most opt->n1_chunk points, by default as many as keep the
+ * chunk buffer within a quarter of the device memory, at least 4096; one
+ * chunk buffer shared by the batch) into its own int_2e_ao.  All programs, plans, the
  * workspace and the chunk buffer are set up first; then the launches of all
  * systems are recorded into one hipGraph, which is instantiated and launched
  * once on the stream. */
@@ -1148,7 +1149,25 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
         goto done;
     }
     {
-        const int64_t cmax = opt->n1_chunk > 0 ? opt->n1_chunk : CUTC_BATCH_CHUNK;
+        /* X chunk: opt->n1_chunk when set, else as many r1 points as keep the
+         * shared chunk buffer within a quarter of the device budget (whole
+         * systems, usually: fewer chunks mean fewer step-2 passes) */
+        int64_t cmax = opt->n1_chunk;
+        if (cmax <= 0) {
+            const size_t quarter = device_budget(opt) / 4;
+            int64_t hi = 1;
+            for (int i = 0; i < nsys; i++)
+                if (sys[i].n_grid1 > hi)
+                    hi = sys[i].n_grid1;
+            cmax = hi;
+            for (int i = 0; i < nsys; i++) {
+                const size_t xper = sizeof(double) * 4 * (size_t)sys[i].n_ao * sys[i].n_ao;
+                if (xper > 0 && (size_t)cmax * xper > quarter)
+                    cmax = (int64_t)(quarter / xper);
+            }
+            if (cmax < CUTC_BATCH_CHUNK)
+                cmax = CUTC_BATCH_CHUNK;
+        }
         size_t xbytes = 0;
         for (int i = 0; i < nsys; i++) {
             const int64_t n1 = sys[i].n_grid1, nch = (n1 + cmax - 1) / cmax;
@@ -1168,8 +1187,10 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             const cutc_system *y = &sys[i];
             if (y->n_grid1 == 0 || y->n_ao == 0)
                 continue;
-            TRY(cutc_make_plan(nc[i], y->n_grid2, y->n_ao, y->n_nuc, opt->precision, opt->slices, budget, nc[i],
-                               &pl[i]));
+            /* G chunks beyond a few thousand r1 points gain nothing and only
+             * grow the workspace the batch allocates up front */
+            TRY(cutc_make_plan(nc[i], y->n_grid2, y->n_ao, y->n_nuc, opt->precision, opt->slices, budget,
+                               nc[i] < 8192 ? nc[i] : 8192, &pl[i]));
             TRY(jprog_load(y->n_nuc, y->size_bh, y->c_bh, y->m_bh, y->n_bh, y->o_bh, y->n_grid2 > 0, &jp[i]));
             /* the autotuner picks the K split of the FP64 step-2 contraction */
             split2[i] = opt->precision == CUTC_PREC_FP64

@@ -191,7 +191,8 @@ typedef struct cutc_batch_stats {
 } cutc_batch_stats;
 
 /* Batched build of independent systems of any sizes: each is built streamed
- * (X in r1 chunks of opt->n1_chunk points, default 4096) into its own
+ * (X in r1 chunks of opt->n1_chunk points, by default whole systems as far
+ * as a quarter of the device memory allows) into its own
  * int_2e_ao; the launches of the whole batch are recorded into one hipGraph
  * and launched once on `stream`.  Before recording, an autotuner times the
  * K splits of the FP64 step-2 contraction for every small pair count (cached
