@@ -33,6 +33,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import statistics
 import os
 import sys
 import time
@@ -41,13 +42,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "int_2e_ao build s + eff TFLOP/s: split-BF16 vs FP64 on GPU at ≤1e-10 rel err"
-PEAK = {"fp64": 78.6, "int8": 5000.0, "bf16": 2500.0, "fp16": 2500.0}  # dense MFMA, MI355X (int8: TOPS)
+PEAK = {"fp64": 78.6, "int8": 5000.0, "bf16": 2500.0, "fp16": 2500.0, "bf16crt": 2500.0,
+        "fp16crt": 2500.0}  # dense MFMA, MI355X (int8: TOPS)
 
 
 # HBM-side bytes per stage-1 contraction launch on the default workload (n1 = 60000,
 # n2 = 200000, 11 moduli), from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by
 # a 1 GiB copy kernel measured in the same run (profiles/cube20_traffic.txt); not measured
 # for other shapes.
+BATCH_PRECISIONS = ("fp64", "bf16", "bf16crt", "int8")
 MEASURED_TRAFFIC = {"int8": 271.8e9, "fp64": 17.83e12}
 
 
@@ -58,9 +61,11 @@ def w_ref(n_ao, n1, n2):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed builds (default: 5 for cube20, 1 for the 20x larger cube40)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="int8", choices=["fp64", "int8", "bf16", "fp16"])
+    ap.add_argument("--precision", default="int8", choices=["fp64", "int8", "bf16", "fp16", "bf16crt", "fp16crt"],
+                    help="int8 / bf16crt / fp16crt: modular (CRT) products; bf16 / fp16: digit slices")
     ap.add_argument("--slices", type=int, default=None,
                     help="int8: moduli (default 11); bf16 / fp16: digit slices per operand (default 4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
@@ -69,6 +74,8 @@ def parse():
                          "n_ao=300, n1=80000, n2=300000, streamed (X never stored); batch64: 64 independent "
                          "systems, sequential FP64 calls vs one batched graph launch")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="batch64: grid sizes times this factor")
+    ap.add_argument("--variants", default="",
+                    help="further split builds compared with the same FP64 build, e.g. bf16:3,bf16:4,fp16:4")
     ap.add_argument("--n1", type=int, default=None)
     ap.add_argument("--n2", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=32, help="r1 points of the CPU-baseline sample (0: skip)")
@@ -76,7 +83,10 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="first run one device-to-device copy of 1 GiB, a plain copy kernel of known size that "
                          "calibrates FETCH_SIZE / WRITE_SIZE counters read by rocprofv3 --pmc in the same run")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 1 if args.workload == "cube40" else 5
+    return args
 
 
 def log(args, *a):
@@ -108,17 +118,21 @@ def build(cutc, ds, X, I, opt, stream):
     return cutc.cutc_int_device(ds, X, I, opt, stream)
 
 
-def timed_builds(cutc, torch, ds, X, I, opt, n, stream):
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+def timed_builds(cutc, torch, ds, X, I, opt, n, stream, per_step=None):
+    """n builds back to back between HIP events on the build stream; returns
+    the mean ms per build (the whole timed region / n) and the statistics of
+    every build.  per_step, a list, receives each build's own event time."""
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
     stats = []
     torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(n):
+    ev[0].record(stream)
+    for i in range(n):
         stats.append(build(cutc, ds, X, I, opt, stream))
-    ev1.record(stream)
+        ev[i + 1].record(stream)
     torch.cuda.synchronize()
-    return ev0.elapsed_time(ev1) / n, stats
+    if per_step is not None:
+        per_step.extend(ev[i].elapsed_time(ev[i + 1]) for i in range(n))
+    return ev[0].elapsed_time(ev[n]) / n, stats
 
 
 def density(n_ao, seed=20261205):
@@ -144,7 +158,7 @@ def batch_bench(args, torch, cutc, synth):
     small = synth.make_system(3, 24, 300, 2000, 1, name="warm")
     dsm = cutc.DeviceSystem.upload(small)
     Ism = torch.empty((24,) * 4, dtype=torch.float64, device="cuda")
-    for prec in ("fp64", "bf16", "int8"):
+    for prec in BATCH_PRECISIONS:
         cutc.cutc_int_streamed_device(dsm, Ism, cutc.options(prec), stream)
         cutc.cutc_int_batch_device([dsm], [Ism], cutc.options(prec), stream)
     # sequential FP64 calls
@@ -164,7 +178,7 @@ def batch_bench(args, torch, cutc, synth):
                       "n_ao_range": [min(s.n_ao for s in sys_), max(s.n_ao for s in sys_)]},
            "w_ref_flops": W, "sequential_fp64_s": seq_s, "sequential_fp64_eff_tflops": W / seq_s / 1e12,
            "batch": {}}
-    for prec in ("fp64", "bf16", "int8"):
+    for prec in BATCH_PRECISIONS:
         opt = cutc.options(prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -230,108 +244,8 @@ def main():
     def dd(Imat):  # sum D_kl D_ab (kl|ab)
         return float(D @ (Imat.reshape(n_ao * n_ao, n_ao * n_ao) @ D))
 
-    out = {"metric": METRIC}
-    fp64 = None
-    Xref_host = None
-    if args.precision == "fp64" or not args.no_fp64:
-        small = s.subset_r1(min(256, s.n1))
-        dsm = cutc.DeviceSystem.upload(small)
-        Xs, Is = cutc.alloc_outputs(dsm)
-        build(cutc, dsm, None if streamed else Xs, Is, cutc.options("fp64"), stream)  # loads the kernels
-        del Xs, Is, dsm
-        nrep = args.steps if args.precision == "fp64" else 1
-        ms, st = timed_builds(cutc, torch, ds, X, I, cutc.options("fp64"), nrep, stream)
-        log(args, f"fp64 build: {ms / 1e3:.2f} s  {st[-1]}")
-        fp64 = {"build_s": ms / 1e3, "eff_tflops": W / (ms / 1e3) / 1e12, "stats": st[-1]}
-        if args.precision != "fp64":
-            cutc.release_workspace()  # the device workspace of one precision is not reused by the other
-            dd_ref = dd(I)
-            Iref = I.cpu() if streamed else I.clone()
-            Xref_host = None if streamed else X.cpu()
-
-    if args.precision == "fp64":
-        ms, last = fp64["build_s"] * 1e3, fp64["stats"]
-        prec_label, opt = "fp64", cutc.options("fp64")
-    else:
-        opt = cutc.options(args.precision, args.slices)
-        prec_label = f"{args.precision}x{opt.slices}"
-        for i in range(args.warmup):
-            st = build(cutc, ds, X, I, opt, stream)
-            log(args, f"warmup {i}: {st['ms_total'] / 1e3:.2f} s")
-        ms, sts = timed_builds(cutc, torch, ds, X, I, opt, args.steps, stream)
-        last = sts[-1]
-        log(args, f"{prec_label} build: {ms / 1e3:.2f} s  {last}")
-    cutc.release_workspace()  # room for the error analysis below
-
-    # roofline of the dominant kernel (the step-1 contraction), on the
-    # algorithmic work (packed pairs, unpadded n2); launch times are the
-    # tracer's HIP events on the build's stream
-    exec1 = 2.0 * P * 4 * s.n1 * s.n2
-    default_shape = (not streamed and (s.n1, s.n2) == (60000, 200000)
-                     and (args.precision == "fp64" or opt.slices == 11))
-    traffic = MEASURED_TRAFFIC[args.precision] if default_shape else None
-    if args.precision == "fp64":
-        launches = last["n_gemm1"]
-        achieved = exec1 / (last["ms_gemm1"] / 1e3) / 1e12
-        roof = {"kernel": "gemm_f64_kernel (step-1 contraction)", "bound": "mfma", "achieved": achieved,
-                "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic,
-                "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": f"2 x {P} pairs x 4 n1 x n2 FP64 flops",
-                "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
-    elif args.precision in ("bf16", "fp16"):
-        launches = last["n_gemm1"]
-        npairs = opt.slices * (opt.slices + 1) // 2
-        sl_flops = 2.0 * npairs * P * 4 * s.n1 * s.n2
-        achieved = sl_flops / (last["ms_gemm1"] / 1e3) / 1e12
-        roof = {"kernel": f"oz_gemm_kernel<{opt.slices}> (step-1 contraction, one launch per r1 chunk)",
-                "bound": "mfma", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                "frac": achieved / PEAK[args.precision], "traffic": None, "launches": launches,
-                "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": f"2 x {npairs} slice products x {P} pairs x 4 n1 x n2 {args.precision} flops",
-                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12}
-    else:
-        launches = last["n_gemm1"] * opt.slices  # one launch per modulus and r1 chunk
-        int8_ops = 2.0 * opt.slices * P * 4 * s.n1 * s.n2
-        achieved = int8_ops / (last["ms_gemm1"] / 1e3) / 1e12
-        roof = {"kernel": "crt_gemm_pk_kernel (step-1 contraction, one launch per modulus and r1 chunk)",
-                "bound": "mfma", "achieved": achieved, "peak": PEAK["int8"], "unit": "TOPS (int8)",
-                "frac": achieved / PEAK["int8"], "traffic": traffic, "launches": launches,
-                "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": f"2 x {opt.slices} moduli x {P} pairs x 4 n1 x n2 int8 ops",
-                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12,
-                "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
-
-    out.update({
-        "value": W / (ms / 1e3) / 1e12,
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else f"{prec_label}->f64",
-        "data": "synthetic",
-        "config": {"workload": (f"cube40: 40 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6, streamed "
-                                "(X never stored)") if streamed else
-                               f"cube20: 20 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6",
-                   "precision": prec_label, "n1_chunk": last["n1_chunk"]},
-        "build_s": ms / 1e3,
-        "w_ref_flops": W,
-        "w_exec_fp64_equiv": last["flops_exec"],
-        "stage_ms": {k: last[k] for k in ("ms_jastrow", "ms_integrands", "ms_split", "ms_gemm1", "ms_gemm2",
-                                          "ms_symm")},
-        "roofline": roof,
-    })
-    if fp64 is not None and args.precision != "fp64":
-        out["fp64_build_s"] = fp64["build_s"]
-        out["fp64_eff_tflops"] = fp64["eff_tflops"]
-        out["speedup_vs_fp64"] = fp64["build_s"] / (ms / 1e3)
-        fp64_g1 = fp64["stats"]["ms_gemm1"] / 1e3
-        out["fp64_roofline"] = {"kernel": "gemm_f64_kernel<0,0,*>", "achieved": exec1 / fp64_g1 / 1e12,
-                                "peak": PEAK["fp64"], "frac": exec1 / fp64_g1 / 1e12 / PEAK["fp64"]}
-        # errors of the split build against the FP64 build, every tensor
+    def errors_vs_fp64():
+        """errors of the split build in X, I against the FP64 build, every tensor"""
         err = {}
         if streamed:  # the FP64 result is on the host: compare in slabs
             num = den = mx = rmx = 0.0
@@ -365,8 +279,135 @@ def main():
         err["DD_contraction"] = {"value": dd_split, "fp64_value": dd_ref,
                                  "rel": abs(dd_split - dd_ref) / abs(dd_ref)}
         err["max_rel_frob"] = max(v["rel_frob"] for v in err.values() if isinstance(v, dict) and "rel_frob" in v)
+        return err
+
+    out = {"metric": METRIC}
+    fp64 = None
+    Xref_host = None
+    if args.precision == "fp64" or not args.no_fp64:
+        small = s.subset_r1(min(256, s.n1))
+        dsm = cutc.DeviceSystem.upload(small)
+        Xs, Is = cutc.alloc_outputs(dsm)
+        build(cutc, dsm, None if streamed else Xs, Is, cutc.options("fp64"), stream)  # loads the kernels
+        del Xs, Is, dsm
+        nrep = args.steps if args.precision == "fp64" else 1
+        fp64_step_ms = []
+        ms, st = timed_builds(cutc, torch, ds, X, I, cutc.options("fp64"), nrep, stream, fp64_step_ms)
+        log(args, f"fp64 build: {ms / 1e3:.2f} s  {st[-1]}")
+        fp64 = {"build_s": ms / 1e3, "eff_tflops": W / (ms / 1e3) / 1e12, "stats": st[-1]}
+        if args.precision != "fp64":
+            cutc.release_workspace()  # the device workspace of one precision is not reused by the other
+            dd_ref = dd(I)
+            Iref = I.cpu() if streamed else I.clone()
+            Xref_host = None if streamed else X.cpu()
+
+    if args.precision == "fp64":
+        ms, last = fp64["build_s"] * 1e3, fp64["stats"]
+        step_ms = fp64_step_ms
+        prec_label, opt = "fp64", cutc.options("fp64")
+    else:
+        opt = cutc.options(args.precision, args.slices)
+        prec_label = f"{args.precision}x{opt.slices}"
+        for i in range(args.warmup):
+            st = build(cutc, ds, X, I, opt, stream)
+            log(args, f"warmup {i}: {st['ms_total'] / 1e3:.2f} s")
+        step_ms = []
+        ms, sts = timed_builds(cutc, torch, ds, X, I, opt, args.steps, stream, step_ms)
+        last = sts[-1]
+        log(args, f"{prec_label} build: {ms / 1e3:.2f} s  {last}")
+    cutc.release_workspace()  # room for the error analysis below
+
+    # roofline of the dominant kernel (the step-1 contraction), on the
+    # algorithmic work (packed pairs, unpadded n2); launch times are the
+    # tracer's HIP events on the build's stream
+    exec1 = 2.0 * P * 4 * s.n1 * s.n2
+    default_shape = (not streamed and (s.n1, s.n2) == (60000, 200000)
+                     and (args.precision == "fp64" or opt.slices == 11))
+    traffic = MEASURED_TRAFFIC[args.precision] if default_shape else None
+    if args.precision == "fp64":
+        launches = last["n_gemm1"]
+        achieved = exec1 / (last["ms_gemm1"] / 1e3) / 1e12
+        roof = {"kernel": "gemm_f64_kernel (step-1 contraction)", "bound": "mfma", "achieved": achieved,
+                "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic,
+                "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
+                "work": f"2 x {P} pairs x 4 n1 x n2 FP64 flops",
+                "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
+    elif args.precision in ("bf16", "fp16"):
+        launches = last["n_gemm1"]
+        npairs = opt.slices * (opt.slices + 1) // 2
+        sl_flops = 2.0 * npairs * P * 4 * s.n1 * s.n2
+        achieved = sl_flops / (last["ms_gemm1"] / 1e3) / 1e12
+        roof = {"kernel": f"oz_gemm_kernel<{opt.slices}> (step-1 contraction, one launch per r1 chunk)",
+                "bound": "mfma", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
+                "frac": achieved / PEAK[args.precision], "traffic": None, "launches": launches,
+                "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
+                "work": f"2 x {npairs} slice products x {P} pairs x 4 n1 x n2 {args.precision} flops",
+                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12}
+    else:
+        launches = last["n_gemm1"] * opt.slices  # one launch per modulus and r1 chunk
+        ops = 2.0 * opt.slices * P * 4 * s.n1 * s.n2
+        achieved = ops / (last["ms_gemm1"] / 1e3) / 1e12
+        i8 = args.precision == "int8"
+        roof = {"kernel": ("crt_gemm_pk_kernel" if i8 else "crt16_gemm_kernel") +
+                          " (step-1 contraction, one launch per modulus and r1 chunk)",
+                "bound": "mfma", "achieved": achieved, "peak": PEAK[args.precision],
+                "unit": "TOPS (int8)" if i8 else "TFLOP/s", "frac": achieved / PEAK[args.precision],
+                "traffic": traffic, "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
+                "work": f"2 x {opt.slices} moduli x {P} pairs x 4 n1 x n2 {'int8 ops' if i8 else 'flops'}",
+                "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12,
+                "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
+
+    out.update({
+        "value": W / (ms / 1e3) / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "ms_per_step_median": statistics.median(step_ms),
+        "step_ms": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else f"{prec_label}->f64",
+        "data": "synthetic",
+        "config": {"workload": (f"cube40: 40 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6, streamed "
+                                "(X never stored)") if streamed else
+                               f"cube20: 20 nuclei, n_ao={n_ao}, n1={s.n1}, n2={s.n2}, size_bh=6",
+                   "precision": prec_label, "n1_chunk": last["n1_chunk"]},
+        "build_s": ms / 1e3,
+        "w_ref_flops": W,
+        "w_exec_fp64_equiv": last["flops_exec"],
+        "stage_ms": {k: last[k] for k in ("ms_jastrow", "ms_integrands", "ms_split", "ms_gemm1", "ms_gemm2",
+                                          "ms_symm")},
+        "roofline": roof,
+    })
+    if fp64 is not None and args.precision != "fp64":
+        out["fp64_build_s"] = fp64["build_s"]
+        out["fp64_eff_tflops"] = fp64["eff_tflops"]
+        out["speedup_vs_fp64"] = fp64["build_s"] / (ms / 1e3)
+        fp64_g1 = fp64["stats"]["ms_gemm1"] / 1e3
+        out["fp64_roofline"] = {"kernel": "gemm_f64_kernel<0,0,*>", "achieved": exec1 / fp64_g1 / 1e12,
+                                "peak": PEAK["fp64"], "frac": exec1 / fp64_g1 / 1e12 / PEAK["fp64"]}
+        err = errors_vs_fp64()
         out["errors_vs_fp64"] = err
         out["meets_1e-10"] = err["max_rel_frob"] <= 1e-10
+        # further split variants against the same FP64 build (one warm-up build each)
+        variants = []
+        for v in filter(None, args.variants.split(",")):
+            vp, vs = v.split(":")
+            vopt = cutc.options(vp, int(vs))
+            build(cutc, ds, X, I, vopt, stream)
+            vms, vst = timed_builds(cutc, torch, ds, X, I, vopt, 1, stream)
+            cutc.release_workspace()
+            verr = errors_vs_fp64()
+            variants.append({"precision": f"{vp}x{vs}", "build_s": vms / 1e3, "eff_tflops": W / (vms / 1e3) / 1e12,
+                             "speedup_vs_fp64": fp64["build_s"] / (vms / 1e3), "stage_ms": {
+                                 k: vst[-1][k] for k in ("ms_jastrow", "ms_split", "ms_gemm1", "ms_gemm2")},
+                             "errors_vs_fp64": verr})
+            log(args, f"variant {vp}x{vs}: {vms / 1e3:.2f} s, max rel frob {verr['max_rel_frob']:.2e}")
+        if variants:
+            out["variants"] = variants
     del X, I, ds
     torch.cuda.empty_cache()
     out["cpu_baseline"] = cpu_baseline(s, args.cpu_sample) if args.cpu_sample > 0 else None

@@ -6,7 +6,8 @@
  * technique"): each operand row x is scaled by 2^(bits - e), ||x||_2 < 2^e,
  * and rounded to integers q; the exact integer product V = sum_k a_k b_k is
  * obtained from its residues modulo n pairwise-coprime moduli m_i <= 256,
- * each residue product being an int8 x int8 -> int32 GEMM; V is rebuilt by
+ * each residue product being an int8 x int8 -> int32 GEMM (or, with moduli
+ * <= 255, a bf16 / fp16 GEMM with exact FP32 sums); V is rebuilt by
  * the Chinese remainder theorem and scaled back to FP64.  Correct
  * reconstruction needs |V| < M/2 with M = prod m_i.  Rounding moves a row by
  * at most sqrt(K)/2 in 2-norm, so ||q||_2 < 2^bits + sqrt(K)/2 and, by
@@ -27,8 +28,13 @@
 #include "cutc_int.h"
 #include "kernels/launch.h"
 
+/* int8 residues: pairwise-coprime moduli <= 256 (largest first) */
 static const int k_moduli[CUTC_CRT_MAX] = {256, 255, 253, 251, 247, 241, 239, 233,
                                            229, 227, 223, 217, 211, 199, 197, 193};
+/* bf16 / fp16 residues (|r| <= 127, which keeps 1024 FP32 products exact):
+ * the int8 set without 256 */
+static const int k_moduli16[CUTC_CRT_MAX] = {255, 253, 251, 247, 241, 239, 233, 229,
+                                             227, 223, 217, 211, 199, 197, 193, 191};
 
 typedef unsigned __int128 u128;
 
@@ -55,17 +61,20 @@ static void split3(u128 v, double out[3])
     out[0] = (double)(uint64_t)(v >> 80);
 }
 
-int cutc_crt_setup(int64_t K, int n, cutc_crt *c)
+int cutc_crt_setup(int64_t K, int n, int kind, cutc_crt *c)
 {
     memset(c, 0, sizeof *c);
-    if (n < 2 || n > CUTC_CRT_MAX || K < 1)
+    if (n < 2 || n > CUTC_CRT_MAX || K < 1 || kind < CUTC_CRT_I8 || kind > CUTC_CRT_FP16)
         return CUTC_ERR_ARG;
+    const int *mods = kind == CUTC_CRT_I8 ? k_moduli : k_moduli16;
     u128 M = 1;
     double log2M = 0.0;
-    for (int i = 0; i < n; i++) {
-        M *= (u128)k_moduli[i];
-        log2M += log2((double)k_moduli[i]);
-    }
+    for (int i = 0; i < n; i++)
+        log2M += log2((double)mods[i]);
+    if (log2M > 127.0) /* M and the sums below must fit 128 bits */
+        return CUTC_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        M *= (u128)mods[i];
     /* largest bits with 2 log2(2^bits + sqrt(K)/2) < log2(M) - 1, with a
      * margin far above the rounding of log2M */
     const double half_rootk = 0.5 * sqrt((double)K);
@@ -76,9 +85,12 @@ int cutc_crt_setup(int64_t K, int n, cutc_crt *c)
         return CUTC_ERR_ARG;
     c->n = n;
     c->bits = bits;
+    c->kind = kind;
     for (int i = 0; i < n; i++) {
-        const int m = k_moduli[i];
+        const int m = mods[i];
         const u128 Mi = M / (u128)m;
+        if (Mi >> 120) /* three 40-bit parts */
+            return CUTC_ERR_ARG;
         c->m[i] = m;
         c->inv_m[i] = 1.0 / (double)m;
         c->y[i] = inv_mod((int)(Mi % (u128)m), m);

@@ -151,12 +151,27 @@ void cutc_default_options(cutc_options *opt)
  * 2-norm row scaling, 11 moduli keep the contractions of the 20-nucleus,
  * n_ao = 200 workload near 1e-11 relative error. */
 #define CUTC_INT8_DEFAULT_MODULI 11
+/* Moduli of the bf16 / fp16 modular contraction (moduli <= 255, about 7.9
+ * bits each): 10 carry as many bits as 10 int8 moduli. */
+#define CUTC_CRT16_DEFAULT_MODULI 10
+
+static int is_crt(int precision)
+{
+    return precision == CUTC_PREC_INT8 || precision == CUTC_PREC_BF16_CRT || precision == CUTC_PREC_FP16_CRT;
+}
+
+/* residue-plane kind of a modular precision */
+static int crt_kind(int precision)
+{
+    return precision == CUTC_PREC_BF16_CRT ? CUTC_CRT_BF16
+                                           : (precision == CUTC_PREC_FP16_CRT ? CUTC_CRT_FP16 : CUTC_CRT_I8);
+}
 
 static int valid_precision(const cutc_options *o)
 {
     if (o->precision == CUTC_PREC_FP64)
         return 1;
-    if (o->precision == CUTC_PREC_INT8 && o->slices >= 6 && o->slices <= CUTC_CRT_MAX)
+    if (is_crt(o->precision) && o->slices >= 6 && o->slices <= CUTC_CRT_MAX)
         return 1;
     if ((o->precision == CUTC_PREC_BF16 || o->precision == CUTC_PREC_FP16) && o->slices >= 2 && o->slices <= 4)
         return 1;
@@ -169,6 +184,8 @@ static cutc_options resolved_options(const cutc_options *in)
     cutc_options o = *in;
     if (o.precision == CUTC_PREC_INT8 && o.slices == 0)
         o.slices = CUTC_INT8_DEFAULT_MODULI;
+    if ((o.precision == CUTC_PREC_BF16_CRT || o.precision == CUTC_PREC_FP16_CRT) && o.slices == 0)
+        o.slices = CUTC_CRT16_DEFAULT_MODULI;
     if ((o.precision == CUTC_PREC_BF16 || o.precision == CUTC_PREC_FP16) && o.slices == 0)
         o.slices = 4;
     return o;
@@ -448,10 +465,10 @@ done:
     return rc;
 }
 
-/* Stage 1 with a split-precision contraction.  INT8: the modular (CRT)
- * scheme, pair residues quantised on the fly from w2 and chi, each G chunk
- * quantised after the Jastrow block, one int8 GEMM per modulus, CRT rebuild
- * scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
+/* Stage 1 with a split-precision contraction.  INT8 / BF16_CRT / FP16_CRT:
+ * the modular (CRT) scheme, pair residues quantised on the fly from w2 and
+ * chi, each G chunk quantised after the Jastrow block, one int8 (or bf16 /
+ * fp16) GEMM per modulus, CRT rebuild scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
  * pairs and of each G chunk, one slice GEMM into FP64, scattered into X. */
 static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                         const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
@@ -468,15 +485,17 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     int *rexpA = NULL, *rexpG = NULL;
     int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
-    const int slice = opt->precision != CUTC_PREC_INT8, f16 = opt->precision == CUTC_PREC_FP16;
+    const int slice = !is_crt(opt->precision), f16 = opt->precision == CUTC_PREC_FP16;
     const int emode = slice ? CUTC_ROWEXP_MAX : CUTC_ROWEXP_NORM;
+    const size_t rb = CUTC_CRT_RBYTES(crt_kind(opt->precision));
     cutc_crt crt;
     if (!slice)
-        TRY(cutc_crt_setup(n2, nmod, &crt));
-    /* int8: residue planes and int8 results; slices: 16-bit digit planes and FP64 results */
-    const size_t bA = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * ldk : (size_t)nmod * CUTC_PK_ROWS_B(P) * ldk;
+        TRY(cutc_crt_setup(n2, nmod, crt_kind(opt->precision), &crt));
+    /* modular: residue planes (int8, or 16-bit for bf16 / fp16) and int8
+     * results; slices: 16-bit digit planes and FP64 results */
+    const size_t bA = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * ldk : rb * nmod * CUTC_PK_ROWS_B(P) * ldk;
     const size_t bG = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(4 * n1c) * ldk
-                            : (size_t)nmod * CUTC_PK_ROWS_A(4 * n1c) * ldk;
+                            : rb * nmod * CUTC_PK_ROWS_A(4 * n1c) * ldk;
     const size_t bC = slice ? sizeof(double) * 4 * n1c * P : (size_t)nmod * 4 * n1c * P;
 
     ws_t ws;
@@ -684,9 +703,10 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
     int rc = CUTC_OK, ev;
     const int split = opt->precision != CUTC_PREC_FP64;
-    const int slice = split && opt->precision != CUTC_PREC_INT8, f16 = opt->precision == CUTC_PREC_FP16;
+    const int slice = split && !is_crt(opt->precision), f16 = opt->precision == CUTC_PREC_FP16;
     const int emode = slice ? CUTC_ROWEXP_MAX : CUTC_ROWEXP_NORM;
     const int nmod = opt->slices;
+    const size_t rb = CUTC_CRT_RBYTES(crt_kind(opt->precision));
 
     const int64_t rows = (n1c + 15) / 16 * 16;
     const int64_t plane = rows * P;
@@ -694,7 +714,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     ws_t ws;
     /* operand planes of both sides and the contraction's results */
     const size_t bpl = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * 3 * kp_max
-                             : (size_t)nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
+                             : rb * nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
     const size_t bcp = slice ? sizeof(double) * P * P : (size_t)nmod * P * P;
     {
         size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
@@ -763,7 +783,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 const int nseg = part == 0 ? 1 : 3;
                 cutc_crt crt;
                 if (!slice)
-                    TRY(cutc_crt_setup(nseg * n1e, nmod, &crt));
+                    TRY(cutc_crt_setup(nseg * n1e, nmod, crt_kind(opt->precision), &crt));
                 cutc_colsrc sa, sb;
                 memset(&sa, 0, sizeof sa);
                 memset(&sb, 0, sizeof sb);

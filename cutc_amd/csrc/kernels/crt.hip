@@ -1,18 +1,25 @@
 // Modular (CRT) split-precision contractions for gfx950: operands rounded to
-// integers per row, products taken modulo n moduli <= 256 with
-// v_mfma_i32_32x32x32_i8 (exact int32 accumulation), rebuilt by CRT into
+// integers per row, products taken modulo n moduli, rebuilt by CRT into
 // FP64.  Scheme: Ozaki, Uchino & Imamura, "Ozaki Scheme II" (constants and
-// the rebuild formula in crt_host.c).
+// the rebuild formula in crt_host.c).  Two plane kinds: int8 residues
+// (moduli <= 256) with v_mfma_i32_32x32x32_i8 (exact int32 accumulation),
+// and residues |r| <= 127 (moduli <= 255) held as bf16 / fp16 values with
+// v_mfma_f32_32x32x16_{bf16,f16}, exact because the FP32 accumulators are
+// reduced modulo m often enough to stay below 2^24.
 //
 // Kernels:
 //   rowexp_*        row exponents e with ||x_row||_2 < 2^e or max|x_row| < 2^e
 //                   (fixed-order reductions)
 //   quant_*_pk      x -> q = rint(x 2^(bits - e_row)) -> residues q mod m_i,
-//                   int8 planes in the fragment-packed layout (launch.h)
+//                   int8 or 16-bit planes in the fragment-packed layouts
+//                   (launch.h)
 //   crt_gemm_pk     per modulus: 512 x 128 block tile, 4 waves of 128 x 128
-//                   (4 x 4 MFMA 32x32x32); A fragments streamed into a
+//                   (4 x 4 MFMA 32x32x32 i8); A fragments streamed into a
 //                   register ring, B through an LDS ring fed by LDS DMA;
 //                   int8 residues C[i][r][p]
+//   crt16_gemm      the same tiling and pipeline with the 16-bit MFMA and
+//                   FP32 accumulators reduced modulo m every 960 K; int8
+//                   residues
 //   rebuild_*       CRT of the n residues per element, scaling, and the
 //                   scatter to the reference's full layouts (as the FP64
 //                   epilogues)
@@ -22,16 +29,20 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
-
-// residue of the integer-valued double q (|q| < 2^52) modulo m, in [-128, 127]
+// residue of the integer-valued double q (|q| < 2^52) modulo m, in
+// [-(m / 2), (m - 1) / 2], within [-128, 127] for m <= 256 (the rounded
+// quotient is within 1 of q / m, so one correction suffices)
 __device__ static inline int residue(double q, int m, double inv_m)
 {
     const double t = rint(q * inv_m);
     int r = (int)fma(-t, (double)m, q);
-    if (r > 127)
+    if (r > ((m - 1) >> 1))
         r -= m;
-    if (r < -128)
+    if (r < -(m >> 1))
         r += m;
     return r;
 }
@@ -191,32 +202,15 @@ extern "C" int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, 
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// ---------------------------------------------------------------- quantisation
+// ---------------------------------------------------------------- residues
 
-// 16 values of one row -> n residue planes, 16 bytes each
-__device__ static inline void store_residues(const cutc_crt &c, const double (&q)[16], int8_t *dst, int64_t plane)
-{
-    for (int i = 0; i < c.n; i++) {
-        int w[4];
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-            unsigned u = 0;
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                u |= ((unsigned)residue(q[4 * g + e], c.m[i], c.inv_m[i]) & 0xffu) << (8 * e);
-            w[g] = (int)u;
-        }
-        *(v4i *)(dst + i * plane) = v4i{w[0], w[1], w[2], w[3]};
-    }
-}
-
-// x mod m into [-128, 127] for |x| < 2^31
+// x mod m into [-(m / 2), (m - 1) / 2] for |x| < 2^31
 __device__ static inline int reduce_mod(int x, int m, double inv_m) { return residue((double)x, m, inv_m); }
 
-// ------------------------------------------------ fragment-packed stage 1
+// ------------------------------------------------ fragment-packed planes
 
-// Packed byte offset of (row, k) for rows of row block rb: the layout of the
-// residue planes fed to crt_gemm_pk_kernel (see launch.h).
+// Packed byte offset of (row 32 rb + r, k) in an int8 plane (k a multiple
+// of 16): the layout fed to crt_gemm_pk_kernel (launch.h), KT = kpad / 64.
 __device__ static inline int64_t pk_offset(int64_t rb, int r, int64_t k, int64_t KT)
 {
     const int64_t kt = k >> 6;
@@ -225,9 +219,88 @@ __device__ static inline int64_t pk_offset(int64_t rb, int r, int64_t k, int64_t
     return (rb * KT + kt) * 2048 + c * 1024 + (32 * h + r) * 16;
 }
 
-// grid (kpad / 128, ceil(rows / 32)): a block covers one row block and two K
-// steps; thread r = tid % 32 takes row 32 rb + r, 16 bytes j = tid / 32, so
-// the 16-byte stores of consecutive threads are consecutive.
+// the same for a 16-bit plane (k a multiple of 8), KT = kpad / 32
+__device__ static inline int64_t pk16_offset(int64_t rb, int r, int64_t k, int64_t KT)
+{
+    const int64_t kt = k >> 5;
+    const int kk = (int)(k & 31);
+    const int c = kk >> 4, h = (kk >> 3) & 1;
+    return (rb * KT + kt) * 2048 + c * 1024 + (32 * h + r) * 16;
+}
+
+// 16-bit pattern of the integer r (|r| <= 256) as a bf16 or fp16 value:
+// exact in both, so the bf16 pattern is the upper half of the float's bits
+template <int KIND> __device__ static inline unsigned half_bits(int r)
+{
+    if constexpr (KIND == CUTC_CRT_FP16)
+        return __builtin_bit_cast(uint16_t, (_Float16)(float)r);
+    else
+        return __float_as_uint((float)r) >> 16;
+}
+
+// K steps per plane of kpad columns
+__host__ __device__ static inline int64_t pk_steps(int kind, int64_t kpad)
+{
+    return kind == CUTC_CRT_I8 ? kpad / 64 : kpad / 32;
+}
+
+// 16 values k0 .. k0 + 15 (k0 a multiple of 16) of row 32 rb + r -> n
+// residue planes: one 16-byte int8 chunk, or two 16-byte chunks of 8
+// 16-bit residues
+template <int KIND>
+__device__ static inline void store_residues(const cutc_crt &c, const double (&q)[16], int8_t *planes, int64_t rb,
+                                             int r, int64_t k0, int64_t KT, int64_t plane)
+{
+    if constexpr (KIND == CUTC_CRT_I8) {
+        int8_t *dst = planes + pk_offset(rb, r, k0, KT);
+        for (int i = 0; i < c.n; i++) {
+            int w[4];
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                unsigned u = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    u |= ((unsigned)residue(q[4 * g + e], c.m[i], c.inv_m[i]) & 0xffu) << (8 * e);
+                w[g] = (int)u;
+            }
+            *(v4i *)(dst + i * plane) = v4i{w[0], w[1], w[2], w[3]};
+        }
+    } else {
+        int8_t *dst0 = planes + pk16_offset(rb, r, k0, KT);
+        int8_t *dst1 = planes + pk16_offset(rb, r, k0 + 8, KT);
+        for (int i = 0; i < c.n; i++) {
+            int w[8];
+#pragma unroll
+            for (int g = 0; g < 8; g++)
+                w[g] = (int)(half_bits<KIND>(residue(q[2 * g], c.m[i], c.inv_m[i])) |
+                             (half_bits<KIND>(residue(q[2 * g + 1], c.m[i], c.inv_m[i])) << 16));
+            *(v4i *)(dst0 + i * plane) = v4i{w[0], w[1], w[2], w[3]};
+            *(v4i *)(dst1 + i * plane) = v4i{w[4], w[5], w[6], w[7]};
+        }
+    }
+}
+
+// kind -> template instance
+#define CRT_KIND_DISPATCH(kind, F)                                                                                     \
+    do {                                                                                                               \
+        if ((kind) == CUTC_CRT_I8) {                                                                                   \
+            F(CUTC_CRT_I8);                                                                                            \
+        } else if ((kind) == CUTC_CRT_BF16) {                                                                          \
+            F(CUTC_CRT_BF16);                                                                                          \
+        } else if ((kind) == CUTC_CRT_FP16) {                                                                          \
+            F(CUTC_CRT_FP16);                                                                                          \
+        } else {                                                                                                       \
+            return CUTC_ERR_ARG;                                                                                       \
+        }                                                                                                              \
+    } while (0)
+
+// ---------------------------------------------------------------- quantisation
+
+// grid (kpad / 128, ceil(rows / 32)): a block covers one row block and 128
+// K; thread r = tid % 32 takes row 32 rb + r, the 16 values from 16 j
+// (j = tid / 32), so the 16-byte stores of consecutive threads are
+// consecutive.
+template <int KIND>
 __global__ __launch_bounds__(256) void quant_k_pk_kernel(cutc_crt c, const double *__restrict__ X, int64_t ld,
                                                          int64_t rows, int64_t K, int64_t KT, int64_t plane,
                                                          const int *__restrict__ rexp, int8_t *planes)
@@ -247,7 +320,7 @@ __global__ __launch_bounds__(256) void quant_k_pk_kernel(cutc_crt c, const doubl
         for (int e = 0; e < 16; e++)
             q[e] = 0.0;
     }
-    store_residues(c, q, planes + pk_offset(rb, r, k0, KT), plane);
+    store_residues<KIND>(c, q, planes, rb, r, k0, KT, plane);
 }
 
 extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K,
@@ -257,13 +330,18 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
         return 0;
     if (kpad % CUTC_PK_KALIGN || (rows + 31) / 32 > 65535)
         return CUTC_ERR_ARG;
-    const int64_t plane = CUTC_PK_ROWS_A(rows) * kpad;
-    hipLaunchKernelGGL(quant_k_pk_kernel, dim3((unsigned)(kpad / 128), (unsigned)((rows + 31) / 32)), dim3(256), 0,
-                       st, *c, X, ld, rows, K, kpad / 64, plane, rexp, planes);
+    const int64_t plane = CUTC_PK_ROWS_A(rows) * kpad * CUTC_CRT_RBYTES(c->kind);
+    const dim3 grid((unsigned)(kpad / 128), (unsigned)((rows + 31) / 32));
+#define QK(K_)                                                                                                         \
+    hipLaunchKernelGGL(quant_k_pk_kernel<K_>, grid, dim3(256), 0, st, *c, X, ld, rows, K, pk_steps(K_, kpad), plane,   \
+                       rexp, planes)
+    CRT_KIND_DISPATCH(c->kind, QK);
+#undef QK
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
 // pair rows p = (k, l) of A[p][r2] = (w chi_l) chi_k, computed on the fly
+template <int KIND>
 __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const double *__restrict__ wr2,
                                                              const double *__restrict__ aos2, int64_t n2, int n_ao,
                                                              int64_t KT, int64_t plane,
@@ -288,7 +366,7 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
         for (int e = 0; e < 16; e++)
             q[e] = 0.0;
     }
-    store_residues(c, q, planes + pk_offset(rb, r, k0, KT), plane);
+    store_residues<KIND>(c, q, planes, rb, r, k0, KT, plane);
 }
 
 extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2,
@@ -299,9 +377,13 @@ extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, con
         return 0;
     if (kpad % CUTC_PK_KALIGN || (P + 31) / 32 > 65535)
         return CUTC_ERR_ARG;
-    const int64_t plane = CUTC_PK_ROWS_B(P) * kpad;
-    hipLaunchKernelGGL(quant_pairw_pk_kernel, dim3((unsigned)(kpad / 128), (unsigned)((P + 31) / 32)), dim3(256), 0,
-                       st, *c, wr2, aos2, n2, n_ao, kpad / 64, plane, rexp, planes);
+    const int64_t plane = CUTC_PK_ROWS_B(P) * kpad * CUTC_CRT_RBYTES(c->kind);
+    const dim3 grid((unsigned)(kpad / 128), (unsigned)((P + 31) / 32));
+#define QP(K_)                                                                                                         \
+    hipLaunchKernelGGL(quant_pairw_pk_kernel<K_>, grid, dim3(256), 0, st, *c, wr2, aos2, n2, n_ao,                     \
+                       pk_steps(K_, kpad), plane, rexp, planes)
+    CRT_KIND_DISPATCH(c->kind, QP);
+#undef QP
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -317,6 +399,7 @@ __device__ static inline int shrink_mod(int x, int m, float inv_m)
 // segment, read coalesced along the rows into LDS and written as 16-byte
 // fragments.  grid (ceil(rows / 32), kp / 128, nsrc); segment s occupies K
 // [s kp, (s + 1) kp), zero past the valid s.K.
+template <int KIND>
 __global__ __launch_bounds__(256) void quant_colk_pk_kernel(cutc_crt c, cutc_colsrc s, int64_t rows, int64_t kp,
                                                             int64_t KT, int64_t plane,
                                                             const int *__restrict__ rexp, int8_t *planes)
@@ -343,7 +426,7 @@ __global__ __launch_bounds__(256) void quant_colk_pk_kernel(cutc_crt c, cutc_col
 #pragma unroll
     for (int e = 0; e < 16; e++)
         q[e] = rint(ldexp(tile[16 * j + e][r], sh));
-    store_residues(c, q, planes + pk_offset(rb, r, seg * kp + k0 + 16 * j, KT), plane);
+    store_residues<KIND>(c, q, planes, rb, r, seg * kp + k0 + 16 * j, KT, plane);
 }
 
 extern "C" int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp,
@@ -355,8 +438,12 @@ extern "C" int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src,
         (rows + 31) / 32 > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t ktot = kp * src->nsrc;
-    hipLaunchKernelGGL(quant_colk_pk_kernel, dim3((unsigned)((rows + 31) / 32), (unsigned)(kp / 128), src->nsrc),
-                       dim3(256), 0, st, *c, *src, rows, kp, ktot / 64, plane_rows * ktot, rexp, planes);
+    const dim3 grid((unsigned)((rows + 31) / 32), (unsigned)(kp / 128), src->nsrc);
+#define QC(K_)                                                                                                         \
+    hipLaunchKernelGGL(quant_colk_pk_kernel<K_>, grid, dim3(256), 0, st, *c, *src, rows, kp, pk_steps(K_, ktot),       \
+                       plane_rows * ktot * CUTC_CRT_RBYTES(K_), rexp, planes)
+    CRT_KIND_DISPATCH(c->kind, QC);
+#undef QC
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -532,24 +619,187 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     }
 }
 
+// v_mfma_f32_32x32x16_{bf16,f16} on two 16-byte fragments
+template <bool F16> __device__ static inline v16f mfma16(v4i a, v4i b, v16f c)
+{
+    if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0,
+                                                      0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0,
+                                                       0, 0);
+}
+
+// x mod m for an integer-valued float |x| < 2^24: x - m rint(x / m) with
+// one rounding of an exact result below 1.5 m (the float quotient is within
+// 1 of x / m), so the value stays an exact integer
+__device__ static inline float fmod_small(float x, float m, float inv_m)
+{
+    return __builtin_fmaf(-m, __builtin_rintf(x * inv_m), x);
+}
+
+// The 16-bit counterpart of crt_gemm_pk_kernel: the same block tile (512 A
+// rows x 128 B rows, 4 waves of 128 x 128), A register ring, B LDS ring and
+// load placement, with v_mfma_f32_32x32x16 (a 64-byte step is 32 K, two
+// MFMAs per tile) and FP32 accumulators.  Residues are at most 127 in
+// magnitude (moduli <= 255), so an accumulator reduced modulo m
+// (|x| <= 1.5 m) takes RW groups of D steps (32 D RW K, at most 1024) more
+// and stays below 1024 * 127^2 + 383 < 2^24: exact.  Every RW-th group ends
+// with the reduction of all accumulators.
+template <bool F16, int D, int L, int BAR, int RW>
+__global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__restrict__ A,
+                                                            const int8_t *__restrict__ B, int64_t KT, int tiles_a,
+                                                            int tiles_b, int64_t M, int64_t N, int mod,
+                                                            double inv_mod, int8_t *__restrict__ C)
+{
+    static_assert(D * RW * 32 * 127 * 127 + 383 < (1 << 24), "exact FP32 sums");
+    static_assert(D >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
+    __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * PK_TILE];
+
+    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    int ta, tb;
+    cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t rbs = KT * PK_TILE; // bytes per row block
+    const int8_t *Aw = A + ((int64_t)ta * 16 + wave * 4) * rbs;
+    const int8_t *Bw = B + ((int64_t)tb * 4 + wave) * rbs;
+    const uint32_t lo = (uint32_t)lane * 16;
+    const float mf = (float)mod, imf = (float)inv_mod;
+    constexpr int WAIT_NEXT = 8 + 10 * (L - BAR - 1);
+
+    v4i ar[D][4][2];
+    v4i bf0[4], bf1[4];
+    v16f acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[i][j] = v16f{};
+
+    auto issue = [&](auto slot_c, int64_t s) {
+        constexpr int slot = decltype(slot_c)::value;
+        const uint32_t so = (uint32_t)s * PK_TILE;
+        pk_dma16(Bw + so + lo, &sB[slot][wave * PK_TILE]);
+        pk_dma16(Bw + so + 1024 + lo, &sB[slot][wave * PK_TILE + 1024]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            ar[slot][i][0] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + lo));
+            ar[slot][i][1] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + 1024 + lo));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    auto step = [&](auto u_c, int64_t k) {
+        constexpr int u = decltype(u_c)::value;
+        constexpr int slot = (u + L) % D;
+        if constexpr (u % BAR == 0) {
+            PK_WAIT_VM(WAIT_NEXT);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            bf1[j] = *(const v4i *)&sB[u][j * PK_TILE + 1024 + lo];
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t so = (uint32_t)(k + L < KT ? k + L : KT - 1) * PK_TILE;
+        cutc_static_for<0, 16>([&](auto q_c) {
+            constexpr int q = decltype(q_c)::value;
+            constexpr int i = q / 4, j = q % 4;
+            acc[i][j] = mfma16<F16>(ar[u][i][0], bf0[j], acc[i][j]);
+            if constexpr (q < 2) {
+                pk_dma16(Bw + so + q * 1024 + lo, &sB[slot][wave * PK_TILE + q * 1024]);
+            } else if constexpr (q < 10) {
+                constexpr int ii = (q - 2) / 2, cc = (q - 2) % 2;
+                ar[slot][ii][cc] = *(const v4i *)(Aw + ((uint32_t)ii * (uint32_t)rbs + so + cc * 1024 + lo));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        cutc_static_for<0, 16>([&](auto q_c) {
+            constexpr int q = decltype(q_c)::value;
+            constexpr int i = q / 4, j = q % 4;
+            acc[i][j] = mfma16<F16>(ar[u][i][1], bf1[j], acc[i][j]);
+            if constexpr (q < 4)
+                bf0[q] = *(const v4i *)&sB[(u + 1) % D][q * PK_TILE + lo];
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    };
+
+    // prologue: steps 0 .. L - 1 in flight; DMA of the first BAR + 1 complete
+    cutc_static_for<0, L>([&](auto s_c) {
+        constexpr int s = decltype(s_c)::value;
+        issue(s_c, (s < KT ? s : KT - 1));
+    });
+    PK_WAIT_VM(WAIT_NEXT);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        bf0[j] = *(const v4i *)&sB[0][j * PK_TILE + lo];
+
+    for (int64_t k0 = 0; k0 < KT; k0 += D * RW) {
+        const int64_t k1 = (KT - k0 < D * RW) ? KT : k0 + D * RW;
+        for (int64_t kb = k0; kb < k1; kb += D)
+            cutc_static_for<0, D>([&](auto u_c) { step(u_c, kb + decltype(u_c)::value); });
+        // one tile at a time, so that few accumulators sit in VGPRs (the
+        // loads of the next steps stay in flight)
+        cutc_static_for<0, 16>([&](auto q_c) {
+            constexpr int q = decltype(q_c)::value;
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc[q / 4][q % 4][r] = fmod_small(acc[q / 4][q % 4][r], mf, imf);
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    }
+    PK_WAIT_VM(0); // the repeated loads past the last step
+
+    // result element r of lane l: A row 8 (r/4) + 4 (l/32) + r%4, B row l%32
+    const int li = lane & 31, lg = lane >> 5;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t col = (int64_t)tb * 128 + 32 * j + li;
+            if (col >= N)
+                continue;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int64_t row = (int64_t)ta * 512 + wave * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
+                if (row < M)
+                    C[row * N + col] = (int8_t)reduce_mod((int)fmod_small(acc[i][j][r], mf, imf), mod, inv_mod);
+            }
+        }
+    }
+}
+
 extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
                                 int64_t kpad, int8_t *C, hipStream_t st)
 {
     if (M <= 0 || N <= 0)
         return 0;
-    if (kpad <= 0 || kpad % CUTC_PK_KALIGN)
+    if (kpad <= 0 || kpad % CUTC_PK_KALIGN || c->kind < CUTC_CRT_I8 || c->kind > CUTC_CRT_FP16)
         return CUTC_ERR_ARG;
-    const int64_t KT = kpad / 64;
+    const int64_t KT = pk_steps(c->kind, kpad), rb = CUTC_CRT_RBYTES(c->kind);
     if (4 * KT * PK_TILE > 4294967295LL) /* 32-bit offsets inside a wave's four row blocks */
         return CUTC_ERR_ARG;
     const int64_t ta = CUTC_PK_ROWS_A(M) / 512, tb = CUTC_PK_ROWS_B(N) / 128;
     if (ta * tb > 2147483647)
         return CUTC_ERR_ARG;
-    const int64_t pa = CUTC_PK_ROWS_A(M) * kpad, pb = CUTC_PK_ROWS_B(N) * kpad;
-    /* 6-stage ring, loads 4 steps ahead, a barrier every second step */
-    for (int i = 0; i < c->n; i++)
-        hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040>), dim3((unsigned)(ta * tb)), dim3(256), 0, st, A + i * pa,
-                           B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], C + (int64_t)i * M * N);
+    const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
+    const dim3 grid((unsigned)(ta * tb));
+    for (int i = 0; i < c->n; i++) {
+        /* 6-stage rings, loads 4 steps ahead, a barrier every second step;
+         * 16-bit: a reduction every 5 groups of 6 steps (960 K) */
+        int8_t *Ci = C + (int64_t)i * M * N;
+        if (c->kind == CUTC_CRT_I8)
+            hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040>), grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT,
+                               (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci);
+        else if (c->kind == CUTC_CRT_BF16)
+            hipLaunchKernelGGL((crt16_gemm_kernel<false, 6, 4, 2, 5>), grid, dim3(256), 0, st, A + i * pa, B + i * pb,
+                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci);
+        else
+            hipLaunchKernelGGL((crt16_gemm_kernel<true, 6, 4, 2, 5>), grid, dim3(256), 0, st, A + i * pa, B + i * pb,
+                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 

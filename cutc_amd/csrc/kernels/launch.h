@@ -83,12 +83,24 @@ int cutc_gemm2_f64(const double *X, int64_t xld, const double *B, int64_t ldb, i
                    const int *pairs, int n_ao, double *M, int mode, hipStream_t st);
 
 /* Modular (CRT) split-precision contractions: crt_host.c (host constants) and
- * crt.hip (kernels).  Residue planes are int8 [n][packed rows x kpad] in the
- * fragment-packed layout below, zero past the valid K.  Each operand row x
+ * crt.hip (kernels).  Residue planes are [n][packed rows x kpad] in the
+ * fragment-packed layouts below, zero past the valid K.  Each operand row x
  * is scaled by 2^(bits - e) with ||x||_2 < 2^e (its row exponent), so that
  * ||q||_2 < 2^bits + sqrt(K)/2 for the rounded row q and, by Cauchy-Schwarz,
- * every integer dot product stays below M/2 in magnitude. */
+ * every integer dot product stays below M/2 in magnitude.
+ *
+ * Kinds: CUTC_CRT_I8, moduli <= 256, int8 residues in [-128, 127] and
+ * v_mfma_i32_32x32x32_i8 (exact int32 sums); CUTC_CRT_BF16 / _FP16, moduli
+ * <= 255, residues |r| <= 127 stored as bf16 / fp16 values (exact in both)
+ * and v_mfma_f32_32x32x16_{bf16,f16}, whose FP32 sums stay exact because
+ * every accumulator is reduced modulo m at least once per 1024 K (at most
+ * 1024 * 127^2 + 1.5 * 255 < 2^24 in between).  GEMM results are int8
+ * residues for every kind. */
 #define CUTC_CRT_MAX 16
+#define CUTC_CRT_I8 0
+#define CUTC_CRT_BF16 1
+#define CUTC_CRT_FP16 2
+#define CUTC_CRT_RBYTES(kind) ((kind) == CUTC_CRT_I8 ? 1 : 2)
 typedef struct cutc_crt {
     int n, bits;
     int m[CUTC_CRT_MAX];
@@ -97,9 +109,10 @@ typedef struct cutc_crt {
     double Mi[CUTC_CRT_MAX][3];
     double M[3];
     double Minv;
+    int kind; /* CUTC_CRT_* */
 } cutc_crt;
 
-int cutc_crt_setup(int64_t K, int n, cutc_crt *c);
+int cutc_crt_setup(int64_t K, int n, int kind, cutc_crt *c);
 
 /* column-major sources for the stage-2 operands: element (row r, k) of
  * segment s is p[s][k * ld + col(r)], col(r) = k_r + n_ao l_r through the
@@ -122,7 +135,7 @@ int cutc_rowexp_pairw(const double *wr2, const double *aos2, int64_t n2, int n_a
 /* partial: CUTC_CRT_KSPLIT x rows doubles of scratch */
 #define CUTC_CRT_KSPLIT 64
 int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, double *partial, int *rexp, hipStream_t st);
-/* Fragment-packed residue planes of the int8 contractions.  Rows are
+/* Fragment-packed residue planes.  int8: rows are
  * grouped in blocks of 32 and K in steps of 64 bytes; the 2048-byte tile of
  * (row block rb, step kt) sits at (rb * KT + kt) * 2048 (KT = kpad / 64) and
  * holds byte k = 64 kt + 32 h + 16 c + e of row 32 rb + r at
@@ -132,7 +145,12 @@ int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, double *par
  * dot products are unchanged.  kpad is a multiple of CUTC_PK_KALIGN (the
  * K steps of every pipeline depth of the GEMM tile it); the
  * G-side planes (A) hold CUTC_PK_ROWS_A(rows) rows, the pair-side planes (B)
- * CUTC_PK_ROWS_B(rows); rows past the valid ones are never stored. */
+ * CUTC_PK_ROWS_B(rows); rows past the valid ones are never stored.
+ * 16-bit kinds: the same 2048-byte tiles, of 32 rows x 32 elements (KT =
+ * kpad / 32), element k = 32 kt + 16 c + 8 h + e of row 32 rb + r at byte
+ * c * 1024 + (32 h + r) * 16 + 2 e (the operands of the c-th
+ * v_mfma_f32_32x32x16 of the step).  Plane bytes: rows x kpad x
+ * CUTC_CRT_RBYTES(kind); every entry point below takes the kind from c. */
 #define CUTC_PK_KALIGN 768
 #define CUTC_PK_ROWS_A(rows) ((((int64_t)(rows)) + 511) / 512 * 512)
 #define CUTC_PK_ROWS_B(rows) ((((int64_t)(rows)) + 127) / 128 * 128)
@@ -145,9 +163,9 @@ int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *
  * CUTC_PK_KALIGN. */
 int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp, int64_t plane_rows,
                            const int *rexp, int8_t *planes, hipStream_t st);
-/* C[i][r][p] = (sum_k A[i][r][k] B[i][p][k]) mod m_i as int8 in [-128, 127]
- * for r < M, p < N, packed planes as above; C holds n M N bytes (row r of the
- * A side is contiguous over p). */
+/* C[i][r][p] = (sum_k A[i][r][k] B[i][p][k]) mod m_i as int8 in
+ * [-(m_i / 2), (m_i - 1) / 2] for r < M, p < N, packed planes as above; C
+ * holds n M N bytes (row r of the A side is contiguous over p). */
 int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
                      int8_t *C, hipStream_t st);
 /* Slice (Ozaki scheme I) contractions, ozaki1.hip: S = 2..4 digit planes of

@@ -39,7 +39,8 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
 {
     if (n1 < 0 || n2 < 0 || n_ao < 0 || n_nuc < 0 || !pl)
         return CUTC_ERR_ARG;
-    const int crt = precision == CUTC_PREC_INT8;
+    const int crt = precision == CUTC_PREC_INT8 || precision == CUTC_PREC_BF16_CRT || precision == CUTC_PREC_FP16_CRT;
+    const int64_t rb = precision == CUTC_PREC_INT8 ? 1 : 2; /* bytes per residue-plane element */
     const int slice = precision == CUTC_PREC_BF16 || precision == CUTC_PREC_FP16;
     const int64_t nmod = (crt || slice) ? slices : 0;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
@@ -50,8 +51,8 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     /* stage 1 */
     int64_t fixed1, per_r1;
     if (crt) {
-        fixed1 = 16 * P + nmod * CUTC_PK_ROWS_B(P) * ldk + nmod * 511 * ldk + 4096;
-        per_r1 = 32 * ldk + 32 + 4 * nmod * ldk + nmod * 4 * P;
+        fixed1 = 16 * P + rb * nmod * CUTC_PK_ROWS_B(P) * ldk + rb * nmod * 511 * ldk + 4096;
+        per_r1 = 32 * ldk + 32 + 4 * rb * nmod * ldk + nmod * 4 * P;
     } else if (slice) { /* digit planes of 2-byte elements, FP64 results */
         fixed1 = 16 * P + 2 * nmod * CUTC_OZ_ROWS(P) * ldk + 2 * nmod * 127 * ldk + 4096;
         per_r1 = 32 * ldk + 32 + 8 * nmod * ldk + 32 * P;
@@ -79,8 +80,9 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     /* stage 2 */
     int64_t fixed2 = 8 * P + 4096, per_r1b = 4 * 8 * P;
     if (crt) {
-        fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + nmod * P * P + 2 * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1);
-        per_r1b += 2 * nmod * 3 * CUTC_PK_ROWS_A(P);
+        fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + nmod * P * P +
+                  2 * rb * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1);
+        per_r1b += 2 * rb * nmod * 3 * CUTC_PK_ROWS_A(P);
     } else if (slice) {
         fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + 8 * P * P +
                   2 * 2 * nmod * CUTC_OZ_ROWS(P) * 3 * (CUTC_PK_KALIGN - 1);

@@ -30,8 +30,11 @@ PREC_FP64 = 0
 PREC_BF16 = 1
 PREC_FP16 = 2
 PREC_INT8 = 3
-PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16, "int8": PREC_INT8}
-DEFAULT_SLICES = {PREC_FP64: 4, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 11}
+PREC_BF16_CRT = 4
+PREC_FP16_CRT = 5
+PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16, "int8": PREC_INT8, "bf16crt": PREC_BF16_CRT,
+              "fp16crt": PREC_FP16_CRT}
+DEFAULT_SLICES = {PREC_FP64: 4, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 11, PREC_BF16_CRT: 10, PREC_FP16_CRT: 10}
 
 
 class CutcError(RuntimeError):
@@ -154,8 +157,9 @@ def _check(rc, where):
 
 
 def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0) -> Options:
-    """slices: bf16/fp16 slices per operand, or the number of moduli for int8
-    (None: 4, resp. 12)."""
+    """slices: bf16/fp16 digit slices per operand, or the number of moduli
+    for int8 / bf16crt / fp16crt (None: the precision's default,
+    DEFAULT_SLICES)."""
     o = Options()
     lib().cutc_default_options(ctypes.byref(o))
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)

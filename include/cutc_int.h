@@ -109,10 +109,14 @@ const char *cutc_error_string(int code);
 #define CUTC_PREC_FP16 2 /* Ozaki slices in fp16, FP32 MFMA accumulate -> FP64         */
 #define CUTC_PREC_INT8 3 /* operands rounded to integers, products modulo `slices`
                             moduli on int8 MFMA (exact int32), CRT rebuild in FP64  */
+#define CUTC_PREC_BF16_CRT 4 /* as INT8 with moduli <= 255, residues as bf16 values,
+                                FP32 MFMA accumulate kept exact, CRT rebuild in FP64 */
+#define CUTC_PREC_FP16_CRT 5 /* the same with fp16 residues                          */
 
 typedef struct cutc_options {
     int precision;    /* CUTC_PREC_*                                             */
-    int slices;       /* bf16/fp16: slices per operand (3, 4); int8: moduli (6..16, 0 = 11) */
+    int slices;       /* bf16/fp16: slices per operand (2..4, 0 = 4); int8: moduli
+                         (6..16, 0 = 11); bf16/fp16 CRT: moduli (6..16, 0 = 10) */
     int verbose;      /* 1: print the reference's stage-time lines               */
     int n1_chunk;     /* r1 rows per G chunk, 0 = planner                        */
     size_t mem_limit; /* device bytes the workspaces may use, 0 = free memory    */

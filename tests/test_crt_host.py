@@ -16,18 +16,21 @@ from cutc_amd import cutc
 NMAX = 16
 
 
+KIND_I8, KIND_BF16, KIND_FP16 = 0, 1, 2
+
+
 class Crt(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("bits", ctypes.c_int), ("m", ctypes.c_int * NMAX), ("y", ctypes.c_int * NMAX),
                 ("inv_m", ctypes.c_double * NMAX), ("Mi", (ctypes.c_double * 3) * NMAX), ("M", ctypes.c_double * 3),
-                ("Minv", ctypes.c_double)]
+                ("Minv", ctypes.c_double), ("kind", ctypes.c_int)]
 
 
-def setup(K, n):
+def setup(K, n, kind=KIND_I8):
     L = cutc.lib()
     L.cutc_crt_setup.restype = ctypes.c_int
-    L.cutc_crt_setup.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(Crt)]
+    L.cutc_crt_setup.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Crt)]
     c = Crt()
-    rc = L.cutc_crt_setup(K, n, ctypes.byref(c))
+    rc = L.cutc_crt_setup(K, n, kind, ctypes.byref(c))
     return rc, c
 
 
@@ -35,12 +38,15 @@ def parts(v):
     return int(v[0]) * 2 ** 80 + int(v[1]) * 2 ** 40 + int(v[2])
 
 
+@pytest.mark.parametrize("kind", [KIND_I8, KIND_BF16, KIND_FP16])
 @pytest.mark.parametrize("K,n", [(200000, 12), (60000, 12), (180000, 12), (1, 6), (131072, 16), (1000, 8)])
-def test_constants(K, n):
-    rc, c = setup(K, n)
-    assert rc == 0
+def test_constants(K, n, kind):
+    rc, c = setup(K, n, kind)
+    assert rc == 0 and c.kind == kind
     m = [c.m[i] for i in range(n)]
-    assert all(2 <= x <= 256 for x in m)
+    # int8 residues lie in [-128, 127]; bf16 / fp16 residues within +-127, so
+    # that 1024 products of two stay exact in an FP32 sum
+    assert all(2 <= x <= (256 if kind == KIND_I8 else 255) for x in m)
     for i in range(n):
         for j in range(i):
             assert math.gcd(m[i], m[j]) == 1
@@ -66,16 +72,29 @@ def test_bad_arguments():
     assert setup(1000, 1)[0] != 0
     assert setup(1000, 17)[0] != 0
     assert setup(0, 12)[0] != 0
+    assert setup(1000, 12, 3)[0] != 0
+    assert setup(1000, 12, -1)[0] != 0
 
 
 def residue(q, m, inv_m):
-    """restatement of residue() in crt.hip (float64, same steps)"""
+    """restatement of residue() in crt.hip (float64, same steps): q mod m in
+    [-(m // 2), (m - 1) // 2]"""
     t = np.rint(q * inv_m)
     r = np.fma(-t, float(m), q) if hasattr(np, "fma") else q - t * m
     r = r.astype(np.int64)
-    r = np.where(r > 127, r - m, r)
-    r = np.where(r < -128, r + m, r)
+    r = np.where(r > (m - 1) // 2, r - m, r)
+    r = np.where(r < -(m // 2), r + m, r)
     return r
+
+
+@pytest.mark.parametrize("m", [256, 255, 193, 191])
+def test_residue_range_and_congruence(m):
+    rng = np.random.default_rng(m)
+    q = np.concatenate([rng.integers(-2 ** 51, 2 ** 51, 20000), np.arange(-3 * m, 3 * m),
+                        [2 ** 51 - 1, -2 ** 51 + 1]]).astype(np.float64)
+    r = residue(q, m, 1.0 / m)
+    assert r.min() >= -(m // 2) and r.max() <= (m - 1) // 2
+    assert np.array_equal((q.astype(np.int64) - r) % m, np.zeros_like(r))
 
 
 def crt_value(c, res):
@@ -96,10 +115,11 @@ def crt_value(c, res):
     return (vh * two80 + vm * two40) + vl
 
 
+@pytest.mark.parametrize("kind", [KIND_I8, KIND_BF16])
 @pytest.mark.parametrize("n", [8, 12, 16])
-def test_rebuild_recovers_integers(n):
+def test_rebuild_recovers_integers(n, kind):
     K = 200000
-    rc, c = setup(K, n)
+    rc, c = setup(K, n, kind)
     rng = np.random.default_rng(n)
     bound = (2 ** c.bits + math.isqrt(K) // 2 + 1) ** 2  # largest |V| the scaling allows
     vals = [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 2000)]
