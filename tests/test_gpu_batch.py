@@ -30,7 +30,7 @@ def systems():
             synth.make_system(7, 55, 5000, 1800, 1105, name="b4")]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "int8", "bf16"])
+@pytest.mark.parametrize("precision", ["fp64", "int8", "bf16", "bf16crt"])
 def test_batch_matches_single_calls(torch, precision):
     sys_ = systems()
     print(f"batch {precision}: {[s.shape_str() for s in sys_]}", flush=True)
@@ -53,7 +53,7 @@ def test_batch_matches_single_calls(torch, precision):
     Io = oracle.int_2e_ao(s, oracle.int2_grad1_u12_ao(s))
     e = relerr(Ib[1].cpu().numpy(), Io)
     print(f"  {s.name} vs oracle {e:.2e}", flush=True)
-    assert e <= {"fp64": 1e-12, "int8": 1e-10, "bf16": 1e-6}[precision]
+    assert e <= {"fp64": 1e-12, "int8": 1e-10, "bf16": 1e-6, "bf16crt": 1e-10}[precision]
 
 
 def test_batch_replays_and_empty(torch):
