@@ -243,7 +243,8 @@ def main():
     stream = torch.cuda.current_stream()
     n_ao, P = s.n_ao, s.n_ao * (s.n_ao + 1) // 2
     W = w_ref(n_ao, s.n1, s.n2)
-    D = torch.from_numpy(density(n_ao)).cuda().reshape(-1)
+    Dh = torch.from_numpy(density(n_ao)).reshape(-1)
+    D = Dh.cuda()
 
     def dd(Imat):  # sum D_kl D_ab (kl|ab)
         return float(D @ (Imat.reshape(n_ao * n_ao, n_ao * n_ao) @ D))
@@ -297,9 +298,13 @@ def main():
         fp64 = {"build_s": ms / 1e3, "eff_tflops": W / (ms / 1e3) / 1e12, "stats": st[-1]}
         if args.precision != "fp64":
             cutc.release_workspace()  # the device workspace of one precision is not reused by the other
-            dd_ref = dd(I)
+            # the FP64 results go to host memory and their DD contraction is
+            # taken there, so that the split builds plan with the same device
+            # memory as without the FP64 build
             Iref = I.cpu()
+            dd_ref = float(Dh @ (Iref.reshape(n_ao * n_ao, n_ao * n_ao) @ Dh))
             Xref_host = None if streamed else X.cpu()
+            torch.cuda.empty_cache()
 
     if args.precision == "fp64":
         ms, last = fp64["build_s"] * 1e3, fp64["stats"]

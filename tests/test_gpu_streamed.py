@@ -48,8 +48,9 @@ def test_streamed_matches_stored_and_oracle(torch, precision):
     Io = oracle.int_2e_ao(s, Xo)
     e_st, e_or = relerr(Is, Id), relerr(Is, Io)
     print(f"  vs stored {e_st:.2e} vs oracle {e_or:.2e}", flush=True)
-    # vs stored: the chunks differ, so the split builds differ by their own error
-    assert e_st <= {"fp64": 1e-13, "int8": 1e-11, "bf16crt": 1e-11, "bf16": 1e-8}[precision]
+    # vs stored: the chunks differ, so two split builds differ by up to twice
+    # their own error
+    assert e_st <= {"fp64": 1e-13, "int8": 1e-10, "bf16crt": 1e-10, "bf16": 1e-8}[precision]
     assert e_or <= {"fp64": 1e-12, "int8": 1e-10, "bf16crt": 1e-10, "bf16": 1e-8}[precision]
     # symmetrised: (kl|ab) = (ab|kl) in the n_ao^2 x n_ao^2 view
     n2 = s.n_ao ** 2
