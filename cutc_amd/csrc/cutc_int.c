@@ -362,6 +362,25 @@ static void *ws_take(ws_t *w, size_t bytes)
     return p;
 }
 
+/* Internal interface for the other host files of the library (internal.h):
+ * the library lock, the device workspace arena and the memory budget. */
+void cutc_internal_lock(void) { pthread_mutex_lock(&g_lock); }
+void cutc_internal_unlock(void) { pthread_mutex_unlock(&g_lock); }
+size_t cutc_internal_budget(size_t mem_limit)
+{
+    cutc_options o;
+    cutc_default_options(&o);
+    o.mem_limit = mem_limit;
+    return device_budget(&o);
+}
+int cutc_internal_workspace(size_t need, char **base)
+{
+    ws_t w;
+    int rc = ws_begin(&w, need);
+    *base = (rc == CUTC_OK) ? w.base : NULL;
+    return rc;
+}
+
 static void release_arenas(void)
 {
     for (int d = 0; d < WS_MAX_DEV; d++) {

@@ -26,7 +26,7 @@
 
 enum { OPA_ROWK = 0, OPA_COLK_GATHER = 1 };
 enum { OPB_ROWK = 0, OPB_COLK = 1 };
-enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4, EPI_RAW = 5 };
+enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4, EPI_RAW = 5, EPI_DENSE = 6 };
 
 struct GemmArgs {
     const double *A;
@@ -42,6 +42,8 @@ struct GemmArgs {
     double alpha;
     int tiles_m, tiles_n;
     int64_t kslice; // split K (gridDim.y > 1): block y takes K rows [y kslice, (y + 1) kslice)
+    int64_t ldc;    // EPI_DENSE: out[col ldc + row] = alpha acc + beta out[...]
+    double beta;
 };
 
 // Fetch this thread's 8 operand values of one K step into registers.
@@ -187,6 +189,11 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
                 if constexpr (EPI == EPI_RAW) {
                     // unscaled partial sum of this K slice: out[y][col][row]
                     g.out[((int64_t)blockIdx.y * g.N + col) * g.M + row] = acc[i][j][r];
+                    continue;
+                }
+                if constexpr (EPI == EPI_DENSE) {
+                    double *o = g.out + col * g.ldc + row;
+                    *o = (g.beta == 0.0) ? g.alpha * acc[i][j][r] : fma(g.alpha, acc[i][j][r], g.beta * *o);
                     continue;
                 }
                 const double v = g.alpha * acc[i][j][r];
@@ -341,4 +348,46 @@ extern "C" int cutc_gemm2_split_count(int64_t K, int nsplit)
     const int64_t Kp = (K + BK - 1) / BK * BK;
     const int64_t kslice = ((Kp + nsplit - 1) / nsplit + BK - 1) / BK * BK;
     return (int)((Kp + kslice - 1) / kslice);
+}
+
+/* Dense C (M x N, column-major, ld ldc) = alpha A^T B + beta C for
+ * K-contiguous operands: A row i at A + i lda, B row j at B + j ldb, K a
+ * multiple of BK with both operands zero past the valid K (used by the
+ * normal-ordering contractions).  nsplit > 1: the K rows run in slices whose
+ * unscaled partial sums go to work[y][j][i] instead (alpha, beta and C are
+ * then unused); the caller adds the slices. */
+extern "C" int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t M, int64_t N,
+                                int64_t K, double alpha, double beta, double *C, int64_t ldc, int nsplit,
+                                double *work, hipStream_t st)
+{
+    if (M <= 0 || N <= 0)
+        return 0;
+    if (K <= 0 || K % BK || lda % 2 || ldb % 2 || lda < K || ldb < K || nsplit < 1 || (nsplit == 1 && ldc < M) ||
+        (nsplit > 1 && !work))
+        return CUTC_ERR_ARG;
+    GemmArgs g = {};
+    g.A = A;
+    g.lda = lda;
+    g.B = B;
+    g.ldb = ldb;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.Ka = K;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.ldc = ldc;
+    if (nsplit == 1) {
+        g.out = C;
+        return launch_gemm<OPA_ROWK, OPB_ROWK, EPI_DENSE>(g, st);
+    }
+    g.out = work;
+    g.kslice = ((K + nsplit - 1) / nsplit + BK - 1) / BK * BK;
+    const int ns = (int)((K + g.kslice - 1) / g.kslice);
+    return launch_gemm<OPA_ROWK, OPB_ROWK, EPI_RAW>(g, st, ns);
+}
+
+extern "C" int cutc_gemm_tn_split_count(int64_t K, int nsplit)
+{
+    return cutc_gemm2_split_count(K, nsplit);
 }

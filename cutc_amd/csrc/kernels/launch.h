@@ -198,6 +198,34 @@ int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int 
                        const int *rexpB, double alpha, const int *pairs, int n_ao, double *M, int mode,
                        hipStream_t st);
 
+/* Dense TN GEMM (gemm_f64.hip): C = alpha A^T B + beta C, K-contiguous rows,
+ * K a multiple of 16 with zero padding; nsplit > 1 writes the unscaled
+ * partial sums of cutc_gemm_tn_split_count(K, nsplit) K slices to
+ * work[y][j][i]. */
+int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t M, int64_t N, int64_t K,
+                     double alpha, double beta, double *C, int64_t ldc, int nsplit, double *work, hipStream_t st);
+int cutc_gemm_tn_split_count(int64_t K, int nsplit);
+
+/* Normal ordering (no.hip): one grid chunk [g0, g0 + ngc) of the inputs
+ * (w, mol, mor, G in the reference's layouts over ng points), intermediates
+ * in chunk-local layouts (see no.hip), GEMM operands padded to ldk points.
+ * ne = max(na, nb) weighted orbitals; nblk: blocks of cutc_no_point over all
+ * chunks (the partial-sum stride). */
+typedef struct cutc_no_args {
+    int64_t ng, g0, ngc, ldk, nblk;
+    int nm, na, nb, ne;
+    const double *w, *mol, *mor, *G;
+    double *J, *O, *S, *TE, *TF, *Dv, *A, *Rt, *E, *F, *Gt, *Ht, *C1x, *Gx, *C2, *D2;
+} cutc_no_args;
+int cutc_no_point(const cutc_no_args *a, double *part, hipStream_t st);
+int cutc_no_orb(const cutc_no_args *a, hipStream_t st);
+int cutc_no_pair(const cutc_no_args *a, hipStream_t st);
+int cutc_no_gemv(const double *Gx, const double *C2, const double *Dv, int nm, int64_t ldk, int64_t ngc,
+                 double *no_1e, hipStream_t st);
+int cutc_no_add_slices(const double *work, int ns, int64_t n, double *out, hipStream_t st);
+int cutc_no_sym_swap(const double *N, int nm, double *out, hipStream_t st);
+int cutc_no_finish_0e(const double *part, int64_t nblk, double *no_0e, hipStream_t st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,15 @@ class BatchStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class NoStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ms_total", "ms_point", "ms_pair", "ms_gemm2e", "ms_1e", "ms_symm",
+                                              "flops_gemm2e")] + [
+        ("chunk", ctypes.c_int64), ("n_chunks", ctypes.c_int64), ("workspace_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Plan(ctypes.Structure):
     _fields_ = [("n1_chunk", ctypes.c_int64), ("ldk", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
                 ("n1_chunk_2e", ctypes.c_int64), ("bytes_stage1", ctypes.c_size_t),
@@ -90,7 +99,7 @@ HEADER_SYMBOLS = [
     "cutc_int", "get_int2_grad1_u12_ao", "get_int_2e_ao", "checkCudaErrors", "checkCublasErrors", "cutc_int_bh",
     "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
     "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
-    "cutc_release_workspace", "cutc_int_streamed_device", "cutc_int_batch_device",
+    "cutc_release_workspace", "cutc_int_streamed_device", "cutc_int_batch_device", "cutc_no", "cutc_no_device",
 ]
 
 _vp = ctypes.c_void_p
@@ -137,6 +146,11 @@ def lib():
         L.cutc_default_options.argtypes = [ctypes.POINTER(Options)]
         L.cutc_release_workspace.restype = None
         L.cutc_release_workspace.argtypes = []
+        L.cutc_no.restype = _ci
+        L.cutc_no.argtypes = [_ci] * 4 + [_vp] * 7
+        L.cutc_no_device.restype = _ci
+        L.cutc_no_device.argtypes = [_ci] * 4 + [_vp] * 7 + [ctypes.c_size_t, ctypes.c_int64, _vp,
+                                                             ctypes.POINTER(NoStats)]
         L.cutc_make_plan.restype = _ci
         L.cutc_make_plan.argtypes = [ctypes.c_int64, ctypes.c_int64, _ci, _ci, _ci, _ci, ctypes.c_size_t,
                                      ctypes.c_int64, ctypes.POINTER(Plan)]
@@ -315,6 +329,39 @@ def cutc_int_batch_device(dss, Is, opt: Options | None = None, stream=None):
     rc = lib().cutc_int_batch_device(ctypes.byref(opt or options()), len(dss), arr,
                                      ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
     _check(rc, "cutc_int_batch_device")
+    return st.as_dict()
+
+
+def cutc_no(ne_a, ne_b, wr1, mos_l, mos_r, G):
+    """Normal ordering on host arrays (the reference's cutc_no).  Inputs in
+    numpy C order of the Fortran layouts: wr1 (n_grid1,), mos_l / mos_r
+    (n_mo, n_grid1), G (n_mo, n_mo, 3, n_grid1) = int2_grad1_u12(n_grid1, 3,
+    n_mo, n_mo).  Returns (no_2e [t, s, q, p], no_1e [q, p], no_0e)."""
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    nm, ng = G.shape[0], G.shape[3]
+    w = np.ascontiguousarray(wr1, dtype=np.float64)
+    ml = np.ascontiguousarray(mos_l, dtype=np.float64)
+    mr = np.ascontiguousarray(mos_r, dtype=np.float64)
+    o2 = np.zeros((nm,) * 4)
+    o1 = np.zeros((nm, nm))
+    o0 = np.zeros(1)
+    rc = lib().cutc_no(ng, nm, int(ne_a), int(ne_b), *(a.ctypes.data for a in (w, ml, mr, G, o2, o1, o0)))
+    _check(rc, "cutc_no")
+    return o2, o1, float(o0[0])
+
+
+def cutc_no_device(ne_a, ne_b, wr1, mos_l, mos_r, G, no_2e, no_1e, no_0e, mem_limit=0, grid_chunk=0, stream=None):
+    """cutc_no_device on torch device tensors (layouts as cutc_no); returns the
+    statistics dict."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    nm, ng = G.shape[0], G.shape[3]
+    st = NoStats()
+    rc = lib().cutc_no_device(ng, nm, int(ne_a), int(ne_b), *(_tp(t) for t in (wr1, mos_l, mos_r, G, no_2e, no_1e,
+                                                                              no_0e)),
+                              mem_limit, grid_chunk, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+    _check(rc, "cutc_no_device")
     return st.as_dict()
 
 

@@ -66,6 +66,8 @@ def lib():
                                                _dp, _dp, _dp, _ip, _ip, _ip, _dp, _i64, _i64]
         L.oracle_int_2e_ao.restype = ctypes.c_int
         L.oracle_int_2e_ao.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.oracle_cutc_no.restype = ctypes.c_int
+        L.oracle_cutc_no.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         ob = _find_openblas()
         if ob and os.environ.get("CUTC_ORACLE_BUILTIN_DGEMM", "0") != "1":
             L.oracle_use_blas(ob.encode(), b"scipy_dgemm_64_")
@@ -180,3 +182,23 @@ def cutc_int(sys, n1c=0, n2c=None):
     """(X[0:3], int_2e_ao) as cutc_int returns them."""
     X = int2_grad1_u12_ao(sys, n1c, n2c)
     return X[:3].copy(), int_2e_ao(sys, X)
+
+
+def cutc_no(ne_a, ne_b, wr1, mos_l, mos_r, G):
+    """Normal ordering (oracle/oracle_no.c).  Inputs in numpy C order of the
+    Fortran layouts: wr1 (n_grid1,), mos_l / mos_r (n_mo, n_grid1) = Fortran
+    (n_grid1, n_mo), G (n_mo, n_mo, 3, n_grid1) = Fortran
+    int2_grad1_u12(n_grid1, 3, n_mo, n_mo), G[j, i, x, g].  Returns (no_2e,
+    no_1e, no_0e): no_2e (n_mo,)*4 = Fortran (p, q, s, t) as [t, s, q, p],
+    no_1e (n_mo, n_mo) = Fortran (p, q) as [q, p], no_0e a float."""
+    G = _f64(G)
+    nm, ng = G.shape[0], G.shape[3]
+    assert G.shape == (nm, nm, 3, ng)
+    no_2e = np.zeros((nm,) * 4)
+    no_1e = np.zeros((nm, nm))
+    no_0e = np.zeros(1)
+    rc = lib().oracle_cutc_no(ng, nm, int(ne_a), int(ne_b), _p(_f64(wr1)), _p(_f64(mos_l)), _p(_f64(mos_r)), _p(G),
+                              _p(no_2e), _p(no_1e), _p(no_0e))
+    if rc != 0:
+        raise ValueError(f"oracle_cutc_no returned {rc}")
+    return no_2e, no_1e, float(no_0e[0])
