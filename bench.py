@@ -69,10 +69,11 @@ def parse():
     ap.add_argument("--slices", type=int, default=None,
                     help="int8: moduli (default 11); bf16 / fp16: digit slices per operand (default 4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
-    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40", "batch64"],
+    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40", "batch64", "no"],
                     help="cube20: 20 nuclei, n_ao=200, X stored (the metric's workload); cube40: 40 nuclei, "
                          "n_ao=300, n1=80000, n2=300000, streamed (X never stored); batch64: 64 independent "
-                         "systems, sequential FP64 calls vs one batched graph launch")
+                         "systems, sequential FP64 calls vs one batched graph launch; no: normal ordering "
+                         "(cutc_no), n_mo=200, n_grid1=60000, 30 doubly occupied orbitals")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="batch64: grid sizes times this factor")
     ap.add_argument("--variants", default="",
                     help="further split builds compared with the same FP64 build, e.g. bf16:3,bf16:4,fp16:4")
@@ -205,6 +206,77 @@ def batch_bench(args, torch, cutc, synth):
     print(json.dumps(out), flush=True)
 
 
+def no_bench(args, torch, cutc):
+    """Normal ordering (cutc_no_device) on synthetic MO-basis inputs resident in
+    HBM: int2_grad1_u12 (n_grid1, 3, n_mo, n_mo), orbitals on the grid, weights.
+    The work is dominated by the 2e contractions, W = 2 n_mo^4 (3 + 1) n_grid1."""
+    nm = 200
+    ng = args.n1 or 60000
+    na = nb = 30
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(20261206)
+    G = torch.randn((nm, nm, 3, ng), dtype=torch.float64, device="cuda", generator=gen)
+    w = torch.rand(ng, dtype=torch.float64, device="cuda", generator=gen)
+    L = torch.randn((nm, ng), dtype=torch.float64, device="cuda", generator=gen)
+    R = L + 0.1 * torch.randn((nm, ng), dtype=torch.float64, device="cuda", generator=gen)
+    o2 = torch.empty((nm,) * 4, dtype=torch.float64, device="cuda")
+    o1 = torch.empty((nm, nm), dtype=torch.float64, device="cuda")
+    o0 = torch.empty(1, dtype=torch.float64, device="cuda")
+    W = 2.0 * nm ** 4 * 4 * ng
+    stream = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        st = cutc.cutc_no_device(na, nb, w, L, R, G, o2, o1, o0, stream=stream)
+        log(args, f"warmup {i}: {st['ms_total'] / 1e3:.2f} s")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    sts = []
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        sts.append(cutc.cutc_no_device(na, nb, w, L, R, G, o2, o1, o0, stream=stream))
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    ms = ev[0].elapsed_time(ev[args.steps]) / args.steps
+    last = sts[-1]
+    log(args, f"cutc_no: {ms / 1e3:.2f} s {last}")
+    g2 = last["ms_gemm2e"] / 1e3
+    achieved = last["flops_gemm2e"] / g2 / 1e12
+    out = {"metric": "cutc_no build s + eff TFLOP/s (2e contraction work 2 n_mo^4 (3+1) n_grid1)",
+           "value": W / (ms / 1e3) / 1e12, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "ms_per_step_median": statistics.median(step_ms),
+           "step_ms": step_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"normal ordering: n_mo={nm}, n_grid1={ng}, ne_a=ne_b={nb} (closed shell)",
+                      "chunk": last["chunk"], "n_chunks": last["n_chunks"]},
+           "build_s": ms / 1e3, "w_flops": W,
+           "stage_ms": {k: last[k] for k in ("ms_point", "ms_pair", "ms_gemm2e", "ms_1e", "ms_symm")},
+           "roofline": {"kernel": "gemm_f64_kernel<0,0,6> (2e contractions, 4 launches per chunk)", "bound": "mfma",
+                        "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                        "frac": achieved / PEAK["fp64"], "traffic": None}}
+    del G, L, R, w, o2, o1, o0
+    cutc.release_workspace()
+    torch.cuda.empty_cache()
+    out["cpu_baseline"] = None
+    if args.cpu_sample > 0:
+        import numpy as np
+        from oracle import oracle
+        rng = np.random.default_rng(1)
+        n = args.cpu_sample
+        Gs = rng.standard_normal((nm, nm, 3, n))
+        ws = rng.random(n)
+        Ls = rng.standard_normal((nm, n))
+        Rs = Ls + 0.1 * rng.standard_normal((nm, n))
+        oracle.lib()
+        t0 = time.perf_counter()
+        oracle.cutc_no(na, nb, ws, Ls, Rs, Gs)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 2.0 * nm ** 4 * 4 * n / dt / 1e12, "unit": "TFLOP/s",
+                               "cores": oracle.threads(), "kind": "port",
+                               "sample": f"FP64 oracle ({oracle.blas_name()} DGEMM), n_mo={nm}, {n} of {ng} grid "
+                                         f"points: {dt:.2f} s; value = W(sample) / s", "seconds": dt}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -215,6 +287,10 @@ def main():
     if args.workload == "batch64":
         from cutc_amd import cutc, synth
         batch_bench(args, torch, cutc, synth)
+        return
+    if args.workload == "no":
+        from cutc_amd import cutc
+        no_bench(args, torch, cutc)
         return
 
     from cutc_amd import cutc, synth
@@ -402,9 +478,15 @@ def main():
         for v in filter(None, args.variants.split(",")):
             vp, vs = v.split(":")
             vopt = cutc.options(vp, int(vs))
-            for _ in range(args.variant_warmup):
-                build(cutc, ds, X, I, vopt, stream)
-            vms, vst = timed_builds(cutc, torch, ds, X, I, vopt, 1, stream)
+            try:
+                for _ in range(args.variant_warmup):
+                    build(cutc, ds, X, I, vopt, stream)
+                vms, vst = timed_builds(cutc, torch, ds, X, I, vopt, 1, stream)
+            except cutc.CutcError as exc:  # e.g. a working set that does not fit
+                cutc.release_workspace()
+                variants.append({"precision": f"{vp}x{vs}", "error": str(exc)})
+                log(args, f"variant {vp}x{vs}: {exc}")
+                continue
             cutc.release_workspace()
             verr = errors_vs_fp64()
             variants.append({"precision": f"{vp}x{vs}", "build_s": vms / 1e3, "eff_tflops": W / (vms / 1e3) / 1e12,

@@ -101,8 +101,9 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
         // this block's K slice of the (K-row) operands; slices start at
         // multiples of BK below K = roundup(Ka, BK), so Ka - start >= 1
         const int64_t ks = (int64_t)blockIdx.y * g.kslice;
-        g.A += ks * g.lda;
-        g.B += ks * g.ldb;
+        // K runs along a row of a ROWK operand and down the columns of a COLK one
+        g.A += (AM == OPA_ROWK) ? ks : ks * g.lda;
+        g.B += (BMODE == OPB_ROWK) ? ks : ks * g.ldb;
         g.K = (g.K - ks < g.kslice) ? g.K - ks : g.kslice;
         g.Ka = (g.Ka - ks < g.kslice) ? g.Ka - ks : g.kslice;
     }

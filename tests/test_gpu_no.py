@@ -92,3 +92,32 @@ def test_no_edge_cases(torch):
     assert not got[0].any() and not got[1].any() and got[2] == 0.0
     with pytest.raises(cutc.CutcError):
         device_run(torch, 6, 2, w, L, R, G)  # more electrons than orbitals
+
+
+@pytest.mark.parametrize("M,N,K,nsplit", [(12, 12, 2640, 2), (200, 130, 5 * 6016, 40), (300, 300, 4096, 1)])
+def test_gemm_tn_split(torch, M, N, K, nsplit):
+    """cutc_gemm_tn_f64 on K-contiguous rows: one slice (dense result) and K
+    slices (partial sums added here) against numpy."""
+    import ctypes
+    print(f"gemm_tn M={M} N={N} K={K} nsplit={nsplit}", flush=True)
+    L = cutc.lib()
+    L.cutc_gemm_tn_f64.restype = ctypes.c_int
+    L.cutc_gemm_tn_f64.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                   ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    L.cutc_gemm_tn_split_count.restype = ctypes.c_int
+    L.cutc_gemm_tn_split_count.argtypes = [ctypes.c_int64, ctypes.c_int]
+    rng = np.random.default_rng(M + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))
+    ns = L.cutc_gemm_tn_split_count(K, nsplit)
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    work = torch.zeros(ns * M * N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc = L.cutc_gemm_tn_f64(dA.data_ptr(), K, dB.data_ptr(), K, M, N, K, 1.0, 0.0, work.data_ptr(), M, nsplit,
+                            work.data_ptr(), None)
+    torch.cuda.synchronize()
+    assert rc == 0
+    got = work.cpu().numpy().reshape(ns, N, M).sum(0).T  # [i, j]
+    want = A @ B.T
+    assert rel(got, want) < 1e-14
