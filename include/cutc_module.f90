@@ -1,4 +1,4 @@
-! Fortran binding of libcutcint's cutc_int (bind(C), name "cutc_int").
+! Fortran binding of libcutcint's cutc_int and cutc_no (bind(C)).
 !
 ! Array shapes follow the reference binding src/cutc_module.f90:23-35.  The
 ! argument list follows the C definition src/cutc_int.c:15-20, which takes six
@@ -43,6 +43,25 @@ module cutc_module
       real(c_double), intent(out)       :: int_2e_ao(n_ao,n_ao,n_ao,n_ao)
 
     end function cutc_int
+
+    ! Normal ordering (src/cutc_no.c:14-16; the reference module has no
+    ! binding for it).  Returns the C status.
+    integer(c_int) function cutc_no(n_grid1, n_mo, ne_a, ne_b,              &
+                                    wr1, mos_l_in_r, mos_r_in_r,            &
+                                    int2_grad1_u12, no_2e, no_1e, no_0e)    &
+                                    bind(C, name = "cutc_no")
+
+      import c_int, c_double
+      integer(c_int), intent(in), value :: n_grid1, n_mo, ne_a, ne_b
+      real(c_double), intent(in)        :: wr1(n_grid1)
+      real(c_double), intent(in)        :: mos_l_in_r(n_grid1,n_mo)
+      real(c_double), intent(in)        :: mos_r_in_r(n_grid1,n_mo)
+      real(c_double), intent(in)        :: int2_grad1_u12(n_grid1,3,n_mo,n_mo)
+      real(c_double), intent(out)       :: no_2e(n_mo,n_mo,n_mo,n_mo)
+      real(c_double), intent(out)       :: no_1e(n_mo,n_mo)
+      real(c_double), intent(out)       :: no_0e
+
+    end function cutc_no
 
   end interface
 
