@@ -69,11 +69,13 @@ def parse():
     ap.add_argument("--slices", type=int, default=None,
                     help="int8: moduli (default 11); bf16 / fp16: digit slices per operand (default 4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
-    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40", "batch64", "no"],
+    ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40", "batch64", "no", "mo", "ao"],
                     help="cube20: 20 nuclei, n_ao=200, X stored (the metric's workload); cube40: 40 nuclei, "
                          "n_ao=300, n1=80000, n2=300000, streamed (X never stored); batch64: 64 independent "
                          "systems, sequential FP64 calls vs one batched graph launch; no: normal ordering "
-                         "(cutc_no), n_mo=200, n_grid1=60000, 30 doubly occupied orbitals")
+                         "(cutc_no), n_mo=200, n_grid1=60000, 30 doubly occupied orbitals; mo: AO -> MO "
+                         "transform of int2_grad1_u12 (cutc_ao_to_mo), n_ao=n_mo=200, n_grid1=60000; ao: AO values and "
+                         "gradients on both grids of the cube20 workload (cutc_aos_device), 3 primitives per AO")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="batch64: grid sizes times this factor")
     ap.add_argument("--variants", default="",
                     help="further split builds compared with the same FP64 build, e.g. bf16:3,bf16:4,fp16:4")
@@ -277,6 +279,140 @@ def no_bench(args, torch, cutc):
     print(json.dumps(out), flush=True)
 
 
+def mo_bench(args, torch, cutc):
+    """AO -> MO transform (cutc_ao_to_mo_device) of a resident
+    int2_grad1_u12_ao (n_ao, n_ao, n_grid1, 3) into int2_grad1_u12 (n_grid1,
+    3, n_mo, n_mo): two FP64 GEMMs, W = 6 n_grid1 n_ao n_mo (n_ao + n_mo)."""
+    na = nm = 200
+    n1 = args.n1 or 60000
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(20261207)
+    X = torch.randn((3, n1, na, na), dtype=torch.float64, device="cuda", generator=gen)
+    cl = torch.randn((nm, na), dtype=torch.float64, device="cuda", generator=gen) / na ** 0.5
+    cr = torch.randn((nm, na), dtype=torch.float64, device="cuda", generator=gen) / na ** 0.5
+    out = torch.empty((nm, nm, 3, n1), dtype=torch.float64, device="cuda")
+    W = 6.0 * n1 * na * nm * (na + nm)
+    stream = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        st = cutc.cutc_ao_to_mo_device(cl, cr, X, out, stream=stream)
+        log(args, f"warmup {i}: {st['ms_total']:.1f} ms")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    sts = []
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        sts.append(cutc.cutc_ao_to_mo_device(cl, cr, X, out, stream=stream))
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    ms = ev[0].elapsed_time(ev[args.steps]) / args.steps
+    last = sts[-1]
+    log(args, f"ao_to_mo: {ms:.1f} ms {last}")
+    gemm_ms = last["ms_left"] + last["ms_right"]
+    achieved = last["flops"] / (gemm_ms / 1e3) / 1e12
+    # HBM bytes the two GEMMs must move at least: A read, Z written and read, out written
+    hbm = 8.0 * (3 * n1) * (na * na + 2 * na * nm + nm * nm)
+    out_js = {"metric": "cutc_ao_to_mo ms + eff TFLOP/s (W = 6 n_grid1 n_ao n_mo (n_ao + n_mo))",
+              "value": W / (ms / 1e3) / 1e12, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": ms, "ms_per_step_median": statistics.median(step_ms),
+              "step_ms": step_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+              "data": "synthetic",
+              "config": {"workload": f"AO->MO transform: n_ao={na}, n_mo={nm}, n_grid1={n1}",
+                         "mo_chunk": last["mo_chunk"], "n_chunks": last["n_chunks"]},
+              "w_flops": W, "min_hbm_bytes": hbm, "hbm_GBps_at_min_bytes": hbm / (ms / 1e3) / 1e9,
+              "stage_ms": {"ms_left": last["ms_left"], "ms_right": last["ms_right"]},
+              "roofline": {"kernel": "gemm_f64_kernel<2,2,6> (2 launches)", "bound": "mfma", "achieved": achieved,
+                           "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"],
+                           "traffic": None}}
+    del X, out, cl, cr
+    cutc.release_workspace()
+    torch.cuda.empty_cache()
+    out_js["cpu_baseline"] = None
+    if args.cpu_sample > 0:
+        import numpy as np
+        from oracle import oracle
+        rng = np.random.default_rng(1)
+        n = 64 * args.cpu_sample
+        Xs = rng.standard_normal((3, n, na, na))
+        cls, crs = rng.standard_normal((nm, na)), rng.standard_normal((nm, na))
+        oracle.lib()
+        t0 = time.perf_counter()
+        oracle.ao_to_mo(cls, crs, Xs)
+        dt = time.perf_counter() - t0
+        out_js["cpu_baseline"] = {"value": 6.0 * n * na * nm * (na + nm) / dt / 1e12, "unit": "TFLOP/s",
+                                  "cores": oracle.threads(), "kind": "port",
+                                  "sample": f"FP64 oracle ({oracle.blas_name()} DGEMM per grid point), n_ao=n_mo="
+                                            f"{na}, {n} of {n1} grid points: {dt:.2f} s; value = W(sample) / s",
+                                  "seconds": dt}
+    print(json.dumps(out_js), flush=True)
+
+
+def ao_bench(args, torch, cutc, synth):
+    """AO values and gradients (cutc_aos_device) on both grids of the
+    20-nucleus workload (60000 + 200000 points, n_ao = 200), each AO of the
+    synthetic basis contracted over 3 primitives.  HBM-bound: 32 bytes
+    written per (point, AO)."""
+    import numpy as np
+    s = synth.cube20(n1=args.n1 or 60000, n2=200000)
+    b = dict(s.basis)
+    f = np.array([0.5, 1.0, 2.0])
+    b["ao_expo"] = np.ascontiguousarray(b["ao_expo"] * f)
+    b["ao_coef"] = np.ascontiguousarray(b["ao_coef"] * np.array([0.3, 0.5, 0.2]))
+    b["ao_prim_num"] = np.full(s.n_ao, 3, np.int32)
+    db = cutc.upload_basis(b)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    r = t(np.concatenate([s.r1, s.r2]))
+    rn = t(s.rn)
+    n, n_ao = r.shape[0], s.n_ao
+    out = torch.empty((4, n_ao, n), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        cutc.cutc_aos_device(r, rn, db, out, stream=stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        cutc.cutc_aos_device(r, rn, db, out, stream=stream)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    ms = ev[0].elapsed_time(ev[args.steps]) / args.steps
+    nbytes = 32.0 * n * n_ao
+    # what the device evaluation replaces: the upload of the same aos_data from pageable host memory
+    host = np.concatenate([s.aos1, s.aos2], axis=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev = torch.from_numpy(host).to("cuda")
+    torch.cuda.synchronize()
+    h2d = time.perf_counter() - t0
+    del dev, host
+    out_js = {"metric": "cutc_aos_device ms + GB/s of aos_data written", "value": nbytes / (ms / 1e3) / 1e9,
+              "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+              "ms_per_step_median": statistics.median(step_ms), "step_ms": step_ms, "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+              "config": {"workload": f"AOs of the cube20 workload: n_ao={n_ao} (3 primitives each), "
+                                     f"{s.n1} + {s.n2} grid points"},
+              "roofline": {"kernel": "aos_kernel", "bound": "hbm", "achieved": nbytes / (ms / 1e3) / 1e9,
+                           "peak": 8000.0, "unit": "GB/s", "frac": nbytes / (ms / 1e3) / 1e9 / 8000.0,
+                           "traffic": None, "bytes_per_launch": nbytes},
+              "h2d_pageable_ms_same_bytes": h2d * 1e3}
+    log(args, f"aos: {ms:.2f} ms, {out_js['value']:.0f} GB/s; upload of the same bytes {h2d * 1e3:.0f} ms")
+    del out, r
+    torch.cuda.empty_cache()
+    out_js["cpu_baseline"] = None
+    if args.cpu_sample > 0:
+        from oracle import oracle
+        oracle.lib()
+        t0 = time.perf_counter()
+        for rg in (s.r1, s.r2):
+            oracle.aos(rg, s.rn, **b)
+        dt = time.perf_counter() - t0
+        out_js["cpu_baseline"] = {"value": nbytes / dt / 1e9, "unit": "GB/s", "cores": oracle.threads(),
+                                  "kind": "port", "sample": f"FP64 oracle (C, OpenMP), the whole workload ({n} points, "
+                                                            f"n_ao={n_ao}): {dt:.2f} s", "seconds": dt}
+    print(json.dumps(out_js), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -291,6 +427,14 @@ def main():
     if args.workload == "no":
         from cutc_amd import cutc
         no_bench(args, torch, cutc)
+        return
+    if args.workload == "mo":
+        from cutc_amd import cutc
+        mo_bench(args, torch, cutc)
+        return
+    if args.workload == "ao":
+        from cutc_amd import cutc, synth
+        ao_bench(args, torch, cutc, synth)
         return
 
     from cutc_amd import cutc, synth
@@ -500,16 +644,21 @@ def main():
     torch.cuda.empty_cache()
     if args.host_path and not streamed:
         Xref_host = Iref = None
-        cutc.release_workspace()
-        t0 = time.perf_counter()
-        Xh, Ih, hst = cutc.cutc_int_ex(s, opt)
-        wall = time.perf_counter() - t0
-        cutc.release_workspace()
-        del Xh, Ih
-        out["host_path"] = {"wall_s": wall, "eff_tflops": W / wall / 1e12, "build_ms": hst["ms_total"],
-                            "note": "cutc_int_ex on host arrays: inputs uploaded, X[0..2] and int_2e_ao downloaded "
-                                    "to pageable host memory"}
-        log(args, f"host path: {wall:.2f} s")
+        hp = {}
+        for mode, hc in (("pipelined", 0), ("blocking", 1)):
+            cutc.release_workspace()
+            hopt = cutc.options(precision=opt.precision, slices=opt.slices, host_copy=hc)
+            t0 = time.perf_counter()
+            Xh, Ih, hst = cutc.cutc_int_ex(s, hopt)
+            wall = time.perf_counter() - t0
+            cutc.release_workspace()
+            del Xh, Ih
+            hp[mode] = {"wall_s": wall, "eff_tflops": W / wall / 1e12, "build_ms": hst["ms_total"]}
+            log(args, f"host path ({mode}): {wall:.2f} s, build {hst['ms_total'] / 1e3:.2f} s")
+        hp["note"] = ("cutc_int_ex on host arrays: inputs uploaded, X[0..2] and int_2e_ao downloaded to pageable "
+                      "(freshly allocated) host memory; pipelined: each r1 chunk of X is downloaded while the build "
+                      "runs; blocking: hipMemcpy after the build, as the reference")
+        out["host_path"] = hp
     out["cpu_baseline"] = cpu_baseline(s, args.cpu_sample) if args.cpu_sample > 0 else None
     print(json.dumps(out), flush=True)
 

@@ -22,11 +22,14 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <hip/hip_runtime_api.h>
 
 #include "cutc_int.h"
+#include "internal.h"
 #include "kernels/launch.h"
+#include "xfer.h"
 
 static double now_ms(void)
 {
@@ -34,8 +37,6 @@ static double now_ms(void)
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
 }
-
-void cutc_set_last_error(int code);
 
 #define NCAT 7
 enum { CAT_JAST = 0, CAT_INTEG, CAT_GEMM1, CAT_GEMM2, CAT_SYMM, CAT_SPLIT, CAT_TOTAL };
@@ -49,6 +50,11 @@ typedef struct {
     hipEvent_t *ev; /* pairs (start, stop) */
     int *cat;
     int launches[NCAT];
+    /* called after the work writing rows [i0, i0 + n1e) of every X
+     * component has been queued on st (host entry points: starts their
+     * download) */
+    void (*x_ready)(void *ctx, int64_t i0, int64_t n1e, hipStream_t st);
+    void *x_ctx;
 } tracer_t;
 
 static void tr_init(tracer_t *t, hipStream_t st, int timing)
@@ -467,6 +473,8 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         rc = cutc_gemm1_f64(Apk, ldk, G, ldk, P, n1e, ldk, pairs, n_ao, X, n1, i0, 0, st);
         tr_end(tr, ev);
         TRY(rc);
+        if (tr->x_ready)
+            tr->x_ready(tr->x_ctx, i0, n1e, st);
         if (stats) {
             /* executed contraction work, FP64-equivalent: 2 x pairs x (4 n1e) x n2 */
             stats->flops_exec += 2.0 * (double)P * 4.0 * (double)n1e * (double)n2;
@@ -579,6 +587,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
                    : cutc_crt_rebuild_x(&crt, Cpl, P, 4 * n1e, rexpA, rexpG, pairs, n_ao, X, n1, n1e, i0, st);
         tr_end(tr, ev);
         TRY(rc);
+        if (tr->x_ready)
+            tr->x_ready(tr->x_ctx, i0, n1e, st);
         if (stats)
             stats->flops_exec += 2.0 * (double)P * 4.0 * (double)n1e * (double)n2;
     }
@@ -892,7 +902,8 @@ static void fill_stats(cutc_stats *stats, tracer_t *tr)
 static int device_path(const cutc_options *opt, int stages, int n1, int n2, int n_ao, int n_nuc, int size_bh,
                        const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
                        const double *aos1, const double *aos2, const double *c_bh, const int *m_bh, const int *n_bh,
-                       const int *o_bh, double *X, double *out, hipStream_t st, cutc_stats *stats)
+                       const int *o_bh, double *X, double *out, hipStream_t st, cutc_stats *stats,
+                       const tracer_t *hook)
 {
     int rc = check_sizes(n1, n2, n_ao, n_nuc, size_bh);
     if (rc)
@@ -927,6 +938,10 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         memset(stats, 0, sizeof *stats);
     tracer_t tr;
     tr_init(&tr, st, stats != NULL || opt->verbose);
+    if (hook) {
+        tr.x_ready = hook->x_ready;
+        tr.x_ctx = hook->x_ctx;
+    }
     int evt = tr_begin(&tr, CAT_TOTAL);
     if ((stages & 1) && n1 > 0 && n_ao > 0) {
         if (n2 == 0) {
@@ -989,7 +1004,7 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
     }
     pthread_mutex_lock(&g_lock);
     int rc = device_path(opt, 3, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
-                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats);
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats, NULL);
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     return rc;
@@ -1345,7 +1360,7 @@ void get_int2_grad1_u12_ao(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int n_grid1, i
     cutc_default_options(&opt);
     pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 1, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, NULL, r2, wr2, rn, NULL, aos_data2,
-                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL);
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL, NULL);
     release_arenas(); /* the reference keeps no device memory between calls */
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
@@ -1360,7 +1375,7 @@ void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double
     cutc_default_options(&opt);
     pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 2, n_grid1, 0, n_ao, 0, 0, NULL, wr1, NULL, NULL, NULL, aos_data1, NULL, NULL, NULL,
-                         NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL);
+                         NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL, NULL);
     release_arenas();
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
@@ -1402,6 +1417,35 @@ static int upload(void **d, const void *h, size_t bytes)
     return hip_rc(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
 }
 
+/* Pipelined download of X by the host entry points: the rows of each r1
+ * chunk (3 components; the 4th stays on the device, as in the reference)
+ * are queued on the transfer pipeline as soon as the build has queued the
+ * work that writes them. */
+typedef struct {
+    cutc_xfer *xf;
+    double *h_X;
+    const double *X;
+    int64_t n1, nao2, queued;
+    int rc;
+} xdownload_t;
+
+static void x_chunk_ready(void *ctx, int64_t i0, int64_t n1e, hipStream_t st)
+{
+    xdownload_t *d = (xdownload_t *)ctx;
+    for (int m = 0; m < 3 && d->rc == CUTC_OK; m++) {
+        const int64_t off = ((int64_t)m * d->n1 + i0) * d->nao2;
+        d->rc = cutc_xfer_d2h(d->xf, d->h_X + off, d->X + off, sizeof(double) * (size_t)(n1e * d->nao2), st);
+    }
+    d->queued = i0 + n1e;
+}
+
+static int xfer_threads(void)
+{
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    n = (n > 1) ? n - 1 : 0;
+    return (int)(n > 7 ? 7 : n);
+}
+
 static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nuc, int size_bh, const double *h_r1,
                      const double *h_wr1, const double *h_r2, const double *h_wr2, const double *h_rn,
                      const double *h_aos1, const double *h_aos2, const double *h_c, const int *h_m, const int *h_n,
@@ -1422,6 +1466,9 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
     int *m = NULL, *n = NULL, *o = NULL;
     double *X = NULL, *out = NULL;
     hipEvent_t e0 = NULL, e1 = NULL, e2 = NULL, e3 = NULL;
+    xdownload_t dl = {0};
+    tracer_t hook;
+    memset(&hook, 0, sizeof hook);
     TRYH(hipEventCreate(&e0));
     TRYH(hipEventCreate(&e1));
     TRYH(hipEventCreate(&e2));
@@ -1442,14 +1489,34 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
         TRYH(hipMalloc((void **)&X, sizeof(double) * 4 * nao2 * n1));
     if (nao2 > 0)
         TRYH(hipMalloc((void **)&out, sizeof(double) * nao2 * nao2));
+    if (!opt->host_copy && (X || out)) {
+        TRY(cutc_xfer_open(&dl.xf, (size_t)64 << 20, 4, xfer_threads()));
+        dl.h_X = h_X;
+        dl.X = X;
+        dl.n1 = n1;
+        dl.nao2 = nao2;
+        if (X) {
+            hook.x_ready = x_chunk_ready;
+            hook.x_ctx = &dl;
+        }
+    }
     TRYH(hipEventRecord(e1, NULL));
     TRY(device_path(opt, 3, n1, n2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos1, aos2, c, m, n, o, X, out, NULL,
-                    stats));
+                    stats, &hook));
     TRYH(hipEventRecord(e2, NULL));
-    if (X)
-        TRYH(hipMemcpy(h_X, X, sizeof(double) * 3 * nao2 * n1, hipMemcpyDeviceToHost));
-    if (out)
-        TRYH(hipMemcpy(h_out, out, sizeof(double) * nao2 * nao2, hipMemcpyDeviceToHost));
+    if (dl.xf) {
+        if (X && dl.queued < n1) /* rows written without r1 chunks (empty grid 2) */
+            x_chunk_ready(&dl, dl.queued, n1 - dl.queued, NULL);
+        TRY(dl.rc);
+        if (out)
+            TRY(cutc_xfer_d2h(dl.xf, h_out, out, sizeof(double) * (size_t)(nao2 * nao2), NULL));
+        TRY(cutc_xfer_finish(dl.xf, NULL));
+    } else {
+        if (X)
+            TRYH(hipMemcpy(h_X, X, sizeof(double) * 3 * nao2 * n1, hipMemcpyDeviceToHost));
+        if (out)
+            TRYH(hipMemcpy(h_out, out, sizeof(double) * nao2 * nao2, hipMemcpyDeviceToHost));
+    }
     TRYH(hipEventRecord(e3, NULL));
     TRYH(hipEventSynchronize(e3));
     if (opt->verbose) {
@@ -1462,6 +1529,7 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
         fflush(stdout);
     }
 done:
+    cutc_xfer_close(dl.xf); /* drains queued copies before the buffers go */
     dfree(r1);
     dfree(wr1);
     dfree(r2);

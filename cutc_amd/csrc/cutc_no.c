@@ -19,13 +19,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include "cutc_no.h"
+#include "internal.h"
 #include "kernels/launch.h"
-
-void cutc_set_last_error(int code);
-void cutc_internal_lock(void);
-void cutc_internal_unlock(void);
-size_t cutc_internal_budget(size_t mem_limit);
-int cutc_internal_workspace(size_t need, char **base);
 
 static int hrc(hipError_t e)
 {

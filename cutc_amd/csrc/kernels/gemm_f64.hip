@@ -24,8 +24,10 @@
 #define BK 16
 #define LDSS 18
 
-enum { OPA_ROWK = 0, OPA_COLK_GATHER = 1 };
-enum { OPB_ROWK = 0, OPB_COLK = 1 };
+// ROWK_ANY: K-contiguous rows with any leading dimension and K (8-byte loads,
+// k >= Ka read as zero)
+enum { OPA_ROWK = 0, OPA_COLK_GATHER = 1, OPA_ROWK_ANY = 2 };
+enum { OPB_ROWK = 0, OPB_COLK = 1, OPB_ROWK_ANY = 2 };
 enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4, EPI_RAW = 5, EPI_DENSE = 6 };
 
 struct GemmArgs {
@@ -48,6 +50,7 @@ struct GemmArgs {
 
 // Fetch this thread's 8 operand values of one K step into registers.
 // ROWK:   thread t -> row t/2, k = 8 (t%2) .. +7   (K-contiguous rows)
+// ROWK_ANY: the same elements one by one, k >= klim read as zero
 // COLK:   thread t -> row t%128, k = 8 (t/128) .. +7 (rows of K at stride ld),
 //         rows >= klim read row klim - 1
 template <int MODE>
@@ -66,6 +69,15 @@ __device__ static inline void fetch(const double *__restrict__ P, int64_t ld, in
             r[2 * i] = v[0];
             r[2 * i + 1] = v[1];
         }
+    } else if constexpr (MODE == 2) {
+        int64_t row = row0 + (t >> 1);
+        if (row >= nrows)
+            row = nrows - 1;
+        const int64_t kb = k0 + 8 * (t & 1);
+        const double *src = P + row * ld;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            r[i] = (kb + i < klim) ? src[kb + i] : 0.0;
     } else {
         const int64_t kb = k0 + 8 * (t >> 7);
         const double *base = P + gidx;
@@ -81,7 +93,7 @@ template <int MODE>
 __device__ static inline void stash(double *__restrict__ S, const double (&r)[8])
 {
     const int t = threadIdx.x;
-    double *d = (MODE == 0) ? S + (t >> 1) * LDSS + 8 * (t & 1) : S + (t & 127) * LDSS + 8 * (t >> 7);
+    double *d = (MODE != 1) ? S + (t >> 1) * LDSS + 8 * (t & 1) : S + (t & 127) * LDSS + 8 * (t >> 7);
 #pragma unroll
     for (int i = 0; i < 4; i++)
         *(v2d *)(d + 2 * i) = v2d{r[2 * i], r[2 * i + 1]};
@@ -102,8 +114,8 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
         // multiples of BK below K = roundup(Ka, BK), so Ka - start >= 1
         const int64_t ks = (int64_t)blockIdx.y * g.kslice;
         // K runs along a row of a ROWK operand and down the columns of a COLK one
-        g.A += (AM == OPA_ROWK) ? ks : ks * g.lda;
-        g.B += (BMODE == OPB_ROWK) ? ks : ks * g.ldb;
+        g.A += (AM != OPA_COLK_GATHER) ? ks : ks * g.lda;
+        g.B += (BMODE != OPB_COLK) ? ks : ks * g.ldb;
         g.K = (g.K - ks < g.kslice) ? g.K - ks : g.kslice;
         g.Ka = (g.Ka - ks < g.kslice) ? g.Ka - ks : g.kslice;
     }
@@ -136,7 +148,8 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
     const int64_t nk = (g.K + BK - 1) / BK;
     double ra[8], rb[8];
     fetch<AM>(g.A, g.lda, g.M, g.Ka, row0, 0, aidx, ra);
-    fetch<BMODE>(g.B, g.ldb, g.N, g.K, col0, 0, bidx, rb);
+    const int64_t kb_lim = (BMODE == OPB_ROWK_ANY) ? g.Ka : g.K;
+    fetch<BMODE>(g.B, g.ldb, g.N, kb_lim, col0, 0, bidx, rb);
     stash<AM>(As[0], ra);
     stash<BMODE>(Bs[0], rb);
     __syncthreads();
@@ -146,7 +159,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
         const bool more = kt + 1 < nk;
         if (more) {
             fetch<AM>(g.A, g.lda, g.M, g.Ka, row0, (kt + 1) * BK, aidx, ra);
-            fetch<BMODE>(g.B, g.ldb, g.N, g.K, col0, (kt + 1) * BK, bidx, rb);
+            fetch<BMODE>(g.B, g.ldb, g.N, kb_lim, col0, (kt + 1) * BK, bidx, rb);
         }
         v4d a[4], b[4];
 #pragma unroll
@@ -352,20 +365,23 @@ extern "C" int cutc_gemm2_split_count(int64_t K, int nsplit)
 }
 
 /* Dense C (M x N, column-major, ld ldc) = alpha A^T B + beta C for
- * K-contiguous operands: A row i at A + i lda, B row j at B + j ldb, K a
- * multiple of BK with both operands zero past the valid K (used by the
- * normal-ordering contractions).  nsplit > 1: the K rows run in slices whose
- * unscaled partial sums go to work[y][j][i] instead (alpha, beta and C are
- * then unused); the caller adds the slices. */
+ * K-contiguous operands: A row i at A + i lda, B row j at B + j ldb.  K a
+ * multiple of BK, even leading dimensions and 16-byte aligned operands take
+ * 16-byte loads (both operands must then read as zero past the valid K, as
+ * the normal-ordering operands do); any other K or leading dimension takes
+ * the 8-byte-load form that stops at K.  nsplit > 1: the K rows run in slices
+ * whose unscaled partial sums go to work[y][j][i] instead (alpha, beta and C
+ * are then unused); the caller adds the slices. */
 extern "C" int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t M, int64_t N,
                                 int64_t K, double alpha, double beta, double *C, int64_t ldc, int nsplit,
                                 double *work, hipStream_t st)
 {
     if (M <= 0 || N <= 0)
         return 0;
-    if (K <= 0 || K % BK || lda % 2 || ldb % 2 || lda < K || ldb < K || nsplit < 1 || (nsplit == 1 && ldc < M) ||
-        (nsplit > 1 && !work))
+    if (K <= 0 || lda < K || ldb < K || nsplit < 1 || (nsplit == 1 && ldc < M) || (nsplit > 1 && !work))
         return CUTC_ERR_ARG;
+    const bool fast = K % BK == 0 && lda % 2 == 0 && ldb % 2 == 0 && ((uintptr_t)A % 16) == 0 &&
+                      ((uintptr_t)B % 16) == 0;
     GemmArgs g = {};
     g.A = A;
     g.lda = lda;
@@ -373,19 +389,21 @@ extern "C" int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, i
     g.ldb = ldb;
     g.M = M;
     g.N = N;
-    g.K = K;
+    g.K = (K + BK - 1) / BK * BK;
     g.Ka = K;
     g.alpha = alpha;
     g.beta = beta;
     g.ldc = ldc;
     if (nsplit == 1) {
         g.out = C;
-        return launch_gemm<OPA_ROWK, OPB_ROWK, EPI_DENSE>(g, st);
+        return fast ? launch_gemm<OPA_ROWK, OPB_ROWK, EPI_DENSE>(g, st)
+                    : launch_gemm<OPA_ROWK_ANY, OPB_ROWK_ANY, EPI_DENSE>(g, st);
     }
     g.out = work;
-    g.kslice = ((K + nsplit - 1) / nsplit + BK - 1) / BK * BK;
-    const int ns = (int)((K + g.kslice - 1) / g.kslice);
-    return launch_gemm<OPA_ROWK, OPB_ROWK, EPI_RAW>(g, st, ns);
+    g.kslice = ((g.K + nsplit - 1) / nsplit + BK - 1) / BK * BK;
+    const int ns = (int)((g.K + g.kslice - 1) / g.kslice);
+    return fast ? launch_gemm<OPA_ROWK, OPB_ROWK, EPI_RAW>(g, st, ns)
+                : launch_gemm<OPA_ROWK_ANY, OPB_ROWK_ANY, EPI_RAW>(g, st, ns);
 }
 
 extern "C" int cutc_gemm_tn_split_count(int64_t K, int nsplit)

@@ -57,6 +57,13 @@ int cutc_launch_herm_ref(int64_t n1, int n_ao, const double *wr1, const double *
 int cutc_launch_nonherm_ref(int64_t n1, int n_ao, const double *wr1, const double *aos1, double *out,
                             hipStream_t st);
 
+/* Contracted Cartesian Gaussian AOs and gradients (ao.hip): out[g + n (i +
+ * n_ao c)], c = 0 value, 1..3 d/dx, d/dy, d/dz; nucl 1-based, power (n_ao,
+ * 3), expo / coef (pmax, n_ao).  Tables must be valid (checked by the
+ * caller); n_ao <= 65535. */
+int cutc_launch_aos(int64_t n, const double *r, const double *rn, int n_ao, const int *nucl, const int *power,
+                    const int *nprim, int pmax, const double *expo, const double *coef, double *out, hipStream_t st);
+
 /* dst[0..n) = value. */
 int cutc_launch_fill(double *dst, int64_t n, double value, hipStream_t st);
 
@@ -198,10 +205,11 @@ int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int 
                        const int *rexpB, double alpha, const int *pairs, int n_ao, double *M, int mode,
                        hipStream_t st);
 
-/* Dense TN GEMM (gemm_f64.hip): C = alpha A^T B + beta C, K-contiguous rows,
- * K a multiple of 16 with zero padding; nsplit > 1 writes the unscaled
- * partial sums of cutc_gemm_tn_split_count(K, nsplit) K slices to
- * work[y][j][i]. */
+/* Dense TN GEMM (gemm_f64.hip): C = alpha A^T B + beta C, K-contiguous rows
+ * (lda, ldb >= K).  K a multiple of 16 with even leading dimensions and
+ * 16-byte aligned operands reads whole 16-wide K steps (operands zero past K);
+ * anything else reads exactly K.  nsplit > 1 writes the unscaled partial sums
+ * of cutc_gemm_tn_split_count(K, nsplit) K slices to work[y][j][i]. */
 int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t M, int64_t N, int64_t K,
                      double alpha, double beta, double *C, int64_t ldc, int nsplit, double *work, hipStream_t st);
 int cutc_gemm_tn_split_count(int64_t K, int nsplit);

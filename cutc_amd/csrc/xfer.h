@@ -1,0 +1,23 @@
+/* Device -> host transfer pipeline (xfer.c): pinned staging ring, a
+ * non-blocking copy stream and host copy threads, fed with blocks of device
+ * memory as the build produces them. */
+#ifndef CUTC_XFER_H
+#define CUTC_XFER_H
+
+#include <stddef.h>
+
+#include <hip/hip_runtime_api.h>
+
+typedef struct cutc_xfer cutc_xfer;
+
+/* nstage pinned buffers of stage_bytes (2..8), nthreads host copy workers
+ * besides the driver thread (0..16) */
+int cutc_xfer_open(cutc_xfer **x, size_t stage_bytes, int nstage, int nthreads);
+/* copy bytes from d_src to h_dst once the work queued on producer so far
+ * has run (an event is recorded on producer now) */
+int cutc_xfer_d2h(cutc_xfer *x, void *h_dst, const void *d_src, size_t bytes, hipStream_t producer);
+/* wait until every queued block is in host memory; returns the first error */
+int cutc_xfer_finish(cutc_xfer *x, size_t *bytes_done);
+void cutc_xfer_close(cutc_xfer *x);
+
+#endif

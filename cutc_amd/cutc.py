@@ -49,7 +49,7 @@ class Dim3(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("slices", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("n1_chunk", ctypes.c_int), ("mem_limit", ctypes.c_size_t)]
+                ("n1_chunk", ctypes.c_int), ("mem_limit", ctypes.c_size_t), ("host_copy", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -88,6 +88,14 @@ class NoStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class MoStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ms_total", "ms_left", "ms_right", "flops")] + [
+        ("mo_chunk", ctypes.c_int64), ("n_chunks", ctypes.c_int64), ("workspace_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Plan(ctypes.Structure):
     _fields_ = [("n1_chunk", ctypes.c_int64), ("ldk", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
                 ("n1_chunk_2e", ctypes.c_int64), ("bytes_stage1", ctypes.c_size_t),
@@ -100,6 +108,7 @@ HEADER_SYMBOLS = [
     "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
     "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
     "cutc_release_workspace", "cutc_int_streamed_device", "cutc_int_batch_device", "cutc_no", "cutc_no_device",
+    "cutc_ao_to_mo", "cutc_ao_to_mo_device", "cutc_aos", "cutc_aos_device",
 ]
 
 _vp = ctypes.c_void_p
@@ -151,6 +160,15 @@ def lib():
         L.cutc_no_device.restype = _ci
         L.cutc_no_device.argtypes = [_ci] * 4 + [_vp] * 7 + [ctypes.c_size_t, ctypes.c_int64, _vp,
                                                              ctypes.POINTER(NoStats)]
+        L.cutc_ao_to_mo.restype = _ci
+        L.cutc_ao_to_mo.argtypes = [_ci] * 3 + [_vp] * 4
+        L.cutc_ao_to_mo_device.restype = _ci
+        L.cutc_ao_to_mo_device.argtypes = [_ci] * 3 + [_vp] * 4 + [ctypes.c_size_t, ctypes.c_int64, _vp,
+                                                                   ctypes.POINTER(MoStats)]
+        L.cutc_aos.restype = _ci
+        L.cutc_aos.argtypes = [_ci] * 4 + [_vp] * 8
+        L.cutc_aos_device.restype = _ci
+        L.cutc_aos_device.argtypes = [_ci] * 4 + [_vp] * 9
         L.cutc_make_plan.restype = _ci
         L.cutc_make_plan.argtypes = [ctypes.c_int64, ctypes.c_int64, _ci, _ci, _ci, _ci, ctypes.c_size_t,
                                      ctypes.c_int64, ctypes.POINTER(Plan)]
@@ -170,10 +188,11 @@ def _check(rc, where):
         raise CutcError(rc, where)
 
 
-def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0) -> Options:
+def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0, host_copy=0) -> Options:
     """slices: bf16/fp16 digit slices per operand, or the number of moduli
     for int8 / bf16crt / fp16crt (None: the precision's default,
-    DEFAULT_SLICES)."""
+    DEFAULT_SLICES).  host_copy (host entry points): 0 = pipelined download
+    during the build, 1 = blocking copies after it."""
     o = Options()
     lib().cutc_default_options(ctypes.byref(o))
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -181,6 +200,7 @@ def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=
     o.verbose = int(verbose)
     o.n1_chunk = n1_chunk
     o.mem_limit = mem_limit
+    o.host_copy = int(host_copy)
     return o
 
 
@@ -269,6 +289,20 @@ class DeviceSystem:
         import torch
         h = _host_inputs(sys)
         d = {k: torch.from_numpy(v).to(device) for k, v in h.items()}
+        return cls(d["r1"], d["wr1"], d["r2"], d["wr2"], d["rn"], d["aos1"], d["aos2"], d["c"], d["m"], d["n"],
+                   d["o"], sys.n1, sys.n2, sys.n_ao, sys.n_nuc, sys.size_bh)
+
+    @classmethod
+    def upload_with_device_aos(cls, sys, device="cuda"):
+        """Upload everything but the AO values, which cutc_aos_device
+        evaluates in HBM from ``sys.basis`` (no aos_data transfer)."""
+        import torch
+        h = _host_inputs(sys)
+        d = {k: torch.from_numpy(v).to(device) for k, v in h.items() if k not in ("aos1", "aos2")}
+        db = upload_basis(sys.basis, device)
+        for k, r in (("aos1", d["r1"]), ("aos2", d["r2"])):
+            d[k] = torch.empty((4, sys.n_ao, r.shape[0]), dtype=torch.float64, device=device)
+            cutc_aos_device(r, d["rn"], db, d[k])
         return cls(d["r1"], d["wr1"], d["r2"], d["wr2"], d["rn"], d["aos1"], d["aos2"], d["c"], d["m"], d["n"],
                    d["o"], sys.n1, sys.n2, sys.n_ao, sys.n_nuc, sys.size_bh)
 
@@ -363,6 +397,94 @@ def cutc_no_device(ne_a, ne_b, wr1, mos_l, mos_r, G, no_2e, no_1e, no_0e, mem_li
                               mem_limit, grid_chunk, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
     _check(rc, "cutc_no_device")
     return st.as_dict()
+
+
+def cutc_ao_to_mo(mo_l_coef, mo_r_coef, X):
+    """AO -> MO transform on host arrays.  numpy C order of the Fortran
+    layouts: mo_l_coef / mo_r_coef (n_mo, n_ao), X (3, n_grid1, n_ao, n_ao) =
+    int2_grad1_u12_ao(n_ao, n_ao, n_grid1, 3).  Returns int2_grad1_u12 as
+    (n_mo, n_mo, 3, n_grid1), the array cutc_no takes."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    cl = np.ascontiguousarray(mo_l_coef, dtype=np.float64)
+    cr = np.ascontiguousarray(mo_r_coef, dtype=np.float64)
+    n1, na = X.shape[1], X.shape[2]
+    nm = cl.shape[0]
+    if X.shape != (3, n1, na, na) or cl.shape != (nm, na) or cr.shape != (nm, na):
+        raise ValueError("cutc_ao_to_mo: shapes do not match")
+    out = np.zeros((nm, nm, 3, n1))
+    rc = lib().cutc_ao_to_mo(n1, na, nm, cl.ctypes.data, cr.ctypes.data, X.ctypes.data, out.ctypes.data)
+    _check(rc, "cutc_ao_to_mo")
+    return out
+
+
+def cutc_ao_to_mo_device(mo_l_coef, mo_r_coef, X, out, mem_limit=0, mo_chunk=0, stream=None):
+    """cutc_ao_to_mo_device on torch device tensors (shapes as cutc_ao_to_mo,
+    X may hold more than 3 components: the first 3 are read); returns the
+    statistics dict."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    n1, na = X.shape[1], X.shape[2]
+    nm = mo_l_coef.shape[0]
+    if (tuple(X.shape[1:]) != (n1, na, na) or X.shape[0] < 3 or tuple(mo_l_coef.shape) != (nm, na)
+            or tuple(mo_r_coef.shape) != (nm, na) or tuple(out.shape) != (nm, nm, 3, n1)):
+        raise ValueError("cutc_ao_to_mo_device: shapes do not match")
+    st = MoStats()
+    rc = lib().cutc_ao_to_mo_device(n1, na, nm, *(_tp(t) for t in (mo_l_coef, mo_r_coef, X, out)), mem_limit,
+                                    mo_chunk, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
+    _check(rc, "cutc_ao_to_mo_device")
+    return st.as_dict()
+
+
+def _basis_arrays(basis):
+    nucl = np.ascontiguousarray(basis["ao_nucl"], dtype=np.int32)
+    n_ao = nucl.shape[0]
+    power = np.ascontiguousarray(basis["ao_power"], dtype=np.int32).reshape(3, n_ao)
+    nprim = np.ascontiguousarray(basis["ao_prim_num"], dtype=np.int32)
+    expo = np.ascontiguousarray(basis["ao_expo"], dtype=np.float64).reshape(n_ao, -1)
+    coef = np.ascontiguousarray(basis["ao_coef"], dtype=np.float64).reshape(n_ao, -1)
+    return nucl, power, nprim, expo, coef
+
+
+def cutc_aos(r, rn, basis):
+    """AO values and gradients on host arrays: r (n, 3), rn (n_nuc, 3), basis
+    a dict of the cutc_aos tables in numpy C order of the Fortran layouts
+    (ao_nucl (n_ao,) 1-based, ao_power (3, n_ao), ao_prim_num (n_ao,),
+    ao_expo / ao_coef (n_ao, prim_max)).  Returns aos_data as (4, n_ao, n)."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    rn = np.ascontiguousarray(rn, dtype=np.float64)
+    nucl, power, nprim, expo, coef = _basis_arrays(basis)
+    n, n_ao, pmax = r.shape[0], nucl.shape[0], expo.shape[1]
+    out = np.zeros((4, n_ao, n))
+    rc = lib().cutc_aos(n, rn.shape[0], n_ao, pmax, *(a.ctypes.data for a in (r, rn, nucl, power, nprim, expo, coef,
+                                                                              out)))
+    _check(rc, "cutc_aos")
+    return out
+
+
+def upload_basis(basis, device="cuda"):
+    """The cutc_aos tables as torch device tensors (dict, same keys)."""
+    import torch
+    nucl, power, nprim, expo, coef = _basis_arrays(basis)
+    return {k: torch.from_numpy(v).to(device) for k, v in
+            (("ao_nucl", nucl), ("ao_power", power), ("ao_prim_num", nprim), ("ao_expo", expo), ("ao_coef", coef))}
+
+
+def cutc_aos_device(r, rn, dbasis, out, stream=None):
+    """cutc_aos_device on torch device tensors: r (n, 3), rn (n_nuc, 3), the
+    tables of upload_basis, out (4, n_ao, n) (or a view with that layout,
+    e.g. DeviceSystem.aos1)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    n, n_ao = r.shape[0], dbasis["ao_nucl"].shape[0]
+    pmax = dbasis["ao_expo"].shape[1]
+    if tuple(out.shape) != (4, n_ao, n):
+        raise ValueError("cutc_aos_device: out must be (4, n_ao, n)")
+    rc = lib().cutc_aos_device(n, rn.shape[0], n_ao, pmax, _tp(r), _tp(rn),
+                               *(_tp(dbasis[k]) for k in ("ao_nucl", "ao_power", "ao_prim_num", "ao_expo", "ao_coef")),
+                               _tp(out), ctypes.c_void_p(stream.cuda_stream))
+    _check(rc, "cutc_aos_device")
 
 
 def release_workspace():

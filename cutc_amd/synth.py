@@ -78,6 +78,10 @@ class System:
     o_bh: np.ndarray
     seed: int = 0
     name: str = ""
+    # the AO basis behind aos1 / aos2 in the arrays cutc_aos takes (numpy C
+    # order of the Fortran layouts): ao_nucl (n_ao,) 1-based, ao_power
+    # (3, n_ao), ao_prim_num (n_ao,), ao_expo / ao_coef (n_ao, prim_max)
+    basis: dict | None = None
 
     @property
     def n_nuc(self):
@@ -168,7 +172,10 @@ def make_system(n_nuc: int, n_ao: int, n1: int, n2: int, seed: int, nuclei=None,
     n = np.zeros((n_nuc, 6), np.int32)
     o = np.zeros((n_nuc, 6), np.int32)
     lib.synth_jastrow(seed, n_nuc, _ptr(c), _ptr(m, _ip), _ptr(n, _ip), _ptr(o, _ip))
-    return System(rn, r1, wr1, r2, wr2, aos1, aos2, c, m, n, o, seed=seed, name=name)
+    basis = {"ao_nucl": ao_nuc + 1, "ao_power": np.ascontiguousarray(ao_pow.reshape(n_ao, 3).T),
+             "ao_prim_num": np.ones(n_ao, np.int32), "ao_expo": ao_alpha.reshape(n_ao, 1).copy(),
+             "ao_coef": ao_norm.reshape(n_ao, 1).copy()}
+    return System(rn, r1, wr1, r2, wr2, aos1, aos2, c, m, n, o, seed=seed, name=name, basis=basis)
 
 
 def triatomic(n1=4000, n2=16000, seed=BASE_SEED + 0):

@@ -120,6 +120,10 @@ typedef struct cutc_options {
     int verbose;      /* 1: print the reference's stage-time lines               */
     int n1_chunk;     /* r1 rows per G chunk, 0 = planner                        */
     size_t mem_limit; /* device bytes the workspaces may use, 0 = free memory    */
+    int host_copy;    /* host entry points (cutc_int, cutc_int_ex): 0 = download
+                         each r1 chunk of X while the build runs (pinned staging,
+                         copy stream, host copy threads); 1 = blocking copies
+                         after the build, as the reference                      */
 } cutc_options;
 
 typedef struct cutc_stats {

@@ -1,4 +1,4 @@
-! Fortran binding of libcutcint's cutc_int and cutc_no (bind(C)).
+! Fortran binding of libcutcint's cutc_int, cutc_no, cutc_ao_to_mo and cutc_aos (bind(C)).
 !
 ! Array shapes follow the reference binding src/cutc_module.f90:23-35.  The
 ! argument list follows the C definition src/cutc_int.c:15-20, which takes six
@@ -62,6 +62,42 @@ module cutc_module
       real(c_double), intent(out)       :: no_0e
 
     end function cutc_no
+
+    ! AO -> MO transform of int2_grad1_u12 between cutc_int and cutc_no (no
+    ! counterpart in the reference): int2_grad1_u12(g,x,i,j) =
+    ! sum_{k,l} mo_l_coef(k,i) int2_grad1_u12_ao(k,l,g,x) mo_r_coef(l,j).
+    integer(c_int) function cutc_ao_to_mo(n_grid1, n_ao, n_mo,              &
+                                          mo_l_coef, mo_r_coef,             &
+                                          int2_grad1_u12_ao, int2_grad1_u12)&
+                                          bind(C, name = "cutc_ao_to_mo")
+
+      import c_int, c_double
+      integer(c_int), intent(in), value :: n_grid1, n_ao, n_mo
+      real(c_double), intent(in)        :: mo_l_coef(n_ao,n_mo)
+      real(c_double), intent(in)        :: mo_r_coef(n_ao,n_mo)
+      real(c_double), intent(in)        :: int2_grad1_u12_ao(n_ao,n_ao,n_grid1,3)
+      real(c_double), intent(out)       :: int2_grad1_u12(n_grid1,3,n_mo,n_mo)
+
+    end function cutc_ao_to_mo
+
+    ! AO values and gradients on a grid, aos_data(n_grid,n_ao,4) as cutc_int
+    ! reads it (no counterpart in the reference).  ao_nucl holds 1-based
+    ! nucleus numbers.
+    integer(c_int) function cutc_aos(n_grid, n_nuc, n_ao, prim_max, r, rn,  &
+                                     ao_nucl, ao_power, ao_prim_num,        &
+                                     ao_expo, ao_coef, aos_data)            &
+                                     bind(C, name = "cutc_aos")
+
+      import c_int, c_double
+      integer(c_int), intent(in), value :: n_grid, n_nuc, n_ao, prim_max
+      real(c_double), intent(in)        :: r(3,n_grid), rn(3,n_nuc)
+      integer(c_int), intent(in)        :: ao_nucl(n_ao), ao_power(n_ao,3)
+      integer(c_int), intent(in)        :: ao_prim_num(n_ao)
+      real(c_double), intent(in)        :: ao_expo(prim_max,n_ao)
+      real(c_double), intent(in)        :: ao_coef(prim_max,n_ao)
+      real(c_double), intent(out)       :: aos_data(n_grid,n_ao,4)
+
+    end function cutc_aos
 
   end interface
 

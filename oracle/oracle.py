@@ -68,6 +68,11 @@ def lib():
         L.oracle_int_2e_ao.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.oracle_cutc_no.restype = ctypes.c_int
         L.oracle_cutc_no.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_aos.restype = ctypes.c_int
+        L.oracle_aos.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip, _ip, _dp, _dp,
+                                 _dp]
+        L.oracle_ao_to_mo.restype = ctypes.c_int
+        L.oracle_ao_to_mo.argtypes = [_i64, _i64, _i64, _dp, _dp, _dp, _dp]
         ob = _find_openblas()
         if ob and os.environ.get("CUTC_ORACLE_BUILTIN_DGEMM", "0") != "1":
             L.oracle_use_blas(ob.encode(), b"scipy_dgemm_64_")
@@ -202,3 +207,39 @@ def cutc_no(ne_a, ne_b, wr1, mos_l, mos_r, G):
     if rc != 0:
         raise ValueError(f"oracle_cutc_no returned {rc}")
     return no_2e, no_1e, float(no_0e[0])
+
+
+def ao_to_mo(mo_l_coef, mo_r_coef, X):
+    """AO -> MO transform (oracle/oracle_mo.c).  numpy C order of the Fortran
+    layouts: mo_l_coef / mo_r_coef (n_mo, n_ao), X (3, n_grid1, n_ao, n_ao) =
+    int2_grad1_u12_ao(n_ao, n_ao, n_grid1, 3).  Returns int2_grad1_u12 as
+    (n_mo, n_mo, 3, n_grid1): [j, i, x, g] = sum_kl C_L[i, k] X[x, g, l, k] C_R[j, l]."""
+    X = _f64(X)
+    cl, cr = _f64(mo_l_coef), _f64(mo_r_coef)
+    n1, na, nm = X.shape[1], X.shape[2], cl.shape[0]
+    assert X.shape == (3, n1, na, na) and cl.shape == (nm, na) and cr.shape == (nm, na)
+    out = np.zeros((nm, nm, 3, n1))
+    rc = lib().oracle_ao_to_mo(n1, na, nm, _p(cl), _p(cr), _p(X), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_ao_to_mo returned {rc}")
+    return out
+
+
+def aos(r, rn, ao_nucl, ao_power, ao_prim_num, ao_expo, ao_coef):
+    """AO values and gradients (oracle/oracle_ao.c).  numpy C order of the
+    Fortran layouts: r (n, 3), rn (n_nuc, 3), ao_nucl (n_ao,) 1-based,
+    ao_power (3, n_ao) = Fortran (n_ao, 3), ao_prim_num (n_ao,), ao_expo /
+    ao_coef (n_ao, prim_max) = Fortran (prim_max, n_ao).  Returns
+    (4, n_ao, n) = Fortran aos_data(n, n_ao, 4)."""
+    r, rn = _f64(r), _f64(rn)
+    e, c = _f64(ao_expo), _f64(ao_coef)
+    nu, pw, npr = _i32(ao_nucl), _i32(ao_power), _i32(ao_prim_num)
+    n, n_ao = r.shape[0], nu.shape[0]
+    pmax = e.shape[1] if e.ndim == 2 else 0
+    out = np.zeros((4, n_ao, n))
+    rc = lib().oracle_aos(n, rn.shape[0], n_ao, pmax, _p(r), _p(rn), _p(nu, _ip), _p(pw, _ip), _p(npr, _ip), _p(e),
+                          _p(c), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_aos returned {rc}")
+    return out
+

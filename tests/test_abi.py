@@ -12,7 +12,7 @@ import pytest
 from cutc_amd import _lib, cutc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("cutc_int.h", "cutc_no.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("cutc_int.h", "cutc_no.h", "cutc_mo.h", "cutc_ao.h")]
 HEADER = HEADERS[0]
 
 

@@ -156,6 +156,28 @@ def test_chunking_and_reference_stage_functions(torch):
     assert relerr(X.cpu().numpy(), X0) <= 1e-14 and relerr(I.cpu().numpy(), I0) <= 1e-14
 
 
+@pytest.mark.parametrize("n1_chunk,precision", [(700, "fp64"), (0, "fp64"), (500, "int8")])
+def test_host_entry_pipelined_download(torch, n1_chunk, precision):
+    """cutc_int_ex on host arrays downloads each r1 chunk of X while the build
+    runs (pinned staging ring, copy stream, host copy threads): the host
+    arrays equal the device results, and the blocking copies (host_copy=1)
+    give the same bytes.  n1_chunk 700: 5 chunks of 20 MB per component; 0:
+    one chunk whose 86 MB components cross the 64 MiB staging pieces."""
+    s = synth.make_system(8, 60, 3000, 4000, 31, name="xfer")
+    print(f"pipelined download n1_chunk={n1_chunk} {precision} {s.shape_str()}", flush=True)
+    X0, I0, st0 = run_device(torch, s, n1_chunk=n1_chunk, precision=precision)
+    Xp, Ip, stp = cutc.cutc_int_ex(s, cutc.options(precision=precision, n1_chunk=n1_chunk))
+    Xb, Ib, _ = cutc.cutc_int_ex(s, cutc.options(precision=precision, n1_chunk=n1_chunk, host_copy=1))
+    assert stp["n1_chunks"] == st0["n1_chunks"]
+    assert (stp["n1_chunks"] > 1) == (n1_chunk > 0)
+    assert np.array_equal(Xp, Xb) and np.array_equal(Ip, Ib)
+    assert relerr(Xp, X0[:3]) <= 1e-15 and relerr(Ip, I0) <= 1e-14
+    # empty grid 2: X written without r1 chunks, downloaded after the build
+    e2 = s.subset_r2(0)
+    Xz, Iz, _ = cutc.cutc_int_ex(e2, cutc.options(precision=precision))
+    assert Xz.shape == Xp.shape and not np.any(Xz) and not np.any(Iz)
+
+
 def test_empty_and_error_paths(torch):
     s = synth.triatomic(n1=5, n2=9)
     # n1 = 0: int_2e_ao is 0
