@@ -252,7 +252,7 @@ def no_bench(args, torch, cutc):
                       "chunk": last["chunk"], "n_chunks": last["n_chunks"]},
            "build_s": ms / 1e3, "w_flops": W,
            "stage_ms": {k: last[k] for k in ("ms_point", "ms_pair", "ms_gemm2e", "ms_1e", "ms_symm")},
-           "roofline": {"kernel": "gemm_f64_kernel<0,0,6> (2e contractions, 4 launches per chunk)", "bound": "mfma",
+           "roofline": {"kernel": "gemm_f64_kernel<0,0,7> (2e contractions, 4 launches per chunk)", "bound": "mfma",
                         "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
                         "frac": achieved / PEAK["fp64"], "traffic": None}}
     del G, L, R, w, o2, o1, o0
@@ -321,7 +321,7 @@ def mo_bench(args, torch, cutc):
                          "mo_chunk": last["mo_chunk"], "n_chunks": last["n_chunks"]},
               "w_flops": W, "min_hbm_bytes": hbm, "hbm_GBps_at_min_bytes": hbm / (ms / 1e3) / 1e9,
               "stage_ms": {"ms_left": last["ms_left"], "ms_right": last["ms_right"]},
-              "roofline": {"kernel": "gemm_f64_kernel<2,2,6> (2 launches)", "bound": "mfma", "achieved": achieved,
+              "roofline": {"kernel": "gemm_f64_kernel<3,3,7> (2 launches; even n_ao: 16-byte pair loads)", "bound": "mfma", "achieved": achieved,
                            "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"],
                            "traffic": None}}
     del X, out, cl, cr

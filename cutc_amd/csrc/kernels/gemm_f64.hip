@@ -25,10 +25,15 @@
 #define LDSS 18
 
 // ROWK_ANY: K-contiguous rows with any leading dimension and K (8-byte loads,
-// k >= Ka read as zero)
-enum { OPA_ROWK = 0, OPA_COLK_GATHER = 1, OPA_ROWK_ANY = 2 };
-enum { OPB_ROWK = 0, OPB_COLK = 1, OPB_ROWK_ANY = 2 };
-enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4, EPI_RAW = 5, EPI_DENSE = 6 };
+// k >= Ka read as zero).  ROWK_PAIR: even leading dimension and Ka, 16-byte
+// aligned rows: 16-byte loads, pairs at k >= Ka read as zero.
+enum { OPA_ROWK = 0, OPA_COLK_GATHER = 1, OPA_ROWK_ANY = 2, OPA_ROWK_PAIR = 3 };
+enum { OPB_ROWK = 0, OPB_COLK = 1, OPB_ROWK_ANY = 2, OPB_ROWK_PAIR = 3 };
+// EPI_DENSE_T: the block computes C^T (operands exchanged by the caller) and
+// stores element (row, col) at out[row ldc + col], so that the 16 lanes of
+// an MFMA column write 16 consecutive doubles of C
+enum { EPI_X = 0, EPI_X_ADD = 1, EPI_M_STORE = 2, EPI_M_ADD = 3, EPI_M_ADDSUB = 4, EPI_RAW = 5, EPI_DENSE = 6,
+       EPI_DENSE_T = 7 };
 
 struct GemmArgs {
     const double *A;
@@ -51,6 +56,7 @@ struct GemmArgs {
 // Fetch this thread's 8 operand values of one K step into registers.
 // ROWK:   thread t -> row t/2, k = 8 (t%2) .. +7   (K-contiguous rows)
 // ROWK_ANY: the same elements one by one, k >= klim read as zero
+// ROWK_PAIR: the 16-byte loads of ROWK, pairs at k >= klim read as zero
 // COLK:   thread t -> row t%128, k = 8 (t/128) .. +7 (rows of K at stride ld),
 //         rows >= klim read row klim - 1
 template <int MODE>
@@ -58,14 +64,15 @@ __device__ static inline void fetch(const double *__restrict__ P, int64_t ld, in
                                     int64_t row0, int64_t k0, int64_t gidx, double (&r)[8])
 {
     const int t = threadIdx.x;
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 || MODE == 3) {
         int64_t row = row0 + (t >> 1);
         if (row >= nrows)
             row = nrows - 1;
-        const v2d *src = (const v2d *)(P + row * ld + k0 + 8 * (t & 1));
+        const int64_t kb = k0 + 8 * (t & 1);
+        const v2d *src = (const v2d *)(P + row * ld + kb);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            v2d v = src[i];
+            v2d v = (MODE == 0 || kb + 2 * i < klim) ? src[i] : v2d{0.0, 0.0};
             r[2 * i] = v[0];
             r[2 * i + 1] = v[1];
         }
@@ -148,7 +155,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
     const int64_t nk = (g.K + BK - 1) / BK;
     double ra[8], rb[8];
     fetch<AM>(g.A, g.lda, g.M, g.Ka, row0, 0, aidx, ra);
-    const int64_t kb_lim = (BMODE == OPB_ROWK_ANY) ? g.Ka : g.K;
+    const int64_t kb_lim = (BMODE == OPB_ROWK_ANY || BMODE == OPB_ROWK_PAIR) ? g.Ka : g.K;
     fetch<BMODE>(g.B, g.ldb, g.N, kb_lim, col0, 0, bidx, rb);
     stash<AM>(As[0], ra);
     stash<BMODE>(Bs[0], rb);
@@ -205,8 +212,8 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g)
                     g.out[((int64_t)blockIdx.y * g.N + col) * g.M + row] = acc[i][j][r];
                     continue;
                 }
-                if constexpr (EPI == EPI_DENSE) {
-                    double *o = g.out + col * g.ldc + row;
+                if constexpr (EPI == EPI_DENSE || EPI == EPI_DENSE_T) {
+                    double *o = (EPI == EPI_DENSE) ? g.out + col * g.ldc + row : g.out + row * g.ldc + col;
                     *o = (g.beta == 0.0) ? g.alpha * acc[i][j][r] : fma(g.alpha, acc[i][j][r], g.beta * *o);
                     continue;
                 }
@@ -365,13 +372,14 @@ extern "C" int cutc_gemm2_split_count(int64_t K, int nsplit)
 }
 
 /* Dense C (M x N, column-major, ld ldc) = alpha A^T B + beta C for
- * K-contiguous operands: A row i at A + i lda, B row j at B + j ldb.  K a
- * multiple of BK, even leading dimensions and 16-byte aligned operands take
- * 16-byte loads (both operands must then read as zero past the valid K, as
- * the normal-ordering operands do); any other K or leading dimension takes
- * the 8-byte-load form that stops at K.  nsplit > 1: the K rows run in slices
- * whose unscaled partial sums go to work[y][j][i] instead (alpha, beta and C
- * are then unused); the caller adds the slices. */
+ * K-contiguous operands: A row i at A + i lda, B row j at B + j ldb.
+ * Operand loads: K a multiple of BK with even leading dimensions and 16-byte
+ * aligned operands reads whole 16-wide K steps; an even K with the same
+ * alignment reads 16-byte pairs up to K; anything else reads 8-byte elements
+ * up to K.  nsplit == 1 runs the block on C^T (operands exchanged) so that
+ * the stores of C run along its columns.  nsplit > 1: the K rows run in
+ * slices whose unscaled partial sums go to work[y][j][i] instead (alpha, beta
+ * and C are then unused); the caller adds the slices. */
 extern "C" int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t M, int64_t N,
                                 int64_t K, double alpha, double beta, double *C, int64_t ldc, int nsplit,
                                 double *work, hipStream_t st)
@@ -380,30 +388,42 @@ extern "C" int cutc_gemm_tn_f64(const double *A, int64_t lda, const double *B, i
         return 0;
     if (K <= 0 || lda < K || ldb < K || nsplit < 1 || (nsplit == 1 && ldc < M) || (nsplit > 1 && !work))
         return CUTC_ERR_ARG;
-    const bool fast = K % BK == 0 && lda % 2 == 0 && ldb % 2 == 0 && ((uintptr_t)A % 16) == 0 &&
-                      ((uintptr_t)B % 16) == 0;
+    const bool wide = lda % 2 == 0 && ldb % 2 == 0 && ((uintptr_t)A % 16) == 0 && ((uintptr_t)B % 16) == 0;
+    const int form = (wide && K % BK == 0) ? 0 : (wide && K % 2 == 0) ? 1 : 2;
     GemmArgs g = {};
-    g.A = A;
-    g.lda = lda;
-    g.B = B;
-    g.ldb = ldb;
-    g.M = M;
-    g.N = N;
     g.K = (K + BK - 1) / BK * BK;
     g.Ka = K;
     g.alpha = alpha;
     g.beta = beta;
     g.ldc = ldc;
     if (nsplit == 1) {
+        g.A = B; // the block computes C^T = B^T A
+        g.lda = ldb;
+        g.B = A;
+        g.ldb = lda;
+        g.M = N;
+        g.N = M;
         g.out = C;
-        return fast ? launch_gemm<OPA_ROWK, OPB_ROWK, EPI_DENSE>(g, st)
-                    : launch_gemm<OPA_ROWK_ANY, OPB_ROWK_ANY, EPI_DENSE>(g, st);
+        switch (form) {
+        case 0: return launch_gemm<OPA_ROWK, OPB_ROWK, EPI_DENSE_T>(g, st);
+        case 1: return launch_gemm<OPA_ROWK_PAIR, OPB_ROWK_PAIR, EPI_DENSE_T>(g, st);
+        default: return launch_gemm<OPA_ROWK_ANY, OPB_ROWK_ANY, EPI_DENSE_T>(g, st);
+        }
     }
+    g.A = A;
+    g.lda = lda;
+    g.B = B;
+    g.ldb = ldb;
+    g.M = M;
+    g.N = N;
     g.out = work;
     g.kslice = ((g.K + nsplit - 1) / nsplit + BK - 1) / BK * BK;
     const int ns = (int)((g.K + g.kslice - 1) / g.kslice);
-    return fast ? launch_gemm<OPA_ROWK, OPB_ROWK, EPI_RAW>(g, st, ns)
-                : launch_gemm<OPA_ROWK_ANY, OPB_ROWK_ANY, EPI_RAW>(g, st, ns);
+    switch (form) {
+    case 0: return launch_gemm<OPA_ROWK, OPB_ROWK, EPI_RAW>(g, st, ns);
+    case 1: return launch_gemm<OPA_ROWK_PAIR, OPB_ROWK_PAIR, EPI_RAW>(g, st, ns);
+    default: return launch_gemm<OPA_ROWK_ANY, OPB_ROWK_ANY, EPI_RAW>(g, st, ns);
+    }
 }
 
 extern "C" int cutc_gemm_tn_split_count(int64_t K, int nsplit)

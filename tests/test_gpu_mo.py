@@ -82,9 +82,12 @@ def test_ao_to_mo_host_entry_and_edges(torch):
     assert L.cutc_ao_to_mo(2, 2, 2, None, None, None, None) == -2
 
 
-@pytest.mark.parametrize("M,N,K,nsplit", [(130, 77, 200, 1), (64, 300, 201, 1), (300, 40, 1001, 3), (5, 3, 7, 1)])
+@pytest.mark.parametrize("M,N,K,nsplit", [(130, 77, 200, 1), (64, 300, 201, 1), (300, 40, 1001, 3), (5, 3, 7, 1),
+                                         (130, 50, 1000, 4), (257, 129, 96, 1)])
 def test_gemm_tn_any_k(torch, M, N, K, nsplit):
-    """cutc_gemm_tn_f64 with K off the 16-step and odd leading dimensions"""
+    """cutc_gemm_tn_f64 in its three load forms: K a multiple of 16 (96), even K
+    (200, 1000: 16-byte pairs up to K), odd K and leading dimensions (8-byte
+    elements); one slice and split K"""
     print(f"gemm_tn any-K M={M} N={N} K={K} nsplit={nsplit}", flush=True)
     L = cutc.lib()
     L.cutc_gemm_tn_f64.restype = ctypes.c_int
