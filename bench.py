@@ -420,6 +420,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.traffic_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(args, "traffic probe: copied 1 GiB device to device")
+
     if args.workload == "batch64":
         from cutc_amd import cutc, synth
         batch_bench(args, torch, cutc, synth)
@@ -438,14 +446,6 @@ def main():
         return
 
     from cutc_amd import cutc, synth
-
-    if args.traffic_probe:
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        log(args, "traffic probe: copied 1 GiB device to device")
 
     t0 = time.perf_counter()
     streamed = args.workload == "cube40"
