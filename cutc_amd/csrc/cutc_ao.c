@@ -44,25 +44,7 @@ static int aos_path(int n_grid, int n_nuc, int n_ao, int prim_max, const double 
         return CUTC_OK;
     if (!r || !nucl || !power || !nprim || !out || n_nuc == 0 || !rn || (prim_max > 0 && (!expo || !coef)))
         return CUTC_ERR_ARG;
-    int rc = CUTC_OK;
-    int *h = (int *)malloc(sizeof(int) * 5 * (size_t)n_ao);
-    if (!h)
-        return CUTC_ERR_NOMEM;
-    rc = hrc(hipMemcpyAsync(h, nucl, sizeof(int) * n_ao, hipMemcpyDeviceToHost, st));
-    if (rc == CUTC_OK)
-        rc = hrc(hipMemcpyAsync(h + n_ao, power, sizeof(int) * 3 * n_ao, hipMemcpyDeviceToHost, st));
-    if (rc == CUTC_OK)
-        rc = hrc(hipMemcpyAsync(h + 4 * n_ao, nprim, sizeof(int) * n_ao, hipMemcpyDeviceToHost, st));
-    if (rc == CUTC_OK)
-        rc = hrc(hipStreamSynchronize(st));
-    if (rc == CUTC_OK)
-        rc = check_tables(n_nuc, n_ao, prim_max, h, h + n_ao, h + 4 * n_ao);
-    free(h);
-    if (rc == CUTC_OK)
-        rc = cutc_launch_aos(n_grid, r, rn, n_ao, nucl, power, nprim, prim_max, expo, coef, out, st);
-    if (rc == CUTC_OK)
-        rc = hrc(hipStreamSynchronize(st));
-    return rc;
+    return cutc_launch_aos(n_grid, r, rn, n_nuc, n_ao, nucl, power, nprim, prim_max, expo, coef, out, st);
 }
 
 int cutc_aos_device(int n_grid, int n_nuc, int n_ao, int prim_max, const double *r, const double *rn,
@@ -105,7 +87,7 @@ int cutc_aos(int n_grid, int n_nuc, int n_ao, int prim_max, double *h_r, double 
         for (int i = 0; i < 7; i++)
             if (sz[i] > 0 && (rc = hrc(hipMemcpy(d[i], hin[i], sz[i], hipMemcpyHostToDevice))) != CUTC_OK)
                 goto out;
-        rc = cutc_launch_aos(n_grid, (const double *)d[0], (const double *)d[1], n_ao, (const int *)d[2],
+        rc = cutc_launch_aos(n_grid, (const double *)d[0], (const double *)d[1], n_nuc, n_ao, (const int *)d[2],
                              (const int *)d[3], (const int *)d[4], prim_max, (const double *)d[5],
                              (const double *)d[6], (double *)d[7], NULL);
         if (rc == CUTC_OK)

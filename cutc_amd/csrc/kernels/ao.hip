@@ -8,6 +8,8 @@
 // the block (scalar loads) and each of the 4 output planes is written as
 // one contiguous 2 KB run per block.  HBM-bound: 32 bytes written per
 // (point, AO) against 24 read (the point, shared by all AOs through L2).
+// Table entries out of range (centre, powers, primitive count) give NaN for
+// that AO instead of reading through them.
 #include "common.h"
 #include "launch.h"
 
@@ -20,7 +22,7 @@ __device__ static inline double ipow_small(double x, int e)
 }
 
 __global__ __launch_bounds__(256) void aos_kernel(int64_t n, const double *__restrict__ r,
-                                                  const double *__restrict__ rn, int n_ao,
+                                                  const double *__restrict__ rn, int n_nuc, int n_ao,
                                                   const int *__restrict__ nucl, const int *__restrict__ power,
                                                   const int *__restrict__ nprim, int pmax,
                                                   const double *__restrict__ expo, const double *__restrict__ coef,
@@ -30,11 +32,18 @@ __global__ __launch_bounds__(256) void aos_kernel(int64_t n, const double *__res
     const int i = blockIdx.y;
     if (g >= n)
         return;
+    const int64_t ld = n * n_ao, o = g + n * i;
     const int a = nucl[i] - 1;
-    const double x = r[3 * g] - rn[3 * a], y = r[3 * g + 1] - rn[3 * a + 1], z = r[3 * g + 2] - rn[3 * a + 2];
-    const double r2 = x * x + y * y + z * z;
     const int ea = power[i], eb = power[i + n_ao], ec = power[i + 2 * n_ao];
     const int np = nprim[i];
+    if (a < 0 || a >= n_nuc || ea < 0 || eb < 0 || ec < 0 || np < 0 || np > pmax) {
+        // a table entry out of range: NaN for this AO, nothing read through it
+        for (int c = 0; c < 4; c++)
+            out[o + c * ld] = __builtin_nan("");
+        return;
+    }
+    const double x = r[3 * g] - rn[3 * a], y = r[3 * g + 1] - rn[3 * a + 1], z = r[3 * g + 2] - rn[3 * a + 2];
+    const double r2 = x * x + y * y + z * z;
     const double *al = expo + (int64_t)pmax * i, *cf = coef + (int64_t)pmax * i;
     // radial part R = sum_p c_p exp(-alpha_p r2) and D = dR/d(r2) * 2 = sum_p -2 alpha_p c_p exp(-alpha_p r2)
     double R = 0.0, D = 0.0;
@@ -48,14 +57,13 @@ __global__ __launch_bounds__(256) void aos_kernel(int64_t n, const double *__res
     const double qx = (ea ? ea * ipow_small(x, ea - 1) * R : 0.0) + px * x * D;
     const double qy = (eb ? eb * ipow_small(y, eb - 1) * R : 0.0) + py * y * D;
     const double qz = (ec ? ec * ipow_small(z, ec - 1) * R : 0.0) + pz * z * D;
-    const int64_t ld = n * n_ao, o = g + n * i;
     out[o] = px * py * pz * R;
     out[o + ld] = qx * py * pz;
     out[o + 2 * ld] = px * qy * pz;
     out[o + 3 * ld] = px * py * qz;
 }
 
-extern "C" int cutc_launch_aos(int64_t n, const double *r, const double *rn, int n_ao, const int *nucl,
+extern "C" int cutc_launch_aos(int64_t n, const double *r, const double *rn, int n_nuc, int n_ao, const int *nucl,
                                const int *power, const int *nprim, int pmax, const double *expo, const double *coef,
                                double *out, hipStream_t st)
 {
@@ -64,7 +72,7 @@ extern "C" int cutc_launch_aos(int64_t n, const double *r, const double *rn, int
     const int64_t nb = (n + 255) / 256;
     if (nb > 2147483647 || n_ao > 65535)
         return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(aos_kernel, dim3((unsigned)nb, (unsigned)n_ao), dim3(256), 0, st, n, r, rn, n_ao, nucl, power,
-                       nprim, pmax, expo, coef, out);
+    hipLaunchKernelGGL(aos_kernel, dim3((unsigned)nb, (unsigned)n_ao), dim3(256), 0, st, n, r, rn, n_nuc, n_ao, nucl,
+                       power, nprim, pmax, expo, coef, out);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

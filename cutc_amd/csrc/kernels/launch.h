@@ -59,10 +59,11 @@ int cutc_launch_nonherm_ref(int64_t n1, int n_ao, const double *wr1, const doubl
 
 /* Contracted Cartesian Gaussian AOs and gradients (ao.hip): out[g + n (i +
  * n_ao c)], c = 0 value, 1..3 d/dx, d/dy, d/dz; nucl 1-based, power (n_ao,
- * 3), expo / coef (pmax, n_ao).  Tables must be valid (checked by the
- * caller); n_ao <= 65535. */
-int cutc_launch_aos(int64_t n, const double *r, const double *rn, int n_ao, const int *nucl, const int *power,
-                    const int *nprim, int pmax, const double *expo, const double *coef, double *out, hipStream_t st);
+ * 3), expo / coef (pmax, n_ao); an AO whose entries are out of range gets
+ * NaN; n_ao <= 65535. */
+int cutc_launch_aos(int64_t n, const double *r, const double *rn, int n_nuc, int n_ao, const int *nucl,
+                    const int *power, const int *nprim, int pmax, const double *expo, const double *coef, double *out,
+                    hipStream_t st);
 
 /* dst[0..n) = value. */
 int cutc_launch_fill(double *dst, int64_t n, double value, hipStream_t st);

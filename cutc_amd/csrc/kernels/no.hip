@@ -193,14 +193,9 @@ __global__ __launch_bounds__(256) void no_orb_kernel(cutc_no_args a)
     }
 }
 
-// Per (point, orbital p), all s: C1x, the copy Gx of the chunk of G, C2 and
-// D2 in the GEMM layouts.  The G_x(p, e) of NO_TILE orbitals e sit in
-// registers while s runs; more occupied orbitals take further passes that
-// add into C2.  Grid (ceil(ldk / 256), nm).
-__global__ __launch_bounds__(256) void no_pair_kernel(cutc_no_args a)
+// one (point, orbital p) of no_pair_kernel
+__device__ static void no_pair_point(const cutc_no_args &a, const int p, const int64_t gl)
 {
-    const int64_t gl = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int p = blockIdx.y;
     if (gl >= a.ldk)
         return;
     const int nm = a.nm, nb = a.nb, ne = a.ne;
@@ -257,6 +252,19 @@ __global__ __launch_bounds__(256) void no_pair_kernel(cutc_no_args a)
             }
         }
     }
+}
+
+// Per (point, orbital p), all s: C1x, the copy Gx of the chunk of G, C2 and
+// D2 in the GEMM layouts.  The G_x(p, e) of NO_TILE orbitals e sit in
+// registers while s runs; more occupied orbitals take further passes that
+// add into C2.  Grid (nm, blocks of 256 points): the nm blocks of one point
+// range are adjacent in launch order, so the occupied rows G_x(e, .) they all
+// read come from L2 once the first has fetched them.
+__global__ __launch_bounds__(256) void no_pair_kernel(cutc_no_args a)
+{
+    const int p = blockIdx.x;
+    for (int64_t gb = blockIdx.y; gb * 256 < a.ldk; gb += gridDim.y)
+        no_pair_point(a, p, gb * 256 + threadIdx.x);
 }
 
 // no_1e[b] += 2 (sum_x sum_k Gx[x][b][k] Dv[x][k] + sum_k C2[b][k] Dv[3][k])
@@ -362,7 +370,8 @@ extern "C" int cutc_no_pair(const cutc_no_args *a, hipStream_t st)
 {
     if (a->ngc <= 0 || a->nm <= 0)
         return 0;
-    hipLaunchKernelGGL(no_pair_kernel, dim3((unsigned)((a->ldk + 255) / 256), (unsigned)a->nm), dim3(256), 0, st,
+    const int64_t nb = (a->ldk + 255) / 256;
+    hipLaunchKernelGGL(no_pair_kernel, dim3((unsigned)a->nm, (unsigned)(nb < 65535 ? nb : 65535)), dim3(256), 0, st,
                        *a);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -41,9 +41,11 @@ extern "C" {
 int cutc_aos(int n_grid, int n_nuc, int n_ao, int prim_max, double *h_r, double *h_rn, int *h_ao_nucl,
              int *h_ao_power, int *h_ao_prim_num, double *h_ao_expo, double *h_ao_coef, double *h_aos_data);
 
-/* The same on device pointers and a caller stream (NULL: the null stream);
- * returns when the values are written.  The integer tables are checked on
- * the host (they are copied back once per call, n_ao * 5 integers). */
+/* The same on device pointers, queued on a caller stream (NULL: the null
+ * stream); no host synchronisation.  The tables are not read on the host:
+ * an AO whose centre, powers or primitive count is out of range gets NaN in
+ * its four rows (nothing is read through the bad entry).  Returns 0, or
+ * CUTC_ERR_ARG for negative sizes, n_ao > 65535 or missing pointers. */
 int cutc_aos_device(int n_grid, int n_nuc, int n_ao, int prim_max, const double *r, const double *rn,
                     const int *ao_nucl, const int *ao_power, const int *ao_prim_num, const double *ao_expo,
                     const double *ao_coef, double *aos_data, void *stream);

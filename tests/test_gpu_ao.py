@@ -62,9 +62,15 @@ def test_aos_host_entry_and_errors(torch):
     with pytest.raises(cutc.CutcError) as e:
         cutc.cutc_aos(s.r1, s.rn, bad)
     assert e.value.code == cutc.CUTC_ERR_ARG
-    with pytest.raises(cutc.CutcError):
-        cutc.cutc_aos_device(*(torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (s.r1, s.rn)),
-                             cutc.upload_basis(bad), torch.empty((4, 50, 500), dtype=torch.float64, device="cuda"))
+    # the device entry point does not read the tables on the host: the bad
+    # AO gets NaN rows, the others their values
+    out = torch.zeros((4, 50, 500), dtype=torch.float64, device="cuda")
+    cutc.cutc_aos_device(*(torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (s.r1, s.rn)),
+                         cutc.upload_basis(bad), out)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    assert np.isnan(o[:, 3]).all() and not np.isnan(np.delete(o, 3, axis=1)).any()
+    close(np.delete(o, 3, axis=1), np.delete(s.aos1, 3, axis=1))
 
 
 def test_cutc_int_on_device_aos(torch):
