@@ -51,7 +51,11 @@ PEAK = {"fp64": 78.6, "int8": 5000.0, "bf16": 2500.0, "fp16": 2500.0, "bf16crt":
 # a 1 GiB copy kernel measured in the same run (profiles/cube20_traffic.txt); not measured
 # for other shapes.
 BATCH_PRECISIONS = ("fp64", "bf16", "bf16crt", "int8")
-MEASURED_TRAFFIC = {"int8": 273.7e9, "fp64": 17.83e12}
+MEASURED_TRAFFIC = {"int8": 273.7e9, "fp64": 17.83e12,
+                    # the transform's GEMM launch (n_ao = n_mo = 200, n_grid1 = 60000) and the AO kernel
+                    # (260000 points, n_ao = 200), profiles/ao_to_mo_nao200_kernel_stats.txt and
+                    # profiles/aos_cube20_kernel_stats.txt
+                    "mo": 2 * 42588146 * 1024 + 56250000 * 1024, "ao": 2 * 3556 * 1024 + 1625000 * 1024}
 
 
 def w_ref(n_ao, n1, n2):
@@ -323,7 +327,9 @@ def mo_bench(args, torch, cutc):
               "stage_ms": {"ms_left": last["ms_left"], "ms_right": last["ms_right"]},
               "roofline": {"kernel": "gemm_f64_kernel<3,3,7> (2 launches; even n_ao: 16-byte pair loads)", "bound": "mfma", "achieved": achieved,
                            "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"],
-                           "traffic": None}}
+                           "traffic": MEASURED_TRAFFIC["mo"] if (n1, na, nm) == (60000, 200, 200) else None,
+                           "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), "
+                                           "profiles/ao_to_mo_nao200_kernel_stats.txt"}}
     del X, out, cl, cr
     cutc.release_workspace()
     torch.cuda.empty_cache()
@@ -394,7 +400,9 @@ def ao_bench(args, torch, cutc, synth):
                                      f"{s.n1} + {s.n2} grid points"},
               "roofline": {"kernel": "aos_kernel", "bound": "hbm", "achieved": nbytes / (ms / 1e3) / 1e9,
                            "peak": 8000.0, "unit": "GB/s", "frac": nbytes / (ms / 1e3) / 1e9 / 8000.0,
-                           "traffic": None, "bytes_per_launch": nbytes},
+                           "traffic": MEASURED_TRAFFIC["ao"] if n == 260000 else None, "bytes_per_launch": nbytes,
+                           "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), "
+                                           "profiles/aos_cube20_kernel_stats.txt"},
               "h2d_pageable_ms_same_bytes": h2d * 1e3}
     log(args, f"aos: {ms:.2f} ms, {out_js['value']:.0f} GB/s; upload of the same bytes {h2d * 1e3:.0f} ms")
     del out, r
@@ -645,16 +653,21 @@ def main():
     if args.host_path and not streamed:
         Xref_host = Iref = None
         hp = {}
-        for mode, hc in (("pipelined", 0), ("blocking", 1)):
-            cutc.release_workspace()
+        cutc.release_workspace()
+        # first call allocates the device workspace; the second (pipelined) and
+        # third (blocking) reuse it
+        for mode, hc in (("pipelined_first_call", 0), ("pipelined", 0), ("blocking", 1)):
             hopt = cutc.options(precision=opt.precision, slices=opt.slices, host_copy=hc)
             t0 = time.perf_counter()
             Xh, Ih, hst = cutc.cutc_int_ex(s, hopt)
             wall = time.perf_counter() - t0
-            cutc.release_workspace()
             del Xh, Ih
-            hp[mode] = {"wall_s": wall, "eff_tflops": W / wall / 1e12, "build_ms": hst["ms_total"]}
-            log(args, f"host path ({mode}): {wall:.2f} s, build {hst['ms_total'] / 1e3:.2f} s")
+            hp[mode] = {"wall_s": wall, "eff_tflops": W / wall / 1e12, "build_ms": hst["ms_total"],
+                        "n1_chunks": hst["n1_chunks"], "ms_gemm1": hst["ms_gemm1"], "ms_gemm2": hst["ms_gemm2"],
+                        "ms_split": hst["ms_split"]}
+            log(args, f"host path ({mode}): {wall:.2f} s, build {hst['ms_total'] / 1e3:.2f} s, "
+                      f"{hst['n1_chunks']} chunks, gemm1 {hst['ms_gemm1'] / 1e3:.2f} s")
+        cutc.release_workspace()
         hp["note"] = ("cutc_int_ex on host arrays: inputs uploaded, X[0..2] and int_2e_ao downloaded to pageable "
                       "(freshly allocated) host memory; pipelined: each r1 chunk of X is downloaded while the build "
                       "runs; blocking: hipMemcpy after the build, as the reference")
