@@ -664,9 +664,11 @@ def main():
             del Xh, Ih
             hp[mode] = {"wall_s": wall, "eff_tflops": W / wall / 1e12, "build_ms": hst["ms_total"],
                         "n1_chunks": hst["n1_chunks"], "ms_gemm1": hst["ms_gemm1"], "ms_gemm2": hst["ms_gemm2"],
-                        "ms_split": hst["ms_split"]}
+                        "ms_split": hst["ms_split"], "ms_host_in": hst["ms_host_in"],
+                        "ms_host_out": hst["ms_host_out"]}
             log(args, f"host path ({mode}): {wall:.2f} s, build {hst['ms_total'] / 1e3:.2f} s, "
-                      f"{hst['n1_chunks']} chunks, gemm1 {hst['ms_gemm1'] / 1e3:.2f} s")
+                      f"{hst['n1_chunks']} chunks, allocation + upload {hst['ms_host_in'] / 1e3:.2f} s, "
+                      f"after the build {hst['ms_host_out'] / 1e3:.2f} s")
         cutc.release_workspace()
         hp["note"] = ("cutc_int_ex on host arrays: inputs uploaded, X[0..2] and int_2e_ao downloaded to pageable "
                       "(freshly allocated) host memory; pipelined: each r1 chunk of X is downloaded while the build "

@@ -387,20 +387,61 @@ int cutc_internal_workspace(size_t need, char **base)
     return rc;
 }
 
+/* Device copies of the host entry points' arrays (inputs, X, int_2e_ao): a
+ * second grow-only allocation per device, kept and released with the
+ * workspace, so that repeated cutc_int_ex calls do not allocate (a fresh
+ * 90 GB hipMalloc beside a full workspace took 2.5 s). */
+static struct {
+    char *base;
+    size_t size;
+} g_hbuf[WS_MAX_DEV];
+
+static void free_on_device(int d, char **base, size_t *size)
+{
+    if (*base) {
+        int cur = 0;
+        hipGetDevice(&cur);
+        hipSetDevice(d);
+        hipDeviceSynchronize();
+        hipFree(*base);
+        hipSetDevice(cur);
+    }
+    *base = NULL;
+    *size = 0;
+}
+
 static void release_arenas(void)
 {
     for (int d = 0; d < WS_MAX_DEV; d++) {
-        if (g_arena[d].base) {
-            int cur = 0;
-            hipGetDevice(&cur);
-            hipSetDevice(d);
-            hipDeviceSynchronize();
-            hipFree(g_arena[d].base);
-            hipSetDevice(cur);
-        }
-        g_arena[d].base = NULL;
-        g_arena[d].size = 0;
+        free_on_device(d, &g_arena[d].base, &g_arena[d].size);
+        free_on_device(d, &g_hbuf[d].base, &g_hbuf[d].size);
     }
+}
+
+/* the host-array buffer of the current device, at least need bytes; when it
+ * has to grow and the device is full, the workspace is given back first */
+static int hbuf_get(size_t need, char **base)
+{
+    int dev = 0;
+    *base = NULL;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
+        return CUTC_ERR_HIP;
+    if (g_hbuf[dev].size < need) {
+        free_on_device(dev, &g_hbuf[dev].base, &g_hbuf[dev].size);
+        void *p = NULL;
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            free_on_device(dev, &g_arena[dev].base, &g_arena[dev].size);
+            e = hipMalloc(&p, need);
+        }
+        if (e != hipSuccess)
+            return hip_rc(e);
+        g_hbuf[dev].base = (char *)p;
+        g_hbuf[dev].size = need;
+    }
+    *base = g_hbuf[dev].base;
+    return CUTC_OK;
 }
 
 void cutc_release_workspace(void)
@@ -1404,17 +1445,18 @@ static int geometry_ok(int nx, int ny, int nz, int bx, int by, int bz)
     return CUTC_OK;
 }
 
-static int upload(void **d, const void *h, size_t bytes)
+/* carve the next 256-byte-aligned piece of bytes from the host-array buffer
+ * and fill it from h (h == NULL: leave it for the build) */
+static int carve_upload(char *base, size_t *off, void **d, const void *h, size_t bytes, int upload)
 {
     *d = NULL;
     if (bytes == 0)
         return CUTC_OK;
-    if (!h)
+    if (upload && !h)
         return CUTC_ERR_ARG;
-    int rc = hip_rc(hipMalloc(d, bytes));
-    if (rc)
-        return rc;
-    return hip_rc(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+    *d = base + *off;
+    *off += ws_round(bytes);
+    return upload ? hip_rc(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice)) : CUTC_OK;
 }
 
 /* Pipelined download of X by the host entry points: the rows of each r1
@@ -1439,11 +1481,18 @@ static void x_chunk_ready(void *ctx, int64_t i0, int64_t n1e, hipStream_t st)
     d->queued = i0 + n1e;
 }
 
+/* host copy workers beside the driver thread: CUTC_HOST_COPY_THREADS if set
+ * (0..16), else one less than the online CPUs, at most 15 */
 static int xfer_threads(void)
 {
+    const char *e = getenv("CUTC_HOST_COPY_THREADS");
+    if (e && *e) {
+        const long v = strtol(e, NULL, 10);
+        return (int)(v < 0 ? 0 : v > 16 ? 16 : v);
+    }
     long n = sysconf(_SC_NPROCESSORS_ONLN);
     n = (n > 1) ? n - 1 : 0;
-    return (int)(n > 7 ? 7 : n);
+    return (int)(n > 15 ? 15 : n);
 }
 
 static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nuc, int size_bh, const double *h_r1,
@@ -1469,26 +1518,31 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
     xdownload_t dl = {0};
     tracer_t hook;
     memset(&hook, 0, sizeof hook);
+    const double t_start = now_ms();
     TRYH(hipEventCreate(&e0));
     TRYH(hipEventCreate(&e1));
     TRYH(hipEventCreate(&e2));
     TRYH(hipEventCreate(&e3));
     TRYH(hipEventRecord(e0, NULL));
-    TRY(upload((void **)&r1, h_r1, sizeof(double) * 3 * n1));
-    TRY(upload((void **)&wr1, h_wr1, sizeof(double) * n1));
-    TRY(upload((void **)&r2, h_r2, sizeof(double) * 3 * n2));
-    TRY(upload((void **)&wr2, h_wr2, sizeof(double) * n2));
-    TRY(upload((void **)&rn, h_rn, sizeof(double) * 3 * n_nuc));
-    TRY(upload((void **)&aos1, h_aos1, sizeof(double) * 4 * (size_t)n1 * n_ao));
-    TRY(upload((void **)&aos2, h_aos2, sizeof(double) * 4 * (size_t)n2 * n_ao));
-    TRY(upload((void **)&c, h_c, sizeof(double) * nt));
-    TRY(upload((void **)&m, h_m, sizeof(int) * nt));
-    TRY(upload((void **)&n, h_n, sizeof(int) * nt));
-    TRY(upload((void **)&o, h_o, sizeof(int) * nt));
-    if (n1 > 0 && nao2 > 0)
-        TRYH(hipMalloc((void **)&X, sizeof(double) * 4 * nao2 * n1));
-    if (nao2 > 0)
-        TRYH(hipMalloc((void **)&out, sizeof(double) * nao2 * nao2));
+    {
+        /* every device array of the call in one cached buffer (hbuf_get) */
+        void **dst[13] = {(void **)&r1, (void **)&wr1, (void **)&r2, (void **)&wr2, (void **)&rn, (void **)&aos1,
+                          (void **)&aos2, (void **)&c, (void **)&m, (void **)&n, (void **)&o, (void **)&X,
+                          (void **)&out};
+        const void *src[13] = {h_r1, h_wr1, h_r2, h_wr2, h_rn, h_aos1, h_aos2, h_c, h_m, h_n, h_o, NULL, NULL};
+        const size_t d8 = sizeof(double), i4 = sizeof(int);
+        const size_t bytes[13] = {d8 * 3 * n1, d8 * n1, d8 * 3 * n2, d8 * n2, d8 * 3 * n_nuc,
+                                  d8 * 4 * (size_t)n1 * n_ao, d8 * 4 * (size_t)n2 * n_ao, d8 * nt, i4 * nt, i4 * nt,
+                                  i4 * nt, d8 * 4 * (size_t)(nao2 * n1), d8 * (size_t)(nao2 * nao2)};
+        size_t need = 0, off = 0;
+        for (int i = 0; i < 13; i++)
+            need += ws_round(bytes[i]);
+        char *base = NULL;
+        if (need > 0)
+            TRY(hbuf_get(need, &base));
+        for (int i = 0; i < 13; i++)
+            TRY(carve_upload(base, &off, dst[i], src[i], bytes[i], i < 11));
+    }
     if (!opt->host_copy && (X || out)) {
         TRY(cutc_xfer_open(&dl.xf, (size_t)64 << 20, 4, xfer_threads()));
         dl.h_X = h_X;
@@ -1501,8 +1555,10 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
         }
     }
     TRYH(hipEventRecord(e1, NULL));
+    const double t_in = now_ms();
     TRY(device_path(opt, 3, n1, n2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos1, aos2, c, m, n, o, X, out, NULL,
                     stats, &hook));
+    const double t_built = now_ms();
     TRYH(hipEventRecord(e2, NULL));
     if (dl.xf) {
         if (X && dl.queued < n1) /* rows written without r1 chunks (empty grid 2) */
@@ -1519,6 +1575,10 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
     }
     TRYH(hipEventRecord(e3, NULL));
     TRYH(hipEventSynchronize(e3));
+    if (stats) {
+        stats->ms_host_in = t_in - t_start;
+        stats->ms_host_out = now_ms() - t_built;
+    }
     if (opt->verbose) {
         float t_up = 0, t_comp = 0, t_down = 0;
         hipEventElapsedTime(&t_up, e0, e1);
@@ -1529,20 +1589,9 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
         fflush(stdout);
     }
 done:
-    cutc_xfer_close(dl.xf); /* drains queued copies before the buffers go */
-    dfree(r1);
-    dfree(wr1);
-    dfree(r2);
-    dfree(wr2);
-    dfree(rn);
-    dfree(aos1);
-    dfree(aos2);
-    dfree(c);
-    dfree(m);
-    dfree(n);
-    dfree(o);
-    dfree(X);
-    dfree(out);
+    cutc_xfer_close(dl.xf); /* drains queued copies before the buffers are reused */
+    if (rc != CUTC_OK)
+        hipDeviceSynchronize();
     if (e0)
         hipEventDestroy(e0);
     if (e1)

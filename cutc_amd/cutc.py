@@ -57,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("ms_integrands", ctypes.c_double), ("ms_gemm1", ctypes.c_double), ("ms_gemm2", ctypes.c_double),
                 ("ms_symm", ctypes.c_double), ("ms_split", ctypes.c_double), ("n_gemm1", ctypes.c_int),
                 ("n_gemm2", ctypes.c_int), ("n_jastrow", ctypes.c_int), ("n1_chunk", ctypes.c_int64),
-                ("n1_chunks", ctypes.c_int), ("flops_exec", ctypes.c_double), ("workspace_bytes", ctypes.c_size_t)]
+                ("n1_chunks", ctypes.c_int), ("flops_exec", ctypes.c_double), ("workspace_bytes", ctypes.c_size_t),
+                ("ms_host_in", ctypes.c_double), ("ms_host_out", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

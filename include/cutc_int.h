@@ -141,6 +141,8 @@ typedef struct cutc_stats {
     int n1_chunks;         /* number of r1 chunks                       */
     double flops_exec;     /* FP64-equivalent multiply-adds x 2 executed by the contractions */
     size_t workspace_bytes;/* device workspace allocated by the call    */
+    double ms_host_in;     /* host entry points: device allocation and upload before the build (wall clock) */
+    double ms_host_out;    /* host entry points: end of the build to the last byte in host memory (wall clock) */
 } cutc_stats;
 
 void cutc_default_options(cutc_options *opt);
