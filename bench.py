@@ -259,7 +259,27 @@ def no_bench(args, torch, cutc):
            "roofline": {"kernel": "gemm_f64_kernel<0,0,7> (2e contractions, 4 launches per chunk)", "bound": "mfma",
                         "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
                         "frac": achieved / PEAK["fp64"], "traffic": None}}
-    del G, L, R, w, o2, o1, o0
+    del o2, o1, o0
+    if args.host_path:
+        # the reference's entry point on host arrays: uploads, build, downloads
+        hG, hw, hL, hR = (t.cpu().numpy() for t in (G, w, L, R))
+        del G, L, R, w
+        cutc.release_workspace()
+        torch.cuda.empty_cache()
+        hp = {}
+        for mode in ("first_call", "repeated"):
+            t0 = time.perf_counter()
+            r = cutc.cutc_no(na, nb, hw, hL, hR, hG)
+            wall = time.perf_counter() - t0
+            del r
+            hp[mode] = {"wall_s": wall, "eff_tflops": W / wall / 1e12}
+            log(args, f"cutc_no on host arrays ({mode}): {wall:.2f} s")
+        hp["note"] = ("cutc_no (host arrays, the reference's entry point): int2_grad1_u12 (57.6 GB) and the other "
+                      "inputs uploaded, no_2e (12.8 GB), no_1e, no_0e downloaded to pageable host memory")
+        out["host_path"] = hp
+        del hG
+    else:
+        del G, L, R, w
     cutc.release_workspace()
     torch.cuda.empty_cache()
     out["cpu_baseline"] = None

@@ -1483,7 +1483,7 @@ static void x_chunk_ready(void *ctx, int64_t i0, int64_t n1e, hipStream_t st)
 
 /* host copy workers beside the driver thread: CUTC_HOST_COPY_THREADS if set
  * (0..16), else one less than the online CPUs, at most 15 */
-static int xfer_threads(void)
+int cutc_internal_copy_threads(void)
 {
     const char *e = getenv("CUTC_HOST_COPY_THREADS");
     if (e && *e) {
@@ -1544,7 +1544,7 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
             TRY(carve_upload(base, &off, dst[i], src[i], bytes[i], i < 11));
     }
     if (!opt->host_copy && (X || out)) {
-        TRY(cutc_xfer_open(&dl.xf, (size_t)64 << 20, 4, xfer_threads()));
+        TRY(cutc_xfer_open(&dl.xf, (size_t)64 << 20, 4, cutc_internal_copy_threads()));
         dl.h_X = h_X;
         dl.X = X;
         dl.n1 = n1;

@@ -21,6 +21,7 @@
 #include "cutc_no.h"
 #include "internal.h"
 #include "kernels/launch.h"
+#include "xfer.h"
 
 static int hrc(hipError_t e)
 {
@@ -94,9 +95,16 @@ static int ph_mark(phtimer *t, int phase)
     return 1;
 }
 
+/* called before the kernels of each grid chunk are queued, with the end of
+ * the chunk: the host entry point waits there for that part of its upload */
+typedef struct {
+    int (*before_chunk)(void *ctx, int64_t g_end);
+    void *ctx;
+} no_hook;
+
 static int no_path(int64_t ng, int nm, int na, int nb, const double *w, const double *mol, const double *mor,
                    const double *G, double *o2, double *o1, double *o0, size_t mem_limit, int64_t req_chunk,
-                   hipStream_t st, cutc_no_stats *stats)
+                   hipStream_t st, cutc_no_stats *stats, const no_hook *hook)
 {
     if (ng < 0 || nm < 0 || na < 0 || nb < 0 || na > nm || nb > nm)
         return CUTC_ERR_ARG;
@@ -213,6 +221,8 @@ static int no_path(int64_t ng, int nm, int na, int nb, const double *w, const do
         for (int64_t c = 0; c < nch; c++) {
             a.g0 = c * chunk;
             a.ngc = (ng - a.g0 < chunk) ? ng - a.g0 : chunk;
+            if (hook)
+                TRY(hook->before_chunk(hook->ctx, a.g0 + a.ngc));
             ph_mark(&pt, 0);
             TRY(cutc_no_point(&a, part + c * bpc, st));
             TRY(cutc_no_orb(&a, st));
@@ -276,10 +286,33 @@ int cutc_no_device(int n_grid1, int n_mo, int ne_a, int ne_b, const double *wr1,
 {
     cutc_internal_lock();
     int rc = no_path(n_grid1, n_mo, ne_a, ne_b, wr1, mos_l_in_r, mos_r_in_r, int2_grad1_u12, no_2e, no_1e, no_0e,
-                     mem_limit, grid_chunk, (hipStream_t)stream, stats);
+                     mem_limit, grid_chunk, (hipStream_t)stream, stats, NULL);
     cutc_internal_unlock();
     cutc_set_last_error(rc);
     return rc;
+}
+
+/* Host entry point: the uploads of int2_grad1_u12 and the build overlap.
+ * The grid is cut into NO_UP_CHUNKS parts; each part of int2_grad1_u12 (a
+ * strided block of the (n_grid1, 3, n_mo, n_mo) array) goes through the
+ * transfer pipeline, and the build starts on a part as soon as it is in
+ * device memory.  no_2e comes back through the same pipeline. */
+#define NO_UP_CHUNKS 8
+
+typedef struct {
+    cutc_xfer *xf;
+    int64_t part; /* grid points per upload part */
+    int64_t tickets[NO_UP_CHUNKS];
+    int nparts;
+} no_upload_t;
+
+static int wait_upload(void *ctx, int64_t g_end)
+{
+    no_upload_t *u = (no_upload_t *)ctx;
+    int64_t k = (g_end + u->part - 1) / u->part - 1;
+    if (k >= u->nparts)
+        k = u->nparts - 1;
+    return k >= 0 ? cutc_xfer_wait(u->xf, u->tickets[k]) : CUTC_OK;
 }
 
 /* Host arrays: upload, build, download (src/cutc_no.c:154-157, 703-704),
@@ -290,6 +323,8 @@ int cutc_no(int n_grid1, int n_mo, int ne_a, int ne_b, double *h_wr1, double *h_
     int rc = CUTC_OK;
     double *d[7] = {NULL, NULL, NULL, NULL, NULL, NULL, NULL};
     cutc_no_stats stt;
+    no_upload_t up;
+    memset(&up, 0, sizeof up);
     if (n_grid1 < 0 || n_mo < 0 || ne_a < 0 || ne_b < 0 || ne_a > n_mo || ne_b > n_mo) {
         cutc_set_last_error(CUTC_ERR_ARG);
         return CUTC_ERR_ARG;
@@ -297,22 +332,60 @@ int cutc_no(int n_grid1, int n_mo, int ne_a, int ne_b, double *h_wr1, double *h_
     printf(" Computing 0e, 1e, and 2e-Elements for Normal-Ordering With CuTC\n");
     const size_t ng = (size_t)n_grid1, nm = (size_t)n_mo, nm2 = nm * nm;
     const size_t sz[7] = {ng, ng * nm, ng * nm, 3 * ng * nm2, nm2 * nm2, nm2, 1};
+    const double *h_in[4] = {h_wr1, h_mos_l_in_r, h_mos_r_in_r, h_int2_grad1_u12};
+    for (int i = 0; i < 4; i++)
+        if (sz[i] > 0 && !h_in[i]) {
+            rc = CUTC_ERR_ARG;
+            goto out;
+        }
     for (int i = 0; i < 7; i++)
         if (sz[i] > 0 && (rc = hrc(hipMalloc((void **)&d[i], 8 * sz[i]))) != CUTC_OK)
             goto out;
-    const double *h_in[4] = {h_wr1, h_mos_l_in_r, h_mos_r_in_r, h_int2_grad1_u12};
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 3; i++)
         if (sz[i] > 0 && (rc = hrc(hipMemcpy(d[i], h_in[i], 8 * sz[i], hipMemcpyHostToDevice))) != CUTC_OK)
             goto out;
-    rc = cutc_no_device(n_grid1, n_mo, ne_a, ne_b, d[0], d[1], d[2], d[3], d[4], d[5], d[6], 0, 0, NULL, &stt);
+    /* the pipeline needs one grid row of a part (8 part bytes) within a staging buffer */
+    const size_t stage = (size_t)64 << 20;
+    int64_t part = (int64_t)((ng + NO_UP_CHUNKS - 1) / NO_UP_CHUNKS);
+    part = (part + 15) / 16 * 16;
+    if (sz[3] > 0 && ng >= 2 * 16 && (size_t)part * 8 <= stage) {
+        if ((rc = cutc_xfer_open(&up.xf, stage, 4, cutc_internal_copy_threads())) != CUTC_OK)
+            goto out;
+        up.part = part;
+        for (int64_t g0 = 0; g0 < (int64_t)ng && up.nparts < NO_UP_CHUNKS; g0 += part) {
+            const int64_t cnt = ((int64_t)ng - g0 < part) ? (int64_t)ng - g0 : part;
+            rc = cutc_xfer_h2d_2d(up.xf, d[3] + g0, 8 * ng, h_int2_grad1_u12 + g0, 8 * ng, 8 * (size_t)cnt, 3 * nm2,
+                                  &up.tickets[up.nparts]);
+            if (rc != CUTC_OK)
+                goto out;
+            up.nparts++;
+        }
+    } else if (sz[3] > 0 && (rc = hrc(hipMemcpy(d[3], h_in[3], 8 * sz[3], hipMemcpyHostToDevice))) != CUTC_OK) {
+        goto out;
+    }
+    {
+        no_hook hook = {wait_upload, &up};
+        cutc_internal_lock();
+        rc = no_path(n_grid1, n_mo, ne_a, ne_b, d[0], d[1], d[2], d[3], d[4], d[5], d[6], 0, up.xf ? up.part : 0,
+                     NULL, &stt, up.xf ? &hook : NULL);
+        cutc_internal_unlock();
+    }
     if (rc == CUTC_OK) {
         double *h_out[3] = {h_no_2e, h_no_1e, h_no_0e};
-        for (int i = 0; i < 3 && rc == CUTC_OK; i++)
+        if (up.xf) { /* no_2e through the pipeline, the small outputs directly */
+            rc = cutc_xfer_d2h(up.xf, h_out[0], d[4], 8 * sz[4], NULL);
+            if (rc == CUTC_OK)
+                rc = cutc_xfer_finish(up.xf, NULL);
+        } else if (sz[4] > 0) {
+            rc = hrc(hipMemcpy(h_out[0], d[4], 8 * sz[4], hipMemcpyDeviceToHost));
+        }
+        for (int i = 1; i < 3 && rc == CUTC_OK; i++)
             if (sz[4 + i] > 0)
                 rc = hrc(hipMemcpy(h_out[i], d[4 + i], 8 * sz[4 + i], hipMemcpyDeviceToHost));
         printf("Ellapsed (effective) time on GPU for cutc_no = %.3f sec\n", stt.ms_total / 1000.0);
     }
 out:
+    cutc_xfer_close(up.xf); /* drains queued copies before the buffers go */
     for (int i = 0; i < 7; i++)
         if (d[i])
             hipFree(d[i]);

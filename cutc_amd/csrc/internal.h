@@ -1,6 +1,6 @@
 /* Internal interface between the host files of the library (defined in
  * cutc_int.c): the library lock, the device workspace arena and the memory
- * budget, and the thread-local last error. */
+ * budget, the thread-local last error and the copy-thread count. */
 #ifndef CUTC_INTERNAL_H
 #define CUTC_INTERNAL_H
 
@@ -14,5 +14,8 @@ void cutc_internal_unlock(void);
 size_t cutc_internal_budget(size_t mem_limit);
 /* make the arena at least need bytes; *base: its start */
 int cutc_internal_workspace(size_t need, char **base);
+/* host copy threads of the transfer pipeline (CUTC_HOST_COPY_THREADS or the
+ * CPU count) */
+int cutc_internal_copy_threads(void);
 
 #endif

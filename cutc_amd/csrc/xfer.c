@@ -1,5 +1,5 @@
 /*
- * Device -> host transfer pipeline of the host entry points.
+ * Host <-> device transfer pipeline of the host entry points.
  *
  * The reference copies int2_grad1_u12_ao and int_2e_ao back with blocking
  * cudaMemcpy calls after the whole build (src/cutc_int.c:210-211).  Here each
@@ -10,7 +10,10 @@
  * the caller's (pageable) array, which a small pool of threads performs in
  * parallel.  The build keeps running on its own stream meanwhile, so the
  * transfers of finished r1 chunks hide behind the contractions of later
- * ones.
+ * ones.  Uploads go the other way through the same ring (host threads
+ * gather rows into a staging buffer while the DMA of the previous one runs)
+ * and are waited for by ticket, so a build can start on the first part of
+ * an input while the rest is still on its way.
  */
 #include <pthread.h>
 #include <stdlib.h>
@@ -30,6 +33,11 @@ typedef struct job {
     const char *src;
     size_t bytes;
     hipEvent_t ready;
+    /* host -> device jobs: rows x width bytes, row pitches dpitch (device)
+     * and spitch (host); ticket = their sequence number */
+    int h2d;
+    size_t dpitch, spitch, width, rows;
+    int64_t ticket;
     struct job *next;
 } job_t;
 
@@ -41,11 +49,13 @@ struct cutc_xfer {
     size_t stage_bytes;
 
     pthread_mutex_t mu;
-    pthread_cond_t cv;
+    pthread_cond_t cv, cv_h2d;
     job_t *head, *tail;
     int closing;
     int err;
     size_t bytes_done;
+    int64_t h2d_issued, h2d_done;
+    int driver_exited;
     pthread_t driver;
     int driver_on;
 
@@ -59,7 +69,8 @@ struct cutc_xfer {
     int cp_quit;
     char *cp_dst;
     const char *cp_src;
-    size_t cp_bytes;
+    size_t cp_bytes;  /* one row: bytes; several rows: row width */
+    size_t cp_rows, cp_dpitch, cp_spitch;
 };
 
 static int hrc(hipError_t e)
@@ -69,18 +80,28 @@ static int hrc(hipError_t e)
     return e == hipErrorOutOfMemory ? CUTC_ERR_NOMEM : CUTC_ERR_HIP;
 }
 
-/* share w of nthr + 1 of the current host copy (64-byte aligned cuts) */
+/* share w of nthr + 1 of the current host copy: one row is cut into
+ * 64-byte aligned pieces, several rows are dealt out by row */
 static void copy_share(cutc_xfer *x, int w)
 {
     const int parts = x->nthr + 1;
-    size_t per = (x->cp_bytes / parts + 63) & ~(size_t)63;
+    if (x->cp_rows <= 1) {
+        size_t per = (x->cp_bytes / parts + 63) & ~(size_t)63;
+        size_t a = per * (size_t)w, b = a + per;
+        if (a > x->cp_bytes)
+            a = x->cp_bytes;
+        if (b > x->cp_bytes)
+            b = x->cp_bytes;
+        if (b > a)
+            memcpy(x->cp_dst + a, x->cp_src + a, b - a);
+        return;
+    }
+    const size_t per = (x->cp_rows + parts - 1) / parts;
     size_t a = per * (size_t)w, b = a + per;
-    if (a > x->cp_bytes)
-        a = x->cp_bytes;
-    if (b > x->cp_bytes)
-        b = x->cp_bytes;
-    if (b > a)
-        memcpy(x->cp_dst + a, x->cp_src + a, b - a);
+    if (b > x->cp_rows)
+        b = x->cp_rows;
+    for (size_t r = a; r < b; r++)
+        memcpy(x->cp_dst + r * x->cp_dpitch, x->cp_src + r * x->cp_spitch, x->cp_bytes);
 }
 
 typedef struct {
@@ -112,16 +133,23 @@ static void *pool_main(void *arg)
     return NULL;
 }
 
-static void host_copy(cutc_xfer *x, char *dst, const char *src, size_t bytes)
+/* rows x width bytes from src (row pitch spitch) to dst (row pitch dpitch),
+ * shared with the pool when large */
+static void host_copy2d(cutc_xfer *x, char *dst, size_t dpitch, const char *src, size_t spitch, size_t width,
+                        size_t rows)
 {
-    if (x->nthr == 0 || bytes < ((size_t)4 << 20)) {
-        memcpy(dst, src, bytes);
+    if (x->nthr == 0 || width * rows < ((size_t)4 << 20)) {
+        for (size_t r = 0; r < rows; r++)
+            memcpy(dst + r * dpitch, src + r * spitch, width);
         return;
     }
     pthread_mutex_lock(&x->pm);
     x->cp_dst = dst;
     x->cp_src = src;
-    x->cp_bytes = bytes;
+    x->cp_bytes = width;
+    x->cp_rows = rows;
+    x->cp_dpitch = dpitch;
+    x->cp_spitch = spitch;
     x->remaining = x->nthr;
     x->gen++;
     pthread_cond_broadcast(&x->pgo);
@@ -144,7 +172,34 @@ static void set_err(cutc_xfer *x, int rc)
     pthread_mutex_lock(&x->mu);
     if (!x->err)
         x->err = rc;
+    pthread_cond_broadcast(&x->cv_h2d);
     pthread_mutex_unlock(&x->mu);
+}
+
+/* One host -> device job, run when no download piece is in flight: the rows
+ * are gathered into the staging ring (host threads) while the DMAs of the
+ * previous pieces run; returns once every piece is in device memory. */
+static int run_h2d(cutc_xfer *x, const job_t *j)
+{
+    const size_t per = x->stage_bytes / j->width;
+    size_t r0 = 0;
+    int64_t issued = 0;
+    int rc = CUTC_OK;
+    while (r0 < j->rows && rc == CUTC_OK) {
+        const int b = (int)(issued % x->nstage);
+        const size_t nr = (j->rows - r0 < per) ? j->rows - r0 : per;
+        if (issued >= x->nstage && (rc = hrc(hipEventSynchronize(x->done[b]))) != CUTC_OK)
+            break;
+        host_copy2d(x, x->stage[b], j->width, j->src + r0 * j->spitch, j->spitch, j->width, nr);
+        rc = hrc(hipMemcpy2DAsync(j->dst + r0 * j->dpitch, j->dpitch, x->stage[b], j->width, j->width, nr,
+                                  hipMemcpyHostToDevice, x->cs));
+        if (rc == CUTC_OK)
+            rc = hrc(hipEventRecord(x->done[b], x->cs));
+        issued++;
+        r0 += nr;
+    }
+    const int rs = hrc(hipStreamSynchronize(x->cs));
+    return rc != CUTC_OK ? rc : rs;
 }
 
 static void *driver_main(void *arg)
@@ -160,6 +215,8 @@ static void *driver_main(void *arg)
             if (!cur) {
                 pthread_mutex_lock(&x->mu);
                 cur = x->head;
+                if (cur && cur->h2d && inflight > 0)
+                    cur = NULL; /* an upload waits for the downloads in flight */
                 if (cur) {
                     x->head = cur->next;
                     if (!x->head)
@@ -169,6 +226,20 @@ static void *driver_main(void *arg)
                 if (!cur)
                     break;
                 off = 0;
+                if (cur->h2d) {
+                    err = run_h2d(x, cur);
+                    pthread_mutex_lock(&x->mu);
+                    if (err == CUTC_OK)
+                        x->h2d_done = cur->ticket;
+                    pthread_cond_broadcast(&x->cv_h2d);
+                    pthread_mutex_unlock(&x->mu);
+                    cur->next = retired;
+                    retired = cur;
+                    cur = NULL;
+                    if (err)
+                        break;
+                    continue;
+                }
                 if ((err = hrc(hipStreamWaitEvent(x->cs, cur->ready, 0))) != CUTC_OK)
                     break;
             }
@@ -207,7 +278,7 @@ static void *driver_main(void *arg)
             set_err(x, err);
             break;
         }
-        host_copy(x, fl[first].dst, x->stage[first], fl[first].bytes);
+        host_copy2d(x, fl[first].dst, 0, x->stage[first], 0, fl[first].bytes, 1);
         pthread_mutex_lock(&x->mu);
         x->bytes_done += fl[first].bytes;
         pthread_mutex_unlock(&x->mu);
@@ -222,9 +293,14 @@ static void *driver_main(void *arg)
     while (retired) {
         job_t *j = retired;
         retired = j->next;
-        hipEventDestroy(j->ready);
+        if (j->ready)
+            hipEventDestroy(j->ready);
         free(j);
     }
+    pthread_mutex_lock(&x->mu);
+    x->driver_exited = 1;
+    pthread_cond_broadcast(&x->cv_h2d);
+    pthread_mutex_unlock(&x->mu);
     return NULL;
 }
 
@@ -244,6 +320,7 @@ int cutc_xfer_open(cutc_xfer **out, size_t stage_bytes, int nstage, int nthreads
         return CUTC_ERR_NOMEM;
     pthread_mutex_init(&x->mu, NULL);
     pthread_cond_init(&x->cv, NULL);
+    pthread_cond_init(&x->cv_h2d, NULL);
     x->nstage = nstage;
     x->stage_bytes = stage_bytes;
     int rc = hrc(hipStreamCreateWithFlags(&x->cs, hipStreamNonBlocking));
@@ -312,6 +389,47 @@ int cutc_xfer_d2h(cutc_xfer *x, void *h_dst, const void *d_src, size_t bytes, hi
     return CUTC_OK;
 }
 
+int cutc_xfer_h2d_2d(cutc_xfer *x, void *d_dst, size_t dpitch, const void *h_src, size_t spitch, size_t width,
+                     size_t rows, int64_t *ticket)
+{
+    *ticket = 0;
+    if (width == 0 || rows == 0)
+        return CUTC_OK;
+    if (width > x->stage_bytes || (rows > 1 && (dpitch < width || spitch < width)))
+        return CUTC_ERR_ARG;
+    job_t *j = (job_t *)calloc(1, sizeof *j);
+    if (!j)
+        return CUTC_ERR_NOMEM;
+    j->h2d = 1;
+    j->dst = (char *)d_dst;
+    j->src = (const char *)h_src;
+    j->dpitch = dpitch;
+    j->spitch = spitch;
+    j->width = width;
+    j->rows = rows;
+    pthread_mutex_lock(&x->mu);
+    j->ticket = ++x->h2d_issued;
+    *ticket = j->ticket;
+    if (x->tail)
+        x->tail->next = j;
+    else
+        x->head = j;
+    x->tail = j;
+    pthread_cond_signal(&x->cv);
+    pthread_mutex_unlock(&x->mu);
+    return CUTC_OK;
+}
+
+int cutc_xfer_wait(cutc_xfer *x, int64_t ticket)
+{
+    pthread_mutex_lock(&x->mu);
+    while (x->h2d_done < ticket && !x->err && !x->driver_exited)
+        pthread_cond_wait(&x->cv_h2d, &x->mu);
+    const int rc = (x->h2d_done >= ticket) ? CUTC_OK : (x->err ? x->err : CUTC_ERR_HIP);
+    pthread_mutex_unlock(&x->mu);
+    return rc;
+}
+
 int cutc_xfer_finish(cutc_xfer *x, size_t *bytes_done)
 {
     pthread_mutex_lock(&x->mu);
@@ -335,7 +453,8 @@ void cutc_xfer_close(cutc_xfer *x)
     while (x->head) { /* jobs left after an error */
         job_t *j = x->head;
         x->head = j->next;
-        hipEventDestroy(j->ready);
+        if (j->ready)
+            hipEventDestroy(j->ready);
         free(j);
     }
     pthread_mutex_lock(&x->pm);
@@ -357,5 +476,6 @@ void cutc_xfer_close(cutc_xfer *x)
         hipStreamDestroy(x->cs);
     pthread_mutex_destroy(&x->mu);
     pthread_cond_destroy(&x->cv);
+    pthread_cond_destroy(&x->cv_h2d);
     free(x);
 }

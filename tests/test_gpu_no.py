@@ -82,6 +82,18 @@ def test_no_host_entry_point(torch):
     check(got, want)
 
 
+def test_no_host_entry_pipelined_upload(torch):
+    """cutc_no on host arrays uploads int2_grad1_u12 in 8 grid parts through
+    the transfer pipeline (here 4 staging pieces per part) while the build
+    runs on the parts already in place; same result as the device entry point
+    on the whole array"""
+    print("cutc_no host entry, pipelined upload", flush=True)
+    args = inputs(20000, 60, 31)
+    got = cutc.cutc_no(8, 6, *args)
+    want = device_run(torch, 8, 6, *args)
+    check(got, want[:3])
+
+
 def test_no_edge_cases(torch):
     print("cutc_no edge cases", flush=True)
     w, L, R, G = inputs(50, 5, 9)
