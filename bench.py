@@ -212,6 +212,21 @@ def batch_bench(args, torch, cutc, synth):
     print(json.dumps(out), flush=True)
 
 
+class stdout_to_stderr:
+    """Send what C code prints to stdout (the reference entry points print
+    their progress lines) to stderr, so that stdout carries only the JSON
+    line."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
 def no_bench(args, torch, cutc):
     """Normal ordering (cutc_no_device) on synthetic MO-basis inputs resident in
     HBM: int2_grad1_u12 (n_grid1, 3, n_mo, n_mo), orbitals on the grid, weights.
@@ -268,9 +283,10 @@ def no_bench(args, torch, cutc):
         torch.cuda.empty_cache()
         hp = {}
         for mode in ("first_call", "repeated"):
-            t0 = time.perf_counter()
-            r = cutc.cutc_no(na, nb, hw, hL, hR, hG)
-            wall = time.perf_counter() - t0
+            with stdout_to_stderr():
+                t0 = time.perf_counter()
+                r = cutc.cutc_no(na, nb, hw, hL, hR, hG)
+                wall = time.perf_counter() - t0
             del r
             hp[mode] = {"wall_s": wall, "eff_tflops": W / wall / 1e12}
             log(args, f"cutc_no on host arrays ({mode}): {wall:.2f} s")
