@@ -366,7 +366,29 @@ def mo_bench(args, torch, cutc):
                            "traffic": MEASURED_TRAFFIC["mo"] if (n1, na, nm) == (60000, 200, 200) else None,
                            "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), "
                                            "profiles/ao_to_mo_nao200_kernel_stats.txt"}}
-    del X, out, cl, cr
+    del out
+    if args.host_path:
+        # the host-array entry point: uploads, transform, download
+        hX, hcl, hcr = X.cpu().numpy(), cl.cpu().numpy(), cr.cpu().numpy()
+        del X
+        cutc.release_workspace()
+        torch.cuda.empty_cache()
+        hp = {}
+        for mode in ("first_call", "repeated"):
+            with stdout_to_stderr():
+                t0 = time.perf_counter()
+                r = cutc.cutc_ao_to_mo(hcl, hcr, hX)
+                wall = time.perf_counter() - t0
+            del r
+            hp[mode] = {"wall_s": wall}
+            log(args, f"cutc_ao_to_mo on host arrays ({mode}): {wall:.2f} s")
+        hp["note"] = ("cutc_ao_to_mo on host arrays: 57.6 GB uploaded, 57.6 GB downloaded (pageable host memory) "
+                      "through the transfer pipeline")
+        out_js["host_path"] = hp
+        del hX
+    else:
+        del X
+    del cl, cr
     cutc.release_workspace()
     torch.cuda.empty_cache()
     out_js["cpu_baseline"] = None
@@ -441,6 +463,19 @@ def ao_bench(args, torch, cutc, synth):
                                            "profiles/aos_cube20_kernel_stats.txt"},
               "h2d_pageable_ms_same_bytes": h2d * 1e3}
     log(args, f"aos: {ms:.2f} ms, {out_js['value']:.0f} GB/s; upload of the same bytes {h2d * 1e3:.0f} ms")
+    if args.host_path:
+        rr = np.concatenate([s.r1, s.r2])
+        hp = {}
+        for mode in ("first_call", "repeated"):
+            with stdout_to_stderr():
+                t0 = time.perf_counter()
+                r_ = cutc.cutc_aos(rr, s.rn, b)
+                wall = time.perf_counter() - t0
+            del r_
+            hp[mode] = {"wall_s": wall}
+            log(args, f"cutc_aos on host arrays ({mode}): {wall * 1e3:.0f} ms")
+        hp["note"] = "cutc_aos on host arrays: the 1.664 GB of aos_data downloaded through the transfer pipeline"
+        out_js["host_path"] = hp
     del out, r
     torch.cuda.empty_cache()
     out_js["cpu_baseline"] = None

@@ -13,6 +13,7 @@
 #include "cutc_ao.h"
 #include "internal.h"
 #include "kernels/launch.h"
+#include "xfer.h"
 
 static int hrc(hipError_t e)
 {
@@ -57,12 +58,14 @@ int cutc_aos_device(int n_grid, int n_nuc, int n_ao, int prim_max, const double 
     return rc;
 }
 
-/* Host arrays: check the tables, upload, evaluate, download, free. */
+/* Host arrays: check the tables, upload, evaluate, download (through the
+ * transfer pipeline), free. */
 int cutc_aos(int n_grid, int n_nuc, int n_ao, int prim_max, double *h_r, double *h_rn, int *h_ao_nucl,
              int *h_ao_power, int *h_ao_prim_num, double *h_ao_expo, double *h_ao_coef, double *h_aos_data)
 {
     int rc = CUTC_OK;
     void *d[8] = {NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+    cutc_xfer *xf = NULL;
     if (n_grid < 0 || n_nuc < 0 || n_ao < 0 || prim_max < 0 || n_ao > 65535) {
         rc = CUTC_ERR_ARG;
         goto out;
@@ -90,10 +93,16 @@ int cutc_aos(int n_grid, int n_nuc, int n_ao, int prim_max, double *h_r, double 
         rc = cutc_launch_aos(n_grid, (const double *)d[0], (const double *)d[1], n_nuc, n_ao, (const int *)d[2],
                              (const int *)d[3], (const int *)d[4], prim_max, (const double *)d[5],
                              (const double *)d[6], (double *)d[7], NULL);
-        if (rc == CUTC_OK)
-            rc = hrc(hipMemcpy(h_aos_data, d[7], sz[7], hipMemcpyDeviceToHost));
+        if (rc == CUTC_OK) { /* the output through the transfer pipeline */
+            rc = cutc_xfer_open(&xf, (size_t)64 << 20, 4, cutc_internal_copy_threads());
+            if (rc == CUTC_OK)
+                rc = cutc_xfer_d2h(xf, h_aos_data, d[7], sz[7], NULL);
+            if (rc == CUTC_OK)
+                rc = cutc_xfer_finish(xf, NULL);
+        }
     }
 out:
+    cutc_xfer_close(xf);
     for (int i = 0; i < 8; i++)
         if (d[i])
             hipFree(d[i]);

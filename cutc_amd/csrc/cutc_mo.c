@@ -28,6 +28,7 @@
 #include "cutc_mo.h"
 #include "internal.h"
 #include "kernels/launch.h"
+#include "xfer.h"
 
 static int hrc(hipError_t e)
 {
@@ -150,12 +151,16 @@ int cutc_ao_to_mo_device(int n_grid1, int n_ao, int n_mo, const double *mo_l_coe
     return rc;
 }
 
-/* Host arrays: upload, transform, download, free every device buffer. */
+/* Host arrays: upload, transform, download, free every device buffer.  The
+ * two big arrays move through the library's transfer pipeline (pinned
+ * staging, host copy threads), which is several times faster than copies
+ * from and to pageable memory; the transform itself is short next to them. */
 int cutc_ao_to_mo(int n_grid1, int n_ao, int n_mo, double *h_mo_l_coef, double *h_mo_r_coef,
                   double *h_int2_grad1_u12_ao, double *h_int2_grad1_u12)
 {
     int rc = CUTC_OK;
     double *d[4] = {NULL, NULL, NULL, NULL};
+    cutc_xfer *xf = NULL;
     if (n_grid1 < 0 || n_ao < 0 || n_mo < 0) {
         cutc_set_last_error(CUTC_ERR_ARG);
         return CUTC_ERR_ARG;
@@ -171,13 +176,25 @@ int cutc_ao_to_mo(int n_grid1, int n_ao, int n_mo, double *h_mo_l_coef, double *
     for (int i = 0; i < 4; i++)
         if (sz[i] > 0 && (rc = hrc(hipMalloc((void **)&d[i], 8 * sz[i]))) != CUTC_OK)
             goto out;
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 2; i++)
         if (sz[i] > 0 && (rc = hrc(hipMemcpy(d[i], h_in[i], 8 * sz[i], hipMemcpyHostToDevice))) != CUTC_OK)
             goto out;
+    if (sz[2] > 0 || sz[3] > 0) {
+        int64_t t = 0;
+        if ((rc = cutc_xfer_open(&xf, (size_t)64 << 20, 4, cutc_internal_copy_threads())) != CUTC_OK)
+            goto out;
+        if (sz[2] > 0 && ((rc = cutc_xfer_h2d(xf, d[2], h_in[2], 8 * sz[2], &t)) != CUTC_OK ||
+                          (rc = cutc_xfer_wait(xf, t)) != CUTC_OK))
+            goto out;
+    }
     rc = cutc_ao_to_mo_device(n_grid1, n_ao, n_mo, d[0], d[1], d[2], d[3], 0, 0, NULL, NULL);
-    if (rc == CUTC_OK && sz[3] > 0)
-        rc = hrc(hipMemcpy(h_int2_grad1_u12, d[3], 8 * sz[3], hipMemcpyDeviceToHost));
+    if (rc == CUTC_OK && sz[3] > 0) {
+        rc = cutc_xfer_d2h(xf, h_int2_grad1_u12, d[3], 8 * sz[3], NULL);
+        if (rc == CUTC_OK)
+            rc = cutc_xfer_finish(xf, NULL);
+    }
 out:
+    cutc_xfer_close(xf);
     for (int i = 0; i < 4; i++)
         if (d[i])
             hipFree(d[i]);

@@ -420,6 +420,24 @@ int cutc_xfer_h2d_2d(cutc_xfer *x, void *d_dst, size_t dpitch, const void *h_src
     return CUTC_OK;
 }
 
+int cutc_xfer_h2d(cutc_xfer *x, void *d_dst, const void *h_src, size_t bytes, int64_t *ticket)
+{
+    /* whole rows of 1 MiB, then the rest as one short row */
+    const size_t w = (size_t)1 << 20;
+    const size_t rows = bytes / w, rest = bytes - rows * w;
+    int64_t t = 0;
+    int rc = CUTC_OK;
+    *ticket = 0;
+    if (rows > 0 && (rc = cutc_xfer_h2d_2d(x, d_dst, w, h_src, w, w, rows, &t)) != CUTC_OK)
+        return rc;
+    *ticket = t;
+    if (rest > 0 && (rc = cutc_xfer_h2d_2d(x, (char *)d_dst + rows * w, rest, (const char *)h_src + rows * w, rest,
+                                           rest, 1, &t)) != CUTC_OK)
+        return rc;
+    *ticket = t;
+    return CUTC_OK;
+}
+
 int cutc_xfer_wait(cutc_xfer *x, int64_t ticket)
 {
     pthread_mutex_lock(&x->mu);

@@ -23,6 +23,8 @@ int cutc_xfer_d2h(cutc_xfer *x, void *h_dst, const void *d_src, size_t bytes, hi
  * numbers it for cutc_xfer_wait (0: nothing to copy). */
 int cutc_xfer_h2d_2d(cutc_xfer *x, void *d_dst, size_t dpitch, const void *h_src, size_t spitch, size_t width,
                      size_t rows, int64_t *ticket);
+/* queue a contiguous upload of bytes (as 1 MiB rows plus a short one) */
+int cutc_xfer_h2d(cutc_xfer *x, void *d_dst, const void *h_src, size_t bytes, int64_t *ticket);
 /* block until the upload with this ticket and every earlier one are in
  * device memory; returns the first error */
 int cutc_xfer_wait(cutc_xfer *x, int64_t ticket);
