@@ -560,10 +560,14 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     if (!slice)
         TRY(cutc_crt_setup(n2, nmod, crt_kind(opt->precision), &crt));
     /* modular: residue planes (int8, or 16-bit for bf16 / fp16) and int8
-     * results; slices: 16-bit digit planes and FP64 results */
-    const size_t bA = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * ldk : rb * nmod * CUTC_PK_ROWS_B(P) * ldk;
+     * results; slices: 16-bit digit planes and FP64 results.  The modular
+     * planes hold one K pass of kc r2 points (kc = ldk: a single pass); the
+     * GEMM of each further pass adds its residues into the results. */
+    const int64_t kc = (slice || pl->k_chunk <= 0 || pl->k_chunk > ldk) ? ldk : pl->k_chunk;
+    const int64_t kpass = (n2 + kc - 1) / kc;
+    const size_t bA = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * ldk : rb * nmod * CUTC_PK_ROWS_B(P) * kc;
     const size_t bG = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(4 * n1c) * ldk
-                            : rb * nmod * CUTC_PK_ROWS_A(4 * n1c) * ldk;
+                            : rb * nmod * CUTC_PK_ROWS_A(4 * n1c) * kc;
     const size_t bC = slice ? sizeof(double) * 4 * n1c * P : (size_t)nmod * 4 * n1c * P;
 
     ws_t ws;
@@ -591,7 +595,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_SPLIT);
     rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpA, st);
-    if (rc == CUTC_OK)
+    if (rc == CUTC_OK && kpass == 1)
         rc = slice ? cutc_oz_slice_pairw(nmod, f16, wr2, aos2, n2, n_ao, ldk, CUTC_OZ_ROWS(P), rexpA, (uint16_t *)Apl,
                                          st)
                    : cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rexpA, Apl, st);
@@ -611,18 +615,35 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
         rc = cutc_rowexp_k(G, ldk, 4 * n1e, n2, emode, rexpG, st);
-        if (rc == CUTC_OK)
+        if (rc == CUTC_OK && kpass == 1)
             rc = slice ? cutc_oz_slice_k(nmod, f16, G, ldk, 4 * n1e, n2, ldk, CUTC_OZ_ROWS(4 * n1e), rexpG,
                                          (uint16_t *)Gpl, st)
                        : cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rexpG, Gpl, st);
         tr_end(tr, ev);
         TRY(rc);
-        ev = tr_begin(tr, CAT_GEMM1);
-        rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
-                                  rexpA, (double *)Cpl, st)
-                   : cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, ldk, Cpl, st);
-        tr_end(tr, ev);
-        TRY(rc);
+        if (kpass == 1) {
+            ev = tr_begin(tr, CAT_GEMM1);
+            rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
+                                      rexpA, (double *)Cpl, st)
+                       : cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, ldk, Cpl, 0, st);
+            tr_end(tr, ev);
+            TRY(rc);
+        }
+        /* several K passes (modular only): the residue planes of both
+         * operands for r2 points [k0, k0 + kn), products added into Cpl */
+        for (int64_t kp = 0; kp < kpass && kpass > 1; kp++) {
+            const int64_t k0 = kp * kc, kn = (n2 - k0 < kc) ? n2 - k0 : kc;
+            ev = tr_begin(tr, CAT_SPLIT);
+            rc = cutc_crt_quant_pairw_pk_range(&crt, wr2 + k0, aos2 + k0, n2, kn, n_ao, kc, rexpA, Apl, st);
+            if (rc == CUTC_OK)
+                rc = cutc_crt_quant_k_pk(&crt, G + k0, ldk, 4 * n1e, kn, kc, rexpG, Gpl, st);
+            tr_end(tr, ev);
+            TRY(rc);
+            ev = tr_begin(tr, CAT_GEMM1);
+            rc = cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, kc, Cpl, kp > 0, st);
+            tr_end(tr, ev);
+            TRY(rc);
+        }
         ev = tr_begin(tr, CAT_SPLIT);
         rc = slice ? cutc_oz_scatter_x((const double *)Cpl, P, 4 * n1e, pairs, n_ao, X, n1, n1e, i0, st)
                    : cutc_crt_rebuild_x(&crt, Cpl, P, 4 * n1e, rexpA, rexpG, pairs, n_ao, X, n1, n1e, i0, st);
@@ -883,7 +904,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 ev = tr_begin(tr, CAT_GEMM2);
                 rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Bpl, P, (const uint16_t *)Apl, P, nseg * kp, rexpB,
                                           rexpA, (double *)Cpl, st)
-                           : cutc_crt_gemm_pk(&crt, Bpl, P, Apl, P, nseg * kp, Cpl, st);
+                           : cutc_crt_gemm_pk(&crt, Bpl, P, Apl, P, nseg * kp, Cpl, 0, st);
                 tr_end(tr, ev);
                 TRY(rc);
                 ev = tr_begin(tr, CAT_SPLIT);

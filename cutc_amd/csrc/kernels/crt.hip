@@ -343,8 +343,8 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
 // pair rows p = (k, l) of A[p][r2] = (w chi_l) chi_k, computed on the fly
 template <int KIND>
 __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const double *__restrict__ wr2,
-                                                             const double *__restrict__ aos2, int64_t n2, int n_ao,
-                                                             int64_t KT, int64_t plane,
+                                                             const double *__restrict__ aos2, int64_t ld2,
+                                                             int64_t n2, int n_ao, int64_t KT, int64_t plane,
                                                              const int *__restrict__ rexp, int8_t *planes)
 {
     const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int64_t i = k0 + e;
-            q[e] = (i < n2) ? rint(ldexp(wr2[i] * aos2[i + n2 * l] * aos2[i + n2 * k], sh)) : 0.0;
+            q[e] = (i < n2) ? rint(ldexp(wr2[i] * aos2[i + ld2 * l] * aos2[i + ld2 * k], sh)) : 0.0;
         }
     } else {
 #pragma unroll
@@ -372,6 +372,15 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
 extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2,
                                        int n_ao, int64_t kpad, const int *rexp, int8_t *planes, hipStream_t st)
 {
+    return cutc_crt_quant_pairw_pk_range(c, wr2, aos2, n2, n2, n_ao, kpad, rexp, planes, st);
+}
+
+/* r2 points [0, n2) of wr2 / aos2 (whose AO columns are ld2 apart): the
+ * planes of one K pass when the caller offsets wr2 and aos2 to its start */
+extern "C" int cutc_crt_quant_pairw_pk_range(const cutc_crt *c, const double *wr2, const double *aos2, int64_t ld2,
+                                             int64_t n2, int n_ao, int64_t kpad, const int *rexp, int8_t *planes,
+                                             hipStream_t st)
+{
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     if (P <= 0 || kpad <= 0)
         return 0;
@@ -380,7 +389,7 @@ extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, con
     const int64_t plane = CUTC_PK_ROWS_B(P) * kpad * CUTC_CRT_RBYTES(c->kind);
     const dim3 grid((unsigned)(kpad / 128), (unsigned)((P + 31) / 32));
 #define QP(K_)                                                                                                         \
-    hipLaunchKernelGGL(quant_pairw_pk_kernel<K_>, grid, dim3(256), 0, st, *c, wr2, aos2, n2, n_ao,                     \
+    hipLaunchKernelGGL(quant_pairw_pk_kernel<K_>, grid, dim3(256), 0, st, *c, wr2, aos2, ld2, n2, n_ao,                \
                        pk_steps(K_, kpad), plane, rexp, planes)
     CRT_KIND_DISPATCH(c->kind, QP);
 #undef QP
@@ -486,7 +495,7 @@ template <int D, int L, int BAR, int SEG>
 __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__restrict__ A,
                                                              const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                              int tiles_b, int64_t M, int64_t N, int mod,
-                                                             double inv_mod, int8_t *__restrict__ C)
+                                                             double inv_mod, int8_t *__restrict__ C, int accum)
 {
     static_assert(SEG % D == 0 && D % BAR == 0 && (int64_t)SEG * 64 * 16384 < 2147483648LL, "segment");
     static_assert(D >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
@@ -613,7 +622,8 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
             for (int r = 0; r < 16; r++) {
                 const int64_t row = (int64_t)ta * 512 + wave * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
                 if (row < M)
-                    C[row * N + col] = (int8_t)reduce_mod(acc[i][j][r], mod, inv_mod);
+                    C[row * N + col] =
+                        (int8_t)reduce_mod(acc[i][j][r] + (accum ? (int)C[row * N + col] : 0), mod, inv_mod);
             }
         }
     }
@@ -650,7 +660,7 @@ template <bool F16, int D, int L, int BAR, int RW>
 __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__restrict__ A,
                                                             const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                             int tiles_b, int64_t M, int64_t N, int mod,
-                                                            double inv_mod, int8_t *__restrict__ C)
+                                                            double inv_mod, int8_t *__restrict__ C, int accum)
 {
     static_assert(D * RW * 32 * 127 * 127 + 383 < (1 << 24), "exact FP32 sums");
     static_assert(D >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
@@ -765,14 +775,15 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
             for (int r = 0; r < 16; r++) {
                 const int64_t row = (int64_t)ta * 512 + wave * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
                 if (row < M)
-                    C[row * N + col] = (int8_t)reduce_mod((int)fmod_small(acc[i][j][r], mf, imf), mod, inv_mod);
+                    C[row * N + col] = (int8_t)reduce_mod(
+                        (int)fmod_small(acc[i][j][r], mf, imf) + (accum ? (int)C[row * N + col] : 0), mod, inv_mod);
             }
         }
     }
 }
 
 extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
-                                int64_t kpad, int8_t *C, hipStream_t st)
+                                int64_t kpad, int8_t *C, int accumulate, hipStream_t st)
 {
     if (M <= 0 || N <= 0)
         return 0;
@@ -792,13 +803,13 @@ extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, c
         int8_t *Ci = C + (int64_t)i * M * N;
         if (c->kind == CUTC_CRT_I8)
             hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040>), grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT,
-                               (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci);
+                               (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
         else if (c->kind == CUTC_CRT_BF16)
             hipLaunchKernelGGL((crt16_gemm_kernel<false, 6, 4, 2, 5>), grid, dim3(256), 0, st, A + i * pa, B + i * pb,
-                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci);
+                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
         else
             hipLaunchKernelGGL((crt16_gemm_kernel<true, 6, 4, 2, 5>), grid, dim3(256), 0, st, A + i * pa, B + i * pb,
-                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci);
+                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
     }
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -166,6 +166,10 @@ int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t 
                         const int *rexp, int8_t *planes, hipStream_t st);
 int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
                             int64_t kpad, const int *rexp, int8_t *planes, hipStream_t st);
+/* the same for r2 points [0, n2) of arrays whose AO columns are ld2 apart
+ * (one K pass: wr2 and aos2 offset to its first point) */
+int cutc_crt_quant_pairw_pk_range(const cutc_crt *c, const double *wr2, const double *aos2, int64_t ld2, int64_t n2,
+                                  int n_ao, int64_t kpad, const int *rexp, int8_t *planes, hipStream_t st);
 /* column-major sources into packed planes of plane_rows rows (CUTC_PK_ROWS_A or _B of rows, by the operand's side); the
  * K of segment s starts at s kp, kp a multiple of 128 and nsrc kp one of
  * CUTC_PK_KALIGN. */
@@ -173,9 +177,10 @@ int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t ro
                            const int *rexp, int8_t *planes, hipStream_t st);
 /* C[i][r][p] = (sum_k A[i][r][k] B[i][p][k]) mod m_i as int8 in
  * [-(m_i / 2), (m_i - 1) / 2] for r < M, p < N, packed planes as above; C
- * holds n M N bytes (row r of the A side is contiguous over p). */
+ * holds n M N bytes (row r of the A side is contiguous over p).
+ * accumulate: add the residues already in C (one K pass of several). */
 int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
-                     int8_t *C, hipStream_t st);
+                     int8_t *C, int accumulate, hipStream_t st);
 /* Slice (Ozaki scheme I) contractions, ozaki1.hip: S = 2..4 digit planes of
  * 16-bit elements (bf16 or fp16 bit patterns, f16 != 0), rows padded to
  * CUTC_OZ_ROWS, kpad a multiple of CUTC_PK_KALIGN.  Row exponents from

@@ -14,7 +14,10 @@
  *                 n_pairs doubles) [INT8: + both operands' packed residues,
  *                 2 x nmod x 3 x n_pairs bytes per r1 row, + the GEMM residues
  *                 nmod x n_pairs^2 bytes]
- * Chunks are balanced (equal sizes rounded up to a multiple of 64). */
+ * Chunks are balanced (equal sizes rounded up to a multiple of 64).  The
+ * modular precisions may also cut K (r2) into passes of k_chunk points whose
+ * residue products are added modulo each modulus, when the pair planes of the
+ * whole K leave room for too few r1 rows. */
 #include <stdint.h>
 
 #include "cutc_int.h"
@@ -50,9 +53,22 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
 
     /* stage 1 */
     int64_t fixed1, per_r1;
+    int64_t kc = ldk;
     if (crt) {
-        fixed1 = 16 * P + rb * nmod * CUTC_PK_ROWS_B(P) * ldk + rb * nmod * 511 * ldk + 4096;
-        per_r1 = 32 * ldk + 32 + 4 * rb * nmod * ldk + nmod * 4 * P;
+        /* the residue planes of one K pass of kc points; more passes when the
+         * pair planes leave room for fewer than min(n1, 1024) r1 rows */
+        const int64_t want = n1 < 1024 ? (n1 > 0 ? n1 : 1) : 1024;
+        for (int64_t nk = 1; nk <= 64; nk++) {
+            kc = rup(cdiv(ldk, nk), CUTC_PK_KALIGN);
+            fixed1 = 16 * P + rb * nmod * CUTC_PK_ROWS_B(P) * kc + rb * nmod * 511 * kc + 4096;
+            per_r1 = 32 * ldk + 32 + 4 * rb * nmod * kc + nmod * 4 * P;
+            if (n_nuc > 32)
+                fixed1 += 8 * (int64_t)n_nuc * ldk;
+            if ((int64_t)budget >= fixed1 + per_r1 * want || kc == CUTC_PK_KALIGN)
+                break;
+        }
+        if (n_nuc > 32)
+            fixed1 -= 8 * (int64_t)n_nuc * ldk; /* added once below */
     } else if (slice) { /* digit planes of 2-byte elements, FP64 results */
         fixed1 = 16 * P + 2 * nmod * CUTC_OZ_ROWS(P) * ldk + 2 * nmod * 127 * ldk + 4096;
         per_r1 = 32 * ldk + 32 + 8 * nmod * ldk + 32 * P;
@@ -76,6 +92,7 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     }
     pl->n1_chunk = n1c;
     pl->bytes_stage1 = (size_t)(fixed1 + per_r1 * n1c);
+    pl->k_chunk = kc;
 
     /* stage 2 */
     int64_t fixed2 = 8 * P + 4096, per_r1b = 4 * 8 * P;

@@ -100,7 +100,7 @@ class MoStats(ctypes.Structure):
 class Plan(ctypes.Structure):
     _fields_ = [("n1_chunk", ctypes.c_int64), ("ldk", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
                 ("n1_chunk_2e", ctypes.c_int64), ("bytes_stage1", ctypes.c_size_t),
-                ("bytes_stage2", ctypes.c_size_t)]
+                ("bytes_stage2", ctypes.c_size_t), ("k_chunk", ctypes.c_int64)]
 
 
 # every C symbol the public header declares

@@ -220,6 +220,8 @@ typedef struct cutc_plan {
     int64_t n1_chunk_2e;   /* r1 rows per integrand chunk in stage 2 */
     size_t bytes_stage1;   /* workspace of stage 1                  */
     size_t bytes_stage2;   /* workspace of stage 2                  */
+    int64_t k_chunk;       /* r2 points per K pass of the stage-1 contraction (ldk: one pass; modular
+                              precisions take several when the pair residue planes do not fit) */
 } cutc_plan;
 
 int cutc_make_plan(int64_t n_grid1, int64_t n_grid2, int n_ao, int n_nuc, int precision, int slices,

@@ -116,3 +116,19 @@ def test_fortran_binding_compiles_and_links(tmp_path):
                         f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert exe.exists()
+
+
+def test_plan_modular_k_passes():
+    """The 16-bit modular planes of all 300k r2 points (10 moduli x 45150
+    pairs x 2 bytes = 271 GB) do not fit beside the rest: the planner cuts K
+    into passes (multiples of 768 points) and keeps r1 chunks of >= 1024."""
+    GB = 1 << 30
+    one = cutc.make_plan(80000, 300000, 300, 40, 1 << 44, precision="bf16crt")
+    assert one.k_chunk == one.ldk
+    p = cutc.make_plan(80000, 300000, 300, 40, 240 * GB, precision="bf16crt")
+    assert p.k_chunk < p.ldk and p.k_chunk % 768 == 0
+    assert p.n1_chunk >= 1024 and p.bytes_stage1 <= 240 * GB
+    # int8 at the same budget needs one pass; FP64 never cuts K
+    assert cutc.make_plan(80000, 300000, 300, 40, 240 * GB, precision="int8").k_chunk == one.ldk
+    f = cutc.make_plan(80000, 300000, 300, 40, 240 * GB)
+    assert f.k_chunk == f.ldk

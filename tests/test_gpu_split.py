@@ -64,6 +64,36 @@ def test_modular_triatomic_and_chunking(torch, precision):
     assert relerr(Xc, X) <= 1e-13 and relerr(Ic, I) <= 1e-12
 
 
+def k_pass_limit(s, precision):
+    """a device memory limit under which the planner cuts stage 1's K (r2)
+    into several passes, found on the host"""
+    big = cutc.make_plan(s.n1, s.n2, s.n_ao, s.n_nuc, 1 << 40, precision=precision)
+    assert big.k_chunk == big.ldk
+    for frac in (0.8, 0.7, 0.6, 0.5, 0.45, 0.4, 0.35, 0.3):
+        lim = int(big.bytes_stage1 * frac)
+        try:
+            p = cutc.make_plan(s.n1, s.n2, s.n_ao, s.n_nuc, lim, precision=precision)
+        except cutc.CutcError:
+            continue
+        if p.k_chunk < p.ldk and p.bytes_stage2 <= lim:
+            return lim, p
+    raise AssertionError("no limit gives K passes")
+
+
+@pytest.mark.parametrize("precision", ["int8", "bf16crt", "fp16crt"])
+def test_modular_k_passes_match_one_pass(torch, precision):
+    """Under a memory limit the modular stage 1 runs K in several passes
+    whose residue products are added modulo each modulus: X and int_2e_ao
+    are bit-identical to the one-pass build (exact integer arithmetic)."""
+    s = synth.make_system(6, 40, 300, 5000, 77, name="kpass")
+    lim, p = k_pass_limit(s, precision)
+    print(f"{precision} K passes: limit {lim / 1e6:.0f} MB, k_chunk {p.k_chunk} of ldk {p.ldk}", flush=True)
+    X1, I1, _ = run(torch, s, precision=precision)
+    Xk, Ik, st = run(torch, s, precision=precision, mem_limit=lim)
+    assert np.array_equal(Xk, X1) and np.array_equal(Ik, I1)
+    assert st["workspace_bytes"] <= lim
+
+
 def test_int8_fewer_moduli_degrade_gracefully(torch):
     s = synth.triatomic(n1=200, n2=3000)
     Xo = oracle.int2_grad1_u12_ao(s)
@@ -120,7 +150,7 @@ def test_gemm_pk_matches_exact_products(torch, M, N, K):
     L = cutc.lib()
     L.cutc_crt_setup.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Crt)]
     L.cutc_crt_gemm_pk.argtypes = [ctypes.POINTER(Crt), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
-                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     c = Crt()
     assert L.cutc_crt_setup(K, 6, 0, ctypes.byref(c)) == 0
     n = 2  # moduli 256 and 255
@@ -138,13 +168,19 @@ def test_gemm_pk_matches_exact_products(torch, M, N, K):
     B = torch.from_numpy(np.concatenate([pack(rb[i], nb) for i in range(n)])).cuda()
     C = torch.zeros(n * M * N, dtype=torch.int8, device="cuda")
     torch.cuda.synchronize()
-    rc = L.cutc_crt_gemm_pk(ctypes.byref(c), A.data_ptr(), M, B.data_ptr(), N, kpad, C.data_ptr(), None)
+    rc = L.cutc_crt_gemm_pk(ctypes.byref(c), A.data_ptr(), M, B.data_ptr(), N, kpad, C.data_ptr(), 0, None)
     torch.cuda.synchronize()
     assert rc == 0
     got = C.cpu().numpy().reshape(n, M, N).astype(np.int64)
+    # a second K pass of the same products added to the residues in C
+    rc = L.cutc_crt_gemm_pk(ctypes.byref(c), A.data_ptr(), M, B.data_ptr(), N, kpad, C.data_ptr(), 1, None)
+    torch.cuda.synchronize()
+    assert rc == 0
+    got2 = C.cpu().numpy().reshape(n, M, N).astype(np.int64)
     for i in range(n):
         m = c.m[i]
         exact = (ra[i].astype(np.float64) @ rb[i].astype(np.float64).T).astype(np.int64)
+        assert np.array_equal(got2[i] % m, (2 * exact) % m), f"modulus {m}, accumulated"
         want = exact % m
         assert np.array_equal(got[i] % m, want), f"modulus {m}"
         assert got[i].min() >= -128 and got[i].max() <= 127
@@ -169,16 +205,18 @@ def to16(D, f16):
 
 
 @pytest.mark.parametrize("kind", [1, 2])
-@pytest.mark.parametrize("M,N,K", [(37, 5, 100), (1100, 300, 5000), (512, 130, 131328), (600, 128, 262000)])
-def test_gemm16_matches_exact_products(torch, kind, M, N, K):
+@pytest.mark.parametrize("M,N,K,accum", [(37, 5, 100, 0), (1100, 300, 5000, 0), (512, 130, 131328, 0),
+                                         (600, 128, 262000, 0), (600, 130, 5000, 1)])
+def test_gemm16_matches_exact_products(torch, kind, M, N, K, accum):
     """The bf16 / fp16 modular GEMM against float64 products (exact) mod m,
     with rows of +-127 (the largest |product|, summed far beyond 2^24) and
-    K beyond several FP32 reduction windows."""
-    print(f"gemm16 kind={kind} M={M} N={N} K={K}", flush=True)
+    K beyond several FP32 reduction windows; accum: the residues already in
+    C are added (one K pass of several)."""
+    print(f"gemm16 kind={kind} M={M} N={N} K={K} accum={accum}", flush=True)
     L = cutc.lib()
     L.cutc_crt_setup.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Crt)]
     L.cutc_crt_gemm_pk.argtypes = [ctypes.POINTER(Crt), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
-                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+                                   ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     c = Crt()
     assert L.cutc_crt_setup(K, 6, kind, ctypes.byref(c)) == 0
     n = 2  # moduli 255 and 253
@@ -196,14 +234,18 @@ def test_gemm16_matches_exact_products(torch, kind, M, N, K):
     ma, nb = (M + 511) // 512 * 512, (N + 127) // 128 * 128
     A = torch.from_numpy(np.concatenate([pack16(to16(ra[i], kind == 2), ma) for i in range(n)]).view(np.int16)).cuda()
     B = torch.from_numpy(np.concatenate([pack16(to16(rb[i], kind == 2), nb) for i in range(n)]).view(np.int16)).cuda()
-    C = torch.zeros(n * M * N, dtype=torch.int8, device="cuda")
+    C0 = np.zeros((n, M, N), np.int64)
+    if accum:
+        for i in range(n):
+            C0[i] = rng.integers(-(c.m[i] // 2), (c.m[i] - 1) // 2 + 1, (M, N))
+    C = torch.from_numpy(C0.astype(np.int8).reshape(-1)).cuda()
     torch.cuda.synchronize()
-    rc = L.cutc_crt_gemm_pk(ctypes.byref(c), A.data_ptr(), M, B.data_ptr(), N, kpad, C.data_ptr(), None)
+    rc = L.cutc_crt_gemm_pk(ctypes.byref(c), A.data_ptr(), M, B.data_ptr(), N, kpad, C.data_ptr(), accum, None)
     torch.cuda.synchronize()
     assert rc == 0
     got = C.cpu().numpy().reshape(n, M, N).astype(np.int64)
     for i in range(n):
         m = c.m[i]
-        exact = (ra[i].astype(np.float64) @ rb[i].astype(np.float64).T).astype(np.int64)
+        exact = (ra[i].astype(np.float64) @ rb[i].astype(np.float64).T).astype(np.int64) + C0[i]
         assert np.array_equal(got[i] % m, exact % m), f"modulus {m}"
         assert got[i].min() >= -(m // 2) and got[i].max() <= (m - 1) // 2
