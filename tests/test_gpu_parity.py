@@ -246,3 +246,59 @@ def test_fortran_caller_zero_demo(torch, tmp_path):
     last = r.stdout.strip().splitlines()[-1].split()
     assert last[0] == "status" and int(last[1]) == 0
     assert float(last[3]) == 0.0
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/llvm/bin/amdflang"), reason="amdflang not present")
+def test_fortran_chain(torch, tmp_path):
+    """AOs -> cutc_int -> cutc_ao_to_mo -> cutc_no from Fortran through
+    include/cutc_module.f90 (tests/fortran/cutc_chain.f90), against the
+    oracle's chain on the same inputs (the device AOs differ from the host
+    ones in the last bits, hence 1e-11)."""
+    print("fortran chain", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    from cutc_amd import _lib
+    libdir = os.path.dirname(_lib.lib_path("libcutcint.so"))
+    exe = tmp_path / "chain"
+    subprocess.run(["/opt/rocm/llvm/bin/amdflang", "-J", str(tmp_path), os.path.join(root, "include", "cutc_module.f90"),
+                    os.path.join(root, "tests", "fortran", "cutc_chain.f90"), "-o", str(exe), f"-L{libdir}",
+                    "-lcutcint", f"-Wl,-rpath,{libdir}"], check=True, capture_output=True)
+    s = synth.triatomic(n1=300, n2=2000, seed=41)
+    b = s.basis
+    n_mo, na, nb = 12, 4, 3
+    rng = np.random.default_rng(3)
+    cl = rng.standard_normal((n_mo, s.n_ao)) / np.sqrt(s.n_ao)  # numpy (n_mo, n_ao) = Fortran cl(n_ao, n_mo)
+    cr = cl + 0.05 * rng.standard_normal((n_mo, s.n_ao)) / np.sqrt(s.n_ao)
+    pmax = b["ao_expo"].shape[1]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([s.n_nuc, s.n_ao, s.n1, s.n2, s.size_bh, pmax, n_mo, na, nb], np.int32).tofile(f)
+        for a in (s.r1, s.wr1, s.r2, s.wr2, s.rn):
+            np.ascontiguousarray(a, np.float64).tofile(f)
+        for k in ("ao_nucl", "ao_power", "ao_prim_num"):
+            np.ascontiguousarray(b[k], np.int32).tofile(f)
+        for k in ("ao_expo", "ao_coef"):
+            np.ascontiguousarray(b[k], np.float64).tofile(f)
+        np.ascontiguousarray(s.c_bh, np.float64).tofile(f)
+        for a in (s.m_bh, s.n_bh, s.o_bh):
+            np.ascontiguousarray(a, np.int32).tofile(f)
+        cl.tofile(f)
+        cr.tofile(f)
+    r = subprocess.run([str(exe), str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(fout, np.uint8)
+    st = raw[:20].view(np.int32)
+    assert not st.any(), st
+    d = raw[20:].view(np.float64)
+    na4, n1 = s.n_ao ** 4, s.n1
+    sizes = [na4, n1 * 3 * n_mo * n_mo, n_mo ** 4, n_mo * n_mo, 1]
+    parts = np.split(d, np.cumsum(sizes)[:-1])
+    Xo, Io = oracle.cutc_int(s)
+    Go = oracle.ao_to_mo(cl, cr, Xo)
+    chi = oracle.aos(s.r1, s.rn, **b)[0]
+    want_no = oracle.cutc_no(na, nb, s.wr1, cl @ chi, cr @ chi, Go)
+    errs = [relerr(parts[0], Io.reshape(-1)), relerr(parts[1], Go.reshape(-1)),
+            relerr(parts[2], want_no[0].reshape(-1)), relerr(parts[3], want_no[1].reshape(-1)),
+            abs(parts[4][0] - want_no[2]) / abs(want_no[2])]
+    print("  relerr int_2e_ao %.2e G %.2e no_2e %.2e no_1e %.2e no_0e %.2e" % tuple(errs), flush=True)
+    assert max(errs) <= 1e-11
+
