@@ -297,7 +297,9 @@ __device__ static inline void store_residues(const cutc_crt &c, const double (&q
 // ---------------------------------------------------------------- quantisation
 
 // grid (kpad / 128, ceil(rows / 32)): a block covers one row block and 128
-// K; thread r = tid % 32 takes row 32 rb + r, the 16 values from 16 j
+// K.  The 32 x 128 tile of X is first read into LDS with coalesced 16-byte
+// loads (8 consecutive lanes cover 128 contiguous bytes of one row); then
+// thread r = tid % 32 takes row 32 rb + r, the 16 values from 16 j
 // (j = tid / 32), so the 16-byte stores of consecutive threads are
 // consecutive.
 template <int KIND>
@@ -305,21 +307,45 @@ __global__ __launch_bounds__(256) void quant_k_pk_kernel(cutc_crt c, const doubl
                                                          int64_t rows, int64_t K, int64_t KT, int64_t plane,
                                                          const int *__restrict__ rexp, int8_t *planes)
 {
-    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
-    const int64_t rb = blockIdx.y, row = rb * 32 + r;
-    const int64_t k0 = (int64_t)blockIdx.x * 128 + 16 * j;
-    double q[16];
-    if (row < rows) {
-        const int sh = c.bits - rexp[row];
-        const double *src = X + row * ld;
+    __shared__ double tile[32][128 + 1]; // odd row pitch: the column reads below are conflict-free
+    const int64_t rb = blockIdx.y, kb = (int64_t)blockIdx.x * 128;
+    {
+        const int rr = threadIdx.x >> 3, seg = threadIdx.x & 7; // 32 rows x 8 lanes
+        const int64_t row = rb * 32 + rr;
+        const bool pairs = (((uintptr_t)X | (uintptr_t)(ld * 8)) & 15) == 0;
 #pragma unroll
-        for (int e = 0; e < 16; e++)
-            q[e] = (k0 + e < K) ? rint(ldexp(src[k0 + e], sh)) : 0.0;
-    } else {
-#pragma unroll
-        for (int e = 0; e < 16; e++)
-            q[e] = 0.0;
+        for (int it = 0; it < 8; it++) {
+            const int col = 16 * it + 2 * seg; // lanes of a row: 16 contiguous bytes each, 128 per pass
+            const int64_t k = kb + col;
+            double v0 = 0.0, v1 = 0.0;
+            if (row < rows) {
+                const double *src = X + row * ld + k;
+                if (k + 1 < K) {
+                    if (pairs) {
+                        const v2d v = *(const v2d *)src;
+                        v0 = v[0];
+                        v1 = v[1];
+                    } else {
+                        v0 = src[0];
+                        v1 = src[1];
+                    }
+                } else if (k < K) {
+                    v0 = src[0];
+                }
+            }
+            tile[rr][col] = v0;
+            tile[rr][col + 1] = v1;
+        }
     }
+    __syncthreads();
+    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const int64_t row = rb * 32 + r;
+    const int64_t k0 = kb + 16 * j;
+    double q[16];
+    const int sh = (row < rows) ? c.bits - rexp[row] : 0;
+#pragma unroll
+    for (int e = 0; e < 16; e++)
+        q[e] = rint(ldexp(tile[r][16 * j + e], sh));
     store_residues<KIND>(c, q, planes, rb, r, k0, KT, plane);
 }
 
