@@ -3,6 +3,8 @@ public header declares (no compute calls without a GPU), and its memory plan
 behaves (pure host function)."""
 import ctypes
 import os
+
+import numpy as np
 import re
 import shutil
 import subprocess
@@ -132,3 +134,23 @@ def test_plan_modular_k_passes():
     assert cutc.make_plan(80000, 300000, 300, 40, 240 * GB, precision="int8").k_chunk == one.ldk
     f = cutc.make_plan(80000, 300000, 300, 40, 240 * GB)
     assert f.k_chunk == f.ldk
+
+
+def test_argument_errors_before_any_device_call():
+    """The entry points reject bad sizes and occupations with -2 before
+    touching the device (runs without a GPU)."""
+    L = cutc.lib()
+    assert L.cutc_ao_to_mo(-1, 2, 2, None, None, None, None) == cutc.CUTC_ERR_ARG
+    assert L.cutc_ao_to_mo(2, 2, 2, None, None, None, None) == cutc.CUTC_ERR_ARG  # missing arrays
+    assert L.cutc_aos(-1, 1, 1, 1, None, None, None, None, None, None, None, None) == cutc.CUTC_ERR_ARG
+    assert L.cutc_aos(4, 1, 70000, 1, None, None, None, None, None, None, None, None) == cutc.CUTC_ERR_ARG
+    nucl = np.array([2], np.int32)  # nucleus 2 of 1: checked on the host
+    power = np.zeros(3, np.int32)
+    nprim = np.ones(1, np.int32)
+    r = np.zeros(3)
+    e = np.ones(1)
+    assert L.cutc_aos(1, 1, 1, 1, r.ctypes.data, r.ctypes.data, nucl.ctypes.data, power.ctypes.data,
+                      nprim.ctypes.data, e.ctypes.data, e.ctypes.data, r.ctypes.data) == cutc.CUTC_ERR_ARG
+    assert L.cutc_no(10, 4, 5, 2, None, None, None, None, None, None, None) == cutc.CUTC_ERR_ARG
+    assert L.cutc_no(10, 4, 1, -1, None, None, None, None, None, None, None) == cutc.CUTC_ERR_ARG
+    assert L.cutc_get_last_error() == cutc.CUTC_ERR_ARG
