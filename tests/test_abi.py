@@ -3,12 +3,11 @@ public header declares (no compute calls without a GPU), and its memory plan
 behaves (pure host function)."""
 import ctypes
 import os
-
-import numpy as np
 import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from cutc_amd import _lib, cutc
