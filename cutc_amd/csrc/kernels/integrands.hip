@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void r1_integrands_kernel(const double *__rest
         s[t] = (r < nr) ? aos1[ii0 + r0 + r + n1_ld * (ao + (int64_t)n_ao * c)] : 0.0;
     }
     if (threadIdx.x < R)
-        sw[threadIdx.x] = (threadIdx.x < nr) ? wr1[ii0 + r0 + threadIdx.x] : 0.0;
+        sw[threadIdx.x] = ((int)threadIdx.x < nr) ? wr1[ii0 + r0 + threadIdx.x] : 0.0;
     __syncthreads();
     const int64_t Q = (int64_t)n_ao * (n_ao + 1) / 2;
     const int64_t cs = (int64_t)n_ao * R;

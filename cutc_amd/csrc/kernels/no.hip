@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void no_point_kernel(cutc_no_args a, double *p
     red[0][threadIdx.x] = s0;
     red[1][threadIdx.x] = s1;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (unsigned h = 128; h > 0; h >>= 1) {
         if (threadIdx.x < h) {
             red[0][threadIdx.x] += red[0][threadIdx.x + h];
             red[1][threadIdx.x] += red[1][threadIdx.x + h];
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void no_gemv_kernel(const double *__restrict__
     __shared__ double red[256];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (unsigned h = 128; h > 0; h >>= 1) {
         if (threadIdx.x < h)
             red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
