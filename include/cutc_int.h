@@ -159,8 +159,9 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
 
 /* cutc_int_device and cutc_int_ex keep their device workspace (one arena
  * per device, grown on demand) between calls, so repeated builds allocate
- * nothing; this frees it.  cutc_int and the get_* functions release it
- * before returning, as the reference frees its buffers.  Calls are
+ * nothing; this frees it.  cutc_int, the get_* functions and the host-array
+ * cutc_no and cutc_ao_to_mo release it before returning, as the reference
+ * frees its buffers.  Calls are
  * serialised process-wide (they share the arenas). */
 void cutc_release_workspace(void);
 

@@ -31,8 +31,11 @@ extern "C" {
 #endif
 
 /* Host arrays in and out (the library allocates, copies and frees device
- * memory).  Returns 0, CUTC_ERR_ARG (-2) for negative sizes or missing
- * arrays, CUTC_ERR_NOMEM (-3), CUTC_ERR_HIP (-4). */
+ * memory).  Like cutc_int and cutc_no it returns holding no device memory:
+ * the cached workspace is released too (its pass buffer alone is
+ * 8 n_ao * 3 n_grid1 * n_mo bytes when it fits), including what an earlier
+ * cutc_int_ex call kept.  Returns 0, CUTC_ERR_ARG (-2) for negative sizes or
+ * missing arrays, CUTC_ERR_NOMEM (-3), CUTC_ERR_HIP (-4). */
 int cutc_ao_to_mo(int n_grid1, int n_ao, int n_mo, double *h_mo_l_coef, double *h_mo_r_coef,
                   double *h_int2_grad1_u12_ao, double *h_int2_grad1_u12);
 
