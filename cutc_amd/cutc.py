@@ -121,6 +121,15 @@ def lib():
     """The loaded libcutcint.so (raises if it is not built)."""
     global _L
     if _L is None:
+        # One HIP runtime per process.  PyTorch-ROCm ships its own
+        # libamdhip64 and asks for it as "libamdhip64.so"; the library asks
+        # for "libamdhip64.so.7".  With torch loaded first the library binds
+        # to torch's copy (same soname); loaded the other way round, torch
+        # brings a second runtime that sees no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(_lib.lib_path("libcutcint.so"))
         L.cutc_int.restype = _ci
         L.cutc_int.argtypes = [_ci] * 11 + [_vp] * 13

@@ -153,3 +153,15 @@ def test_argument_errors_before_any_device_call():
     assert L.cutc_no(10, 4, 5, 2, None, None, None, None, None, None, None) == cutc.CUTC_ERR_ARG
     assert L.cutc_no(10, 4, 1, -1, None, None, None, None, None, None, None) == cutc.CUTC_ERR_ARG
     assert L.cutc_get_last_error() == cutc.CUTC_ERR_ARG
+
+
+def test_one_hip_runtime_per_process():
+    """The wrapper loads torch before the library, so a process that uses both
+    maps a single libamdhip64 (torch asks for "libamdhip64.so", the library
+    for "libamdhip64.so.7"; the other order maps two runtimes and torch's then
+    sees no device)."""
+    pytest.importorskip("torch")
+    cutc.lib()
+    with open("/proc/self/maps") as f:
+        mapped = {line.split()[-1] for line in f if "libamdhip64" in line}
+    assert len(mapped) == 1, mapped
