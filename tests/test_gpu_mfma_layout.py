@@ -1,8 +1,10 @@
 """Lane layouts of the MFMA instructions the contraction kernels use, checked
 on the GPU with one instruction on known operands (in-bounds probe kernels in
-cutc_amd/csrc/kernels/mfma_probe.hip).  The kernels' epilogues and loaders
-assume exactly these mappings."""
+tests/hip/mfma_probe.hip, a test-only library built beside the tests, not part
+of libcutcint.so).  The kernels' epilogues and loaders assume exactly these
+mappings."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -19,8 +21,23 @@ def torch():
     return t
 
 
+PROBE_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hip", "libmfmaprobe.so")
+_P = None
+
+
+def probe_lib():
+    """tests/hip/libmfmaprobe.so, loaded after libcutcint.so (and torch), so
+    that the process keeps one HIP runtime"""
+    global _P
+    if _P is None:
+        cutc.lib()
+        assert os.path.exists(PROBE_LIB), f"{PROBE_LIB} missing: run __graft_entry__.build()"
+        _P = ctypes.CDLL(PROBE_LIB)
+    return _P
+
+
 def probe(torch, kind, a, b, out_dtype, nout):
-    L = cutc.lib()
+    L = probe_lib()
     L.cutc_mfma_probe.restype = ctypes.c_int
     L.cutc_mfma_probe.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     da = torch.from_numpy(np.ascontiguousarray(a)).cuda()
