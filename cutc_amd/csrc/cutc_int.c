@@ -38,8 +38,12 @@ static double now_ms(void)
     return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
 }
 
-#define NCAT 7
-enum { CAT_JAST = 0, CAT_INTEG, CAT_GEMM1, CAT_GEMM2, CAT_SYMM, CAT_SPLIT, CAT_TOTAL };
+/* CAT_LONGR: stage-1 pair weights (the long-range integrand); CAT_SHORTR:
+ * stage-2 r1 integrands (herm and non-herm in one kernel); CAT_STAGE1/2:
+ * each stage as a whole */
+#define NCAT 10
+enum { CAT_JAST = 0, CAT_LONGR, CAT_GEMM1, CAT_GEMM2, CAT_SYMM, CAT_SPLIT, CAT_TOTAL, CAT_SHORTR, CAT_STAGE1,
+       CAT_STAGE2 };
 
 typedef struct {
     hipStream_t st;
@@ -146,20 +150,13 @@ static int hip_rc(hipError_t e)
     } while (0)
 #define TRYH(x) TRY(hip_rc(x))
 
+/* slices = 0: every precision takes its own default (cutc_default_slices) */
 void cutc_default_options(cutc_options *opt)
 {
     memset(opt, 0, sizeof *opt);
     opt->precision = CUTC_PREC_FP64;
-    opt->slices = 4;
+    opt->slices = 0;
 }
-
-/* Moduli of the int8 contraction when the caller leaves slices at 0: with
- * 2-norm row scaling, 11 moduli keep the contractions of the 20-nucleus,
- * n_ao = 200 workload near 1e-11 relative error. */
-#define CUTC_INT8_DEFAULT_MODULI 11
-/* Moduli of the bf16 / fp16 modular contraction (moduli <= 255, about 7.9
- * bits each): 10 carry as many bits as 10 int8 moduli. */
-#define CUTC_CRT16_DEFAULT_MODULI 10
 
 static int is_crt(int precision)
 {
@@ -188,12 +185,8 @@ static int valid_precision(const cutc_options *o)
 static cutc_options resolved_options(const cutc_options *in)
 {
     cutc_options o = *in;
-    if (o.precision == CUTC_PREC_INT8 && o.slices == 0)
-        o.slices = CUTC_INT8_DEFAULT_MODULI;
-    if ((o.precision == CUTC_PREC_BF16_CRT || o.precision == CUTC_PREC_FP16_CRT) && o.slices == 0)
-        o.slices = CUTC_CRT16_DEFAULT_MODULI;
-    if ((o.precision == CUTC_PREC_BF16 || o.precision == CUTC_PREC_FP16) && o.slices == 0)
-        o.slices = 4;
+    if (o.slices == 0)
+        o.slices = cutc_default_slices(o.precision);
     return o;
 }
 
@@ -494,7 +487,7 @@ static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
 
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
-    ev = tr_begin(tr, CAT_INTEG);
+    ev = tr_begin(tr, CAT_LONGR);
     rc = cutc_launch_pair_weights(wr2, aos2, n2, 0, n2, ldk, n_ao, Apk, ldk, st);
     tr_end(tr, ev);
     TRY(rc);
@@ -838,7 +831,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         const int64_t n1e = (cnt - i0 < n1c) ? cnt - i0 : n1c;
         const int first = i0 == 0 && !accumulate;
         const int64_t pad0 = n1e * P, padn = ((n1e + 15) / 16 * 16 - n1e) * P;
-        ev = tr_begin(tr, CAT_INTEG);
+        ev = tr_begin(tr, CAT_SHORTR);
         rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, NHp, P, plane, st);
         for (int m = 0; m < 4 && rc == CUTC_OK && padn > 0; m++)
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
@@ -950,7 +943,11 @@ static void fill_stats(cutc_stats *stats, tracer_t *tr)
     double ms[NCAT];
     tr_collect(tr, ms);
     stats->ms_jastrow = ms[CAT_JAST];
-    stats->ms_integrands = ms[CAT_INTEG];
+    stats->ms_integrands = ms[CAT_LONGR] + ms[CAT_SHORTR];
+    stats->ms_long_range = ms[CAT_LONGR];
+    stats->ms_short_range = ms[CAT_SHORTR];
+    stats->ms_stage1 = ms[CAT_STAGE1];
+    stats->ms_stage2 = ms[CAT_STAGE2];
     stats->ms_gemm1 = ms[CAT_GEMM1];
     stats->ms_gemm2 = ms[CAT_GEMM2];
     stats->ms_symm = ms[CAT_SYMM];
@@ -959,6 +956,62 @@ static void fill_stats(cutc_stats *stats, tracer_t *tr)
     stats->n_gemm1 = tr->launches[CAT_GEMM1];
     stats->n_gemm2 = tr->launches[CAT_GEMM2];
     stats->n_jastrow = tr->launches[CAT_JAST];
+}
+
+/* The reference's chunk-plan lines (src/build_int2_grad1_u12_ao.cu:60-61,
+ * 81-95) for this build's plan: free and total device memory, r1 rows per
+ * chunk with the count of full chunks and the rows of the last, r2 points
+ * per K pass likewise. */
+static void print_plan(const cutc_plan *pl, int64_t n1, int64_t n2)
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess)
+        fr = tot = 0;
+    printf(" free  memory = %.3f GB\n", (double)fr / 1073741824.0);
+    printf(" total memory = %.3f GB\n", (double)tot / 1073741824.0);
+    const int64_t p1 = pl->n1_chunk > 0 ? pl->n1_chunk : 1;
+    int64_t p2 = pl->k_chunk < n2 ? pl->k_chunk : n2;
+    if (p2 < 1)
+        p2 = 1;
+    printf("n_grid1_pass = %lld\n", (long long)(pl->n1_chunk < n1 ? pl->n1_chunk : n1));
+    printf("n_grid1_rest = %lld\n", (long long)(n1 % p1));
+    printf("n1_pass = %lld\n", (long long)(n1 / p1));
+    printf("n_grid2_pass = %lld\n", (long long)(n2 > 0 ? p2 : 0));
+    printf("n_grid2_rest = %lld\n", (long long)(n2 % p2));
+    printf("n2_pass = %lld\n", (long long)(n2 / p2));
+    fflush(stdout);
+}
+
+/* The reference's stage-time lines, in its order: stage 1
+ * (src/build_int2_grad1_u12_ao.cu:342-344), the stage-1 total as cutc_int
+ * prints it (src/cutc_int.c:153, host entry points only), stage 2
+ * (src/build_int_2e_ao.cu:139-142) and its total (src/cutc_int.c:198).  The
+ * herm and non-herm integrands run as one kernel here: its time is on the
+ * herm line.  Split precisions add a line for their quantisation and CRT
+ * rebuild kernels, which the reference does not have. */
+static void print_stage_times(tracer_t *tr, int stages, int split, int host)
+{
+    double ms[NCAT];
+    tr_collect(tr, ms);
+    if (stages & 1) {
+        printf("Ellapsed time for DGEMM to build int2_grad1_u12_ao = %.3f sec\n", ms[CAT_GEMM1] / 1000.0);
+        printf("Ellapsed time for int_long_range kernel = %.3f sec\n", ms[CAT_LONGR] / 1000.0);
+        printf("Ellapsed time for cutc_int_bh kernel = %.3f sec\n", ms[CAT_JAST] / 1000.0);
+        if (host)
+            printf("Ellapsed time for get_int2_grad1_u12_ao = %.3f sec\n", ms[CAT_STAGE1] / 1000.0);
+    }
+    if (stages & 2) {
+        printf("Ellapsed time for DGEMM to build int_2e_ao = %.3f sec\n", ms[CAT_GEMM2] / 1000.0);
+        printf("Ellapsed time for int_short_range_herm kernel = %.3f sec\n", ms[CAT_SHORTR] / 1000.0);
+        printf("Ellapsed time for int_short_range_nonherm kernel = %.3f sec (computed by the herm kernel)\n", 0.0);
+        printf("Ellapsed time for addT = %.3f sec\n", ms[CAT_SYMM] / 1000.0);
+        if (host)
+            printf("Ellapsed time for get_int_2e_ao = %.3f sec\n", ms[CAT_STAGE2] / 1000.0);
+    }
+    if (split)
+        printf("Ellapsed time for split-precision quantisation and rebuild kernels = %.3f sec\n",
+               ms[CAT_SPLIT] / 1000.0);
+    fflush(stdout);
 }
 
 static int device_path(const cutc_options *opt, int stages, int n1, int n2, int n_ao, int n_nuc, int size_bh,
@@ -1004,7 +1057,10 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         tr.x_ready = hook->x_ready;
         tr.x_ctx = hook->x_ctx;
     }
+    if (opt->verbose && (stages & 1))
+        print_plan(&pl, n1, n2);
     int evt = tr_begin(&tr, CAT_TOTAL);
+    int evs = tr_begin(&tr, CAT_STAGE1);
     if ((stages & 1) && n1 > 0 && n_ao > 0) {
         if (n2 == 0) {
             int ev = tr_begin(&tr, CAT_GEMM1);
@@ -1016,6 +1072,8 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
         }
     }
+    tr_end(&tr, evs);
+    evs = tr_begin(&tr, CAT_STAGE2);
     if (rc == CUTC_OK && (stages & 2) && n_ao > 0) {
         if (n1 == 0) {
             int ev = tr_begin(&tr, CAT_GEMM2);
@@ -1026,6 +1084,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
                         split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
         }
     }
+    tr_end(&tr, evs);
     tr_end(&tr, evt);
     if (rc == CUTC_OK)
         rc = hip_rc(hipStreamSynchronize(st));
@@ -1033,20 +1092,8 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
         hipStreamSynchronize(st);
     if (rc == CUTC_OK) {
         fill_stats(stats, &tr);
-        if (opt->verbose) {
-            double ms[NCAT];
-            tr_collect(&tr, ms);
-            if (stages & 1) {
-                printf("Ellapsed time for DGEMM to build int2_grad1_u12_ao = %.3f sec\n", ms[CAT_GEMM1] / 1000.0);
-                printf("Ellapsed time for int_long_range kernel = %.3f sec\n", ms[CAT_INTEG] / 1000.0);
-                printf("Ellapsed time for cutc_int_bh kernel = %.3f sec\n", ms[CAT_JAST] / 1000.0);
-            }
-            if (stages & 2) {
-                printf("Ellapsed time for DGEMM to build int_2e_ao = %.3f sec\n", ms[CAT_GEMM2] / 1000.0);
-                printf("Ellapsed time for addT = %.3f sec\n", ms[CAT_SYMM] / 1000.0);
-            }
-            fflush(stdout);
-        }
+        if (opt->verbose)
+            print_stage_times(&tr, stages, opt->precision != CUTC_PREC_FP64, hook != NULL);
     }
     tr_free(&tr);
     jprog_free(&prog);
@@ -1084,6 +1131,7 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
     int rc = CUTC_OK;
     for (int64_t r0 = 0; r0 < n1 && rc == CUTC_OK; r0 += nc) {
         const int64_t cnt = (n1 - r0 < nc) ? n1 - r0 : nc;
+        int evs = tr_begin(tr, CAT_STAGE1);
         if (n2 == 0) {
             int ev = tr_begin(tr, CAT_GEMM1);
             rc = cutc_launch_fill(Xc, 4 * cnt * nao2, 0.0, tr->st);
@@ -1093,8 +1141,12 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
         } else {
             rc = stage1(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats);
         }
-        if (rc == CUTC_OK)
+        tr_end(tr, evs);
+        if (rc == CUTC_OK) {
+            evs = tr_begin(tr, CAT_STAGE2);
             rc = stage2(opt, pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, nsplit2, tr, stats);
+            tr_end(tr, evs);
+        }
     }
     return rc;
 }
@@ -1182,6 +1234,8 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
                 stats->n1_chunk = nc;
                 stats->n1_chunks = (int)((n1 + nc - 1) / nc);
             }
+            if (opt->verbose)
+                print_stage_times(&tr, 3, opt->precision != CUTC_PREC_FP64, 0);
         }
     }
 done:
@@ -1420,6 +1474,7 @@ void get_int2_grad1_u12_ao(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int n_grid1, i
     (void)dimBlock;
     cutc_options opt;
     cutc_default_options(&opt);
+    opt.verbose = 1; /* the reference prints its plan and stage times */
     pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 1, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, NULL, r2, wr2, rn, NULL, aos_data2,
                          c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL, NULL);
@@ -1435,6 +1490,7 @@ void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double
 {
     cutc_options opt;
     cutc_default_options(&opt);
+    opt.verbose = 1; /* the reference prints its stage times */
     pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 2, n_grid1, 0, n_ao, 0, 0, NULL, wr1, NULL, NULL, NULL, aos_data1, NULL, NULL, NULL,
                          NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL, NULL);

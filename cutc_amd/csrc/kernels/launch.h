@@ -105,6 +105,24 @@ int cutc_gemm2_f64(const double *X, int64_t xld, const double *B, int64_t ldb, i
  * 1024 * 127^2 + 1.5 * 255 < 2^24 in between).  GEMM results are int8
  * residues for every kind. */
 #define CUTC_CRT_MAX 16
+/* Moduli (modular kinds) or digit slices (slice kinds) a precision takes
+ * when cutc_options.slices is 0.  int8: 11 moduli; with 2-norm row scaling
+ * they keep the contractions of the 20-nucleus, n_ao = 200 workload near
+ * 1e-12 relative error, two orders of magnitude inside 1e-10.  bf16 / fp16
+ * modular: 10 moduli (moduli <= 255, about 7.9 bits each).  Slices: 4. */
+#define CUTC_INT8_DEFAULT_MODULI 11
+#define CUTC_CRT16_DEFAULT_MODULI 10
+#define CUTC_SLICE_DEFAULT 4
+static inline int cutc_default_slices(int precision)
+{
+    if (precision == CUTC_PREC_INT8)
+        return CUTC_INT8_DEFAULT_MODULI;
+    if (precision == CUTC_PREC_BF16_CRT || precision == CUTC_PREC_FP16_CRT)
+        return CUTC_CRT16_DEFAULT_MODULI;
+    if (precision == CUTC_PREC_BF16 || precision == CUTC_PREC_FP16)
+        return CUTC_SLICE_DEFAULT;
+    return 0;
+}
 #define CUTC_CRT_I8 0
 #define CUTC_CRT_BF16 1
 #define CUTC_CRT_FP16 2

@@ -40,8 +40,10 @@ static int64_t balanced(int64_t n, int64_t cmax, int64_t align)
 int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, int slices, size_t budget,
                    int64_t n1c_req, cutc_plan *pl)
 {
-    if (n1 < 0 || n2 < 0 || n_ao < 0 || n_nuc < 0 || !pl)
+    if (n1 < 0 || n2 < 0 || n_ao < 0 || n_nuc < 0 || slices < 0 || !pl)
         return CUTC_ERR_ARG;
+    if (slices == 0)
+        slices = cutc_default_slices(precision);
     const int crt = precision == CUTC_PREC_INT8 || precision == CUTC_PREC_BF16_CRT || precision == CUTC_PREC_FP16_CRT;
     const int64_t rb = precision == CUTC_PREC_INT8 ? 1 : 2; /* bytes per residue-plane element */
     const int slice = precision == CUTC_PREC_BF16 || precision == CUTC_PREC_FP16;

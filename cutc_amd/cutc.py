@@ -34,7 +34,8 @@ PREC_BF16_CRT = 4
 PREC_FP16_CRT = 5
 PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp16": PREC_FP16, "int8": PREC_INT8, "bf16crt": PREC_BF16_CRT,
               "fp16crt": PREC_FP16_CRT}
-DEFAULT_SLICES = {PREC_FP64: 4, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 11, PREC_BF16_CRT: 10, PREC_FP16_CRT: 10}
+# what cutc_options.slices = 0 resolves to in the library (cutc_default_slices)
+DEFAULT_SLICES = {PREC_FP64: 0, PREC_BF16: 4, PREC_FP16: 4, PREC_INT8: 11, PREC_BF16_CRT: 10, PREC_FP16_CRT: 10}
 
 
 class CutcError(RuntimeError):
@@ -58,7 +59,9 @@ class Stats(ctypes.Structure):
                 ("ms_symm", ctypes.c_double), ("ms_split", ctypes.c_double), ("n_gemm1", ctypes.c_int),
                 ("n_gemm2", ctypes.c_int), ("n_jastrow", ctypes.c_int), ("n1_chunk", ctypes.c_int64),
                 ("n1_chunks", ctypes.c_int), ("flops_exec", ctypes.c_double), ("workspace_bytes", ctypes.c_size_t),
-                ("ms_host_in", ctypes.c_double), ("ms_host_out", ctypes.c_double)]
+                ("ms_host_in", ctypes.c_double), ("ms_host_out", ctypes.c_double),
+                ("ms_long_range", ctypes.c_double), ("ms_short_range", ctypes.c_double),
+                ("ms_stage1", ctypes.c_double), ("ms_stage2", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

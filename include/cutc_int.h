@@ -143,8 +143,13 @@ typedef struct cutc_stats {
     size_t workspace_bytes;/* device workspace allocated by the call    */
     double ms_host_in;     /* host entry points: device allocation and upload before the build (wall clock) */
     double ms_host_out;    /* host entry points: end of the build to the last byte in host memory (wall clock) */
+    double ms_long_range;  /* of ms_integrands: stage-1 pair weights (the long-range integrand) */
+    double ms_short_range; /* of ms_integrands: stage-2 r1 integrands (herm and non-herm, one kernel) */
+    double ms_stage1;      /* stage 1 as a whole (Jastrow, integrand, contraction, split kernels) */
+    double ms_stage2;      /* stage 2 as a whole, with the symmetrisation */
 } cutc_stats;
 
+/* FP64, slices = 0 (each precision's own default), everything else 0. */
 void cutc_default_options(cutc_options *opt);
 
 /* Whole path on device pointers (inputs already in HBM), on `stream`
@@ -213,7 +218,8 @@ int cutc_int_batch_device(const cutc_options *opt, int n_systems, const cutc_sys
                           cutc_batch_stats *stats);
 
 /* Memory plan (pure host function): r1 chunk for the given device budget.
- * Returns 0 or CUTC_ERR_NOMEM. */
+ * slices as in cutc_options (0: the precision's default).  Returns 0,
+ * CUTC_ERR_ARG or CUTC_ERR_NOMEM. */
 typedef struct cutc_plan {
     int64_t n1_chunk;      /* r1 rows per G chunk                   */
     int64_t ldk;           /* padded r2 length of the K-contiguous operands */

@@ -15,6 +15,7 @@ from cutc_amd import _lib, cutc
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", h) for h in ("cutc_int.h", "cutc_no.h", "cutc_mo.h", "cutc_ao.h")]
 HEADER = HEADERS[0]
+GB = 1 << 30
 
 
 def header_functions():
@@ -43,7 +44,29 @@ def test_library_loads_and_binds():
         assert hasattr(L, name)
     assert L.cutc_error_string(-1).decode().startswith("launch geometry")
     o = cutc.options()
-    assert o.precision == 0 and o.slices == 4
+    assert o.precision == 0 and o.slices == 0
+
+
+def test_default_options_resolve_for_every_precision():
+    """cutc_default_options leaves slices at 0, which every precision
+    resolves to its own default: a C or Fortran caller that takes the
+    defaults and sets only the precision gets a valid call.  Checked
+    through the pure planner (no GPU): slices 0 plans exactly as the
+    precision's default count."""
+    L = cutc.lib()
+    o = cutc.Options()
+    L.cutc_default_options(ctypes.byref(o))
+    assert o.precision == cutc.PREC_FP64 and o.slices == 0 and o.n1_chunk == 0 and o.mem_limit == 0
+    for name, prec in cutc.PRECISIONS.items():
+        p0, pd = cutc.Plan(), cutc.Plan()
+        assert L.cutc_make_plan(60000, 200000, 200, 20, prec, o.slices, 200 * GB, 0, ctypes.byref(p0)) == 0, name
+        assert L.cutc_make_plan(60000, 200000, 200, 20, prec, cutc.DEFAULT_SLICES[prec], 200 * GB, 0,
+                                ctypes.byref(pd)) == 0, name
+        assert (p0.n1_chunk, p0.bytes_stage1, p0.bytes_stage2, p0.k_chunk) == \
+            (pd.n1_chunk, pd.bytes_stage1, pd.bytes_stage2, pd.k_chunk), name
+    assert cutc.DEFAULT_SLICES[cutc.PREC_INT8] == 11
+    p = cutc.Plan()
+    assert L.cutc_make_plan(10, 10, 4, 1, cutc.PREC_INT8, -1, GB, 0, ctypes.byref(p)) == cutc.CUTC_ERR_ARG
 
 
 def test_reference_entry_points_keep_their_signatures():
@@ -60,9 +83,6 @@ def test_reference_entry_points_keep_their_signatures():
     assert ("void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double *int2_grad1_u12_ao, "
             "double *tc_int_2e_ao);") in src
     assert ctypes.sizeof(cutc.Dim3) == 12
-
-
-GB = 1 << 30
 
 
 def test_plan_fits_budget_and_covers_r1():
