@@ -15,6 +15,8 @@
  *   oracle_trans_inplace   src/add_trans_inplace.cu:6-19
  *   oracle_int2_grad1_u12_ao  src/build_int2_grad1_u12_ao.cu:97-331 (chunk loop, TN DGEMMs)
  *   oracle_int_2e_ao       src/build_int_2e_ao.cu:44-124 (two NN DGEMMs + M + M^T)
+ *   oracle_int_2e_ao_cols  the same for one block of columns (large shapes)
+ *   oracle_dd_2e           sum D D int_2e_ao through the factors of M (large shapes)
  *   oracle_cutc_int        src/cutc_int.c:94-105,174-176,210-211 (what is returned)
  * It implements the intended semantics where the reference is defective:
  * 64-bit sizes everywhere and beta reset to 0 at the first r2 chunk of every
@@ -32,6 +34,9 @@
  *   stage drivers: brute-force quadruple loops at tiny shapes, chunk
  *     invariance, linearity in the weights, X_kl == X_lk.
  *   trans_inplace: symmetry, diagonal doubling.
+ *   int_2e_ao_cols and dd_2e: against the columns of oracle_int_2e_ao and an
+ *     einsum over it (tests/test_oracle.py), so that a wrong offset, a
+ *     missing transpose or a dropped non-herm term fails.
  * The reference ships no golden vectors; its only known answer (all-zero
  * inputs with c = 0 give all-zero outputs, src/cutc_int_f.f90:41-55 with
  * src/jast_bh.cu:203) is checked too.
@@ -405,6 +410,87 @@ int oracle_int_2e_ao(int64_t n1, int n_ao, const double *wr1, const double *aos1
     oracle_dgemm('N', 'N', nao2, nao2, 3 * n1, -0.5, X, nao2, NH, 3 * n1, 1.0, out, nao2);
     free(NH);
     oracle_trans_inplace(out, nao2);
+    return 0;
+}
+
+/* Columns [c0, c0 + nc) of int_2e_ao = M + M^T, as an n_ao^2 x nc
+ * column-major block (ld n_ao^2), without forming the n_ao^4 array: the two
+ * NN DGEMMs of src/build_int_2e_ao.cu:56-124 restricted to those columns of
+ * M, the same two DGEMMs for the matching rows of M, and the pairwise sum
+ * of src/add_trans_inplace.cu:6-19 for that block.  For the large shapes
+ * whose n_ao^4 array the tests do not hold on the host. */
+int oracle_int_2e_ao_cols(int64_t n1, int n_ao, const double *wr1, const double *aos1, const double *X, int64_t c0,
+                          int64_t nc, double *out)
+{
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    if (c0 < 0 || nc < 0 || c0 + nc > nao2)
+        return -2;
+    if (nc == 0)
+        return 0;
+    double *H = malloc(sizeof(double) * (size_t)n1 * nao2);
+    double *NH = malloc(sizeof(double) * 3 * (size_t)n1 * nao2);
+    double *R = malloc(sizeof(double) * (size_t)nc * nao2); /* rows c0.. of M, nc x n_ao^2, ld nc */
+    if (!H || !NH || !R) {
+        free(H);
+        free(NH);
+        free(R);
+        return -3;
+    }
+    oracle_herm(n1, n_ao, wr1, aos1, H);
+    oracle_nonherm(n1, n_ao, wr1, aos1, NH);
+    const double *X3 = X + (size_t)3 * nao2 * n1;
+    /* M[:, c0..] */
+    oracle_dgemm('N', 'N', nao2, nc, n1, 1.0, X3, nao2, H + (size_t)n1 * c0, n1, 0.0, out, nao2);
+    oracle_dgemm('N', 'N', nao2, nc, 3 * n1, -0.5, X, nao2, NH + (size_t)3 * n1 * c0, 3 * n1, 1.0, out, nao2);
+    /* M[c0.., :] */
+    oracle_dgemm('N', 'N', nc, nao2, n1, 1.0, X3 + c0, nao2, H, n1, 0.0, R, nc);
+    oracle_dgemm('N', 'N', nc, nao2, 3 * n1, -0.5, X + c0, nao2, NH, 3 * n1, 1.0, R, nc);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < nc; j++)
+        for (int64_t i = 0; i < nao2; i++)
+            out[i + nao2 * j] += R[j + nc * i];
+    free(H);
+    free(NH);
+    free(R);
+    return 0;
+}
+
+/* sum_{kl,ab} d_kl d_ab int_2e_ao[kl][ab] for a vector d of n_ao^2 entries,
+ * without forming int_2e_ao: with int_2e_ao = M + M^T and
+ * M = X^3 H - 1/2 X^{0..2} NH (src/build_int_2e_ao.cu:56-124) the sum is
+ * 2 d^T M d = 2 [ (X^3T d) . (H^T d) - 1/2 (X^{0..2}T d) . (NH^T d) ],
+ * four DGEMM-vector products. */
+int oracle_dd_2e(int64_t n1, int n_ao, const double *wr1, const double *aos1, const double *X, const double *d,
+                 double *result)
+{
+    const int64_t nao2 = (int64_t)n_ao * n_ao;
+    double *H = malloc(sizeof(double) * (size_t)n1 * nao2);
+    double *NH = malloc(sizeof(double) * 3 * (size_t)n1 * nao2);
+    double *u = malloc(sizeof(double) * (3 * (size_t)n1 + 1));
+    double *v = malloc(sizeof(double) * (3 * (size_t)n1 + 1));
+    if (!H || !NH || !u || !v) {
+        free(H);
+        free(NH);
+        free(u);
+        free(v);
+        return -3;
+    }
+    oracle_herm(n1, n_ao, wr1, aos1, H);
+    oracle_nonherm(n1, n_ao, wr1, aos1, NH);
+    double s = 0.0;
+    oracle_dgemm('T', 'N', n1, 1, nao2, 1.0, X + (size_t)3 * nao2 * n1, nao2, d, nao2, 0.0, u, n1);
+    oracle_dgemm('N', 'N', n1, 1, nao2, 1.0, H, n1, d, nao2, 0.0, v, n1);
+    for (int64_t r = 0; r < n1; r++)
+        s += u[r] * v[r];
+    oracle_dgemm('T', 'N', 3 * n1, 1, nao2, 1.0, X, nao2, d, nao2, 0.0, u, 3 * n1);
+    oracle_dgemm('N', 'N', 3 * n1, 1, nao2, 1.0, NH, 3 * n1, d, nao2, 0.0, v, 3 * n1);
+    for (int64_t r = 0; r < 3 * n1; r++)
+        s -= 0.5 * u[r] * v[r];
+    *result = 2.0 * s;
+    free(H);
+    free(NH);
+    free(u);
+    free(v);
     return 0;
 }
 

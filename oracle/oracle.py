@@ -66,6 +66,10 @@ def lib():
                                                _dp, _dp, _dp, _ip, _ip, _ip, _dp, _i64, _i64]
         L.oracle_int_2e_ao.restype = ctypes.c_int
         L.oracle_int_2e_ao.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.oracle_int_2e_ao_cols.restype = ctypes.c_int
+        L.oracle_int_2e_ao_cols.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp, _i64, _i64, _dp]
+        L.oracle_dd_2e.restype = ctypes.c_int
+        L.oracle_dd_2e.argtypes = [_i64, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_cutc_no.restype = ctypes.c_int
         L.oracle_cutc_no.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_aos.restype = ctypes.c_int
@@ -181,6 +185,33 @@ def int_2e_ao(sys, X):
     if rc != 0:
         raise MemoryError(f"oracle_int_2e_ao returned {rc}")
     return out
+
+
+def int_2e_ao_cols(sys, X, c0, nc):
+    """Columns [c0, c0 + nc) of int_2e_ao (Fortran column index a + n_ao b)
+    without the n_ao^4 array: numpy (nc, n_ao^2), equal to
+    int_2e_ao(sys, X).reshape(n_ao^2, n_ao^2)[c0:c0 + nc]."""
+    X = _f64(X)
+    nao2 = sys.n_ao ** 2
+    out = np.zeros((nc, nao2))
+    rc = lib().oracle_int_2e_ao_cols(sys.n1, sys.n_ao, _p(_f64(sys.wr1)), _p(_f64(sys.aos1)), _p(X), c0, nc, _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_int_2e_ao_cols returned {rc}")
+    return out
+
+
+def dd_2e(sys, X, D):
+    """sum_{kl,ab} D_kl D_ab (kl|ab) over int_2e_ao without forming it; D
+    (n_ao, n_ao) symmetric (any vector of n_ao^2 entries in the Fortran
+    order k + n_ao l)."""
+    X = _f64(X)
+    d = _f64(np.asarray(D).reshape(-1))
+    assert d.size == sys.n_ao ** 2
+    r = np.zeros(1)
+    rc = lib().oracle_dd_2e(sys.n1, sys.n_ao, _p(_f64(sys.wr1)), _p(_f64(sys.aos1)), _p(X), _p(d), _p(r))
+    if rc != 0:
+        raise ValueError(f"oracle_dd_2e returned {rc}")
+    return float(r[0])
 
 
 def cutc_int(sys, n1c=0, n2c=None):

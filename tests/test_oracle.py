@@ -233,6 +233,35 @@ def test_stage2_bruteforce_and_symmetry():
     np.testing.assert_array_equal(Im, Im.T)
 
 
+@pytest.mark.parametrize("c0,nc", [(0, 1), (0, 576), (37, 100), (575, 1), (200, 376)])
+def test_int_2e_ao_column_block_matches_full(c0, nc):
+    """oracle_int_2e_ao_cols (the large-shape checker) against the columns
+    of the full oracle_int_2e_ao: a wrong offset, a missing M^T half or a
+    dropped non-herm term changes whole columns."""
+    s = small_system(n1=13, n2=19, terms=ALL_TERMS)
+    X = oracle.int2_grad1_u12_ao(s)
+    full = oracle.int_2e_ao(s, X).reshape(s.n_ao ** 2, s.n_ao ** 2)
+    blk = oracle.int_2e_ao_cols(s, X, c0, nc)
+    np.testing.assert_allclose(blk, full[c0:c0 + nc], rtol=1e-13, atol=1e-14 * np.abs(full).max())
+
+
+def test_dd_contraction_matches_einsum():
+    """oracle_dd_2e (sum D D int_2e_ao through the factors of M) against an
+    einsum over the full int_2e_ao, for a symmetric PSD D and for a
+    non-symmetric d (which the factor form handles as 2 d^T M d only when
+    int_2e_ao = M + M^T is symmetric, as it is)."""
+    s = small_system(n1=13, n2=19, terms=ALL_TERMS)
+    X = oracle.int2_grad1_u12_ao(s)
+    I = oracle.int_2e_ao(s, X)
+    rng = np.random.default_rng(3)
+    C = rng.standard_normal((s.n_ao, 10))
+    for D in (C @ C.T, rng.standard_normal((s.n_ao, s.n_ao))):
+        d = D.reshape(-1)  # numpy D[l, k] = Fortran D(k, l): index k + n_ao l
+        want = float(d @ (I.reshape(s.n_ao ** 2, s.n_ao ** 2) @ d))
+        got = oracle.dd_2e(s, X, D)
+        assert abs(got - want) <= 1e-12 * abs(want), (got, want)
+
+
 def test_linearity_in_weights():
     s = small_system(n1=8, n2=15, terms=ALL_TERMS)
     X, I = oracle.cutc_int(s)
