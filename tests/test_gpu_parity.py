@@ -153,7 +153,12 @@ def test_chunking_and_reference_stage_functions(torch):
     cutc.get_int2_grad1_u12_ao(ds, X)
     cutc.get_int_2e_ao(ds, X, I)
     torch.cuda.synchronize()
-    assert relerr(X.cpu().numpy(), X0) <= 1e-14 and relerr(I.cpu().numpy(), I0) <= 1e-14
+    Xg, Ig = X.cpu().numpy(), I.cpu().numpy()
+    assert relerr(Xg, X0) <= 1e-14 and relerr(Ig, I0) <= 1e-14
+    # and against the oracle directly
+    Xo = oracle.int2_grad1_u12_ao(s)
+    Io = oracle.int_2e_ao(s, Xo)
+    assert relerr(Xg, Xo) <= TOL and relerr(Ig, Io) <= TOL
 
 
 @pytest.mark.parametrize("n1_chunk,precision", [(700, "fp64"), (0, "fp64"), (500, "int8")])
