@@ -8,10 +8,13 @@ synthetic inputs).
 
 Default run: one FP64 build (the plain path, after a small warm-up call),
 then W warm-up and K timed builds with the split-precision contraction
-(--precision, default int8 = modular int8 MFMA with 11 moduli), then the
-split results are compared with the FP64 ones: relative Frobenius and
-max-abs error of every X component and of int_2e_ao, and of
-sum_{kl,ab} D_kl D_ab (kl|ab) for a seeded rank-10 PSD density D.
+(--precision, default int8 = modular int8 MFMA with the library's default
+of 11 moduli), then the split results are compared with the FP64 ones:
+relative Frobenius and max-abs error of every X component and of int_2e_ao,
+and of sum_{kl,ab} D_kl D_ab (kl|ab) for a seeded rank-10 PSD density D.
+Then the BF16 leg: one warm-up and one timed build with the contractions as
+bf16 modular products (10 moduli, the library default), compared with the
+same FP64 build (--no-bf16 skips it).
 
 --workload batch64 times 64 independent systems of uneven sizes: 64
 sequential FP64 calls against one batched hipGraph launch in FP64, bf16
@@ -47,14 +50,11 @@ PEAK = {"fp64": 78.6, "int8": 5000.0, "bf16": 2500.0, "fp16": 2500.0, "bf16crt":
 
 
 BATCH_PRECISIONS = ("fp64", "bf16", "bf16crt", "int8")
-# int8 moduli of the timed build: the fewest that keep every output of the 20-nucleus workload
-# within the metric's 1e-10 (9: 3.4e-10, 10: 2.1e-11, 11: 1.3e-12 relative Frobenius error)
-BENCH_INT8_MODULI = 10
 # HBM-side bytes per stage-1 contraction launch on the default workload (n1 = 60000,
-# n2 = 200000, BENCH_INT8_MODULI moduli), from two rocprofv3 --pmc passes (FETCH_SIZE,
+# n2 = 200000, the library's default moduli), from two rocprofv3 --pmc passes (FETCH_SIZE,
 # WRITE_SIZE) scaled by a 1 GiB copy kernel measured in the same run
 # (profiles/cube20_traffic.txt); not measured for other shapes.
-MEASURED_TRAFFIC = {"int8": 279.5e9, "fp64": 17.83e12,
+MEASURED_TRAFFIC = {"int8": 273.7e9, "fp64": 17.83e12,
                     # the transform's GEMM launch (n_ao = n_mo = 200, n_grid1 = 60000) and the AO kernel
                     # (260000 points, n_ao = 200), profiles/ao_to_mo_nao200_kernel_stats.txt and
                     # profiles/aos_cube20_kernel_stats.txt
@@ -74,9 +74,10 @@ def parse():
     ap.add_argument("--precision", default="int8", choices=["fp64", "int8", "bf16", "fp16", "bf16crt", "fp16crt"],
                     help="int8 / bf16crt / fp16crt: modular (CRT) products; bf16 / fp16: digit slices")
     ap.add_argument("--slices", type=int, default=None,
-                    help="int8: moduli (default 10 here, the fewest that keep the workload within 1e-10: 2.1e-11; "
-                         "the library's own default is 11); bf16crt / fp16crt: moduli (default 10); bf16 / fp16: "
-                         "digit slices per operand (default 4)")
+                    help="int8: moduli (default: the library's, 11); bf16crt / fp16crt: moduli (default 10); "
+                         "bf16 / fp16: digit slices per operand (default 4)")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 leg (bf16 modular build beside the "
+                                                           "timed split build)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison build")
     ap.add_argument("--workload", default="cube20", choices=["cube20", "cube40", "batch64", "no", "mo", "ao"],
                     help="cube20: 20 nuclei, n_ao=200, X stored (the metric's workload); cube40: 40 nuclei, "
@@ -100,8 +101,6 @@ def parse():
                     help="first run one device-to-device copy of 1 GiB, a plain copy kernel of known size that "
                          "calibrates FETCH_SIZE / WRITE_SIZE counters read by rocprofv3 --pmc in the same run")
     args = ap.parse_args()
-    if args.slices is None and args.precision == "int8":
-        args.slices = BENCH_INT8_MODULI
     if args.steps is None:
         args.steps = 1 if args.workload == "cube40" else 5
     return args
@@ -112,22 +111,51 @@ def log(args, *a):
         print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sysfull, npts):
-    """The FP64 oracle (C + OpenMP + OpenBLAS) on the first npts r1 points
-    of the same system, timed on this host."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(synth, sysfull, npts, runs=3):
+    """The FP64 oracle (C + OpenMP + OpenBLAS) timed on this host, median of
+    `runs`: the 3-nucleus workload in full (n_ao = 24, n1 = 4000, n2 =
+    16000), and the first npts r1 points of the benchmarked system (all its
+    r2 points), whose rate is `value`; the full system's time at that rate
+    is reported as a linear extrapolation in n1."""
     from oracle import oracle
-    s = sysfull.subset_r1(npts)
     oracle.lib()
-    t0 = time.perf_counter()
-    X = oracle.int2_grad1_u12_ao(s)
-    I = oracle.int_2e_ao(s, X)
-    dt = time.perf_counter() - t0
-    del X, I
-    w = w_ref(s.n_ao, s.n1, s.n2)
-    return {"value": w / dt / 1e12, "unit": "TFLOP/s", "cores": oracle.threads(), "kind": "port",
-            "sample": f"FP64 oracle ({oracle.blas_name()} DGEMM) on the first {s.n1} of {sysfull.n1} r1 points, all "
-                      f"{s.n2} r2 points, n_ao={s.n_ao}: {dt:.2f} s; value = W_ref(sample) / s",
-            "seconds": dt}
+
+    def timed(s):
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            X = oracle.int2_grad1_u12_ao(s)
+            I = oracle.int_2e_ao(s, X)
+            ts.append(time.perf_counter() - t0)
+            del X, I
+        med = statistics.median(ts)
+        w = w_ref(s.n_ao, s.n1, s.n2)
+        return {"workload": s.shape_str(), "seconds": ts, "seconds_median": med, "w_ref_flops": w,
+                "tflops_median": w / med / 1e12}
+
+    c0 = timed(synth.triatomic())
+    sub = timed(sysfull.subset_r1(npts))
+    w_full = w_ref(sysfull.n_ao, sysfull.n1, sysfull.n2)
+    rate = sub["tflops_median"]
+    return {"value": rate, "unit": "TFLOP/s", "cores": oracle.threads(), "kind": "port",
+            "cpu_model": cpu_model(), "runs": runs,
+            "sample": f"FP64 oracle ({oracle.blas_name()} DGEMM, {oracle.threads()} threads) on the first {npts} of "
+                      f"{sysfull.n1} r1 points with all {sysfull.n2} r2 points, n_ao={sysfull.n_ao}: median "
+                      f"{sub['seconds_median']:.2f} s of {runs}; value = W_ref(sample) / s",
+            "seconds": sub["seconds_median"], "config1_sample": sub, "config0_full": c0,
+            "config1_full_s_extrapolated": w_full / (rate * 1e12),
+            "extrapolation_note": "linear in n1 from the sample's rate (the oracle's work is linear in n1)"}
 
 
 def build(cutc, ds, X, I, opt, stream):
@@ -633,7 +661,7 @@ def main():
     # tracer's HIP events on the build's stream
     exec1 = 2.0 * P * 4 * s.n1 * s.n2
     default_shape = (not streamed and (s.n1, s.n2) == (60000, 200000)
-                     and (args.precision == "fp64" or opt.slices == BENCH_INT8_MODULI))
+                     and (args.precision == "fp64" or (args.precision == "int8" and opt.slices == 11)))
     traffic = MEASURED_TRAFFIC[args.precision] if default_shape else None
     if args.precision == "fp64":
         launches = last["n_gemm1"]
@@ -703,6 +731,38 @@ def main():
         err = errors_vs_fp64()
         out["errors_vs_fp64"] = err
         out["meets_1e-10"] = err["max_rel_frob"] <= 1e-10
+        if not args.no_bf16 and args.precision not in ("bf16crt", "fp16crt"):
+            # the BF16 leg: the contractions as bf16 modular products at the library default
+            bopt = cutc.options("bf16crt")
+            try:
+                for _ in range(args.variant_warmup):
+                    build(cutc, ds, X, I, bopt, stream)
+                bms, bst = timed_builds(cutc, torch, ds, X, I, bopt, 1, stream)
+            except cutc.CutcError as exc:
+                cutc.release_workspace()
+                out["bf16_split"] = {"precision": "bf16crt", "error": str(exc)}
+            else:
+                cutc.release_workspace()
+                b = bst[-1]
+                berr = errors_vs_fp64()
+                bops = 2.0 * bopt.slices * P * 4 * s.n1 * s.n2
+                blaunch = b["n_gemm1"] * bopt.slices
+                out["bf16_split"] = {
+                    "precision": f"bf16crt x{bopt.slices} moduli (bf16 MFMA, FP32 accumulation kept exact, CRT "
+                                 "rebuild in FP64)",
+                    "build_s": bms / 1e3, "eff_tflops": W / (bms / 1e3) / 1e12,
+                    "speedup_vs_fp64": fp64["build_s"] / (bms / 1e3), "errors_vs_fp64": berr,
+                    "meets_1e-10": berr["max_rel_frob"] <= 1e-10,
+                    "stage_ms": {k: b[k] for k in ("ms_jastrow", "ms_integrands", "ms_split", "ms_gemm1",
+                                                   "ms_gemm2", "ms_symm")},
+                    "roofline": {"kernel": "crt16_gemm_kernel (step-1 contraction, one launch per modulus and r1 "
+                                           "chunk)", "bound": "mfma",
+                                 "achieved": bops / (b["ms_gemm1"] / 1e3) / 1e12, "peak": PEAK["bf16"],
+                                 "unit": "TFLOP/s", "frac": bops / (b["ms_gemm1"] / 1e3) / 1e12 / PEAK["bf16"],
+                                 "launches": blaunch, "avg_launch_ms": b["ms_gemm1"] / max(1, blaunch),
+                                 "work": f"2 x {bopt.slices} moduli x {P} pairs x 4 n1 x n2 bf16 flops"}}
+                log(args, f"bf16 leg: {bms / 1e3:.2f} s, {fp64['build_s'] / (bms / 1e3):.2f}x FP64, "
+                          f"max rel frob {berr['max_rel_frob']:.2e}")
         # further split variants against the same FP64 build (one warm-up build each)
         variants = []
         for v in filter(None, args.variants.split(",")):
@@ -752,7 +812,7 @@ def main():
                       "(freshly allocated) host memory; pipelined: each r1 chunk of X is downloaded while the build "
                       "runs; blocking: hipMemcpy after the build, as the reference")
         out["host_path"] = hp
-    out["cpu_baseline"] = cpu_baseline(s, args.cpu_sample) if args.cpu_sample > 0 else None
+    out["cpu_baseline"] = cpu_baseline(synth, s, args.cpu_sample) if args.cpu_sample > 0 else None
     print(json.dumps(out), flush=True)
 
 
