@@ -674,23 +674,42 @@ __device__ static inline float fmod_small(float x, float m, float inv_m)
     return __builtin_fmaf(-m, __builtin_rintf(x * inv_m), x);
 }
 
+// Window positions of the in-loop reductions of the 16-bit GEMM: in a window
+// of W steps x 32 MFMA slots, tile q (MFMAs at slots q and 16 + q of every
+// step) has its 16 accumulators reduced two per slot at absolute positions
+// 33 q + 18 .. 33 q + 25, i.e. from two slots after its second MFMA of step
+// q up to at most slot q - 7 of step q + 1, always before its next MFMA.
+// Returns the tile reduced at window position pos, or -1.
+__host__ __device__ constexpr int crt16_red_tile(int pos)
+{
+    return (pos >= 18 && (pos - 18) % 33 < 8 && (pos - 18) / 33 < 16) ? (pos - 18) / 33 : -1;
+}
+__host__ __device__ constexpr int crt16_red_elem(int pos) { return 2 * ((pos - 18) % 33); }
+
 // The 16-bit counterpart of crt_gemm_pk_kernel: the same block tile (512 A
 // rows x 128 B rows, 4 waves of 128 x 128), A register ring, B LDS ring and
 // load placement, with v_mfma_f32_32x32x16 (a 64-byte step is 32 K, two
 // MFMAs per tile) and FP32 accumulators.  Residues are at most 127 in
 // magnitude (moduli <= 255), so an accumulator reduced modulo m
-// (|x| <= 1.5 m) takes RW groups of D steps (32 D RW K, at most 1024) more
-// and stays below 1024 * 127^2 + 383 < 2^24: exact.  Every RW-th group ends
-// with the reduction of all accumulators.
-template <bool F16, int D, int L, int BAR, int RW>
+// (|x| <= 1.5 m) takes W more steps (32 W K, at most 1024) and stays below
+// 1024 * 127^2 + 383 < 2^24: exact.  The K loop runs in windows of W steps,
+// fully unrolled, so that the A ring (NA slots, loads L steps ahead), the B
+// LDS ring (DB stages) and the reductions all have compile-time slots: once
+// per window each tile is reduced in place (x - m rint(x / m), one fma of an
+// exact result), two accumulators per MFMA slot while the tile is idle
+// (crt16_red_tile), so that at most two values pass through VGPRs at a time
+// and nothing goes to scratch.  KT is a multiple of W.
+template <bool F16, int NA, int DB, int L, int BAR, int W>
 __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__restrict__ A,
                                                             const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                             int tiles_b, int64_t M, int64_t N, int mod,
                                                             double inv_mod, int8_t *__restrict__ C, int accum)
 {
-    static_assert(D * RW * 32 * 127 * 127 + 383 < (1 << 24), "exact FP32 sums");
-    static_assert(D >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
-    __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * PK_TILE];
+    static_assert(W * 32 * 127 * 127 + 383 < (1 << 24), "exact FP32 sums between reductions");
+    static_assert(W % NA == 0 && W % DB == 0 && W % BAR == 0, "compile-time ring slots in a window");
+    static_assert(NA >= L + 1 && DB >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
+    static_assert(33 * 15 + 25 < 32 * W, "every tile reduced in each window");
+    __shared__ __attribute__((aligned(16))) int8_t sB[DB][4 * PK_TILE];
 
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
     int ta, tb;
@@ -704,7 +723,7 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     const float mf = (float)mod, imf = (float)inv_mod;
     constexpr int WAIT_NEXT = 8 + 10 * (L - BAR - 1);
 
-    v4i ar[D][4][2];
+    v4i ar[NA][4][2];
     v4i bf0[4], bf1[4];
     v16f acc[4][4];
 #pragma unroll
@@ -713,79 +732,80 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
         for (int j = 0; j < 4; j++)
             acc[i][j] = v16f{};
 
-    auto issue = [&](auto slot_c, int64_t s) {
-        constexpr int slot = decltype(slot_c)::value;
-        const uint32_t so = (uint32_t)s * PK_TILE;
-        pk_dma16(Bw + so + lo, &sB[slot][wave * PK_TILE]);
-        pk_dma16(Bw + so + 1024 + lo, &sB[slot][wave * PK_TILE + 1024]);
+    // loads of step s (< KT) into A slot s % NA and LDS stage s % DB
+    auto issue = [&](auto s_c) {
+        constexpr int s = decltype(s_c)::value, sa = s % NA, sb = s % DB;
+        const uint32_t so = (uint32_t)(s < KT ? s : KT - 1) * PK_TILE;
+        pk_dma16(Bw + so + lo, &sB[sb][wave * PK_TILE]);
+        pk_dma16(Bw + so + 1024 + lo, &sB[sb][wave * PK_TILE + 1024]);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            ar[slot][i][0] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + lo));
-            ar[slot][i][1] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + 1024 + lo));
+            ar[sa][i][0] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + lo));
+            ar[sa][i][1] = *(const v4i *)(Aw + ((uint32_t)i * (uint32_t)rbs + so + 1024 + lo));
         }
         __builtin_amdgcn_sched_barrier(0);
     };
 
+    // the reductions placed at window position pos
+    auto reduce_at = [&](auto pos_c) {
+        constexpr int pos = decltype(pos_c)::value, q = crt16_red_tile(pos);
+        if constexpr (q >= 0) {
+            constexpr int r = crt16_red_elem(pos);
+            acc[q / 4][q % 4][r] = fmod_small(acc[q / 4][q % 4][r], mf, imf);
+            acc[q / 4][q % 4][r + 1] = fmod_small(acc[q / 4][q % 4][r + 1], mf, imf);
+        }
+    };
+
+    // step u of a window, K step k: during it every wave issues the 2 DMA
+    // and 8 A loads of step k + L (the last step's again past the end) and
+    // reads the c = 0 fragments of step k + 1
     auto step = [&](auto u_c, int64_t k) {
-        constexpr int u = decltype(u_c)::value;
-        constexpr int slot = (u + L) % D;
+        constexpr int u = decltype(u_c)::value, sa = u % NA, sb = u % DB;
+        constexpr int la = (u + L) % NA, lb = (u + L) % DB, nb = (u + 1) % DB;
         if constexpr (u % BAR == 0) {
             PK_WAIT_VM(WAIT_NEXT);
             __syncthreads();
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            bf1[j] = *(const v4i *)&sB[u][j * PK_TILE + 1024 + lo];
+            bf1[j] = *(const v4i *)&sB[sb][j * PK_TILE + 1024 + lo];
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t so = (uint32_t)(k + L < KT ? k + L : KT - 1) * PK_TILE;
         cutc_static_for<0, 16>([&](auto q_c) {
             constexpr int q = decltype(q_c)::value;
             constexpr int i = q / 4, j = q % 4;
-            acc[i][j] = mfma16<F16>(ar[u][i][0], bf0[j], acc[i][j]);
+            acc[i][j] = mfma16<F16>(ar[sa][i][0], bf0[j], acc[i][j]);
             if constexpr (q < 2) {
-                pk_dma16(Bw + so + q * 1024 + lo, &sB[slot][wave * PK_TILE + q * 1024]);
+                pk_dma16(Bw + so + q * 1024 + lo, &sB[lb][wave * PK_TILE + q * 1024]);
             } else if constexpr (q < 10) {
                 constexpr int ii = (q - 2) / 2, cc = (q - 2) % 2;
-                ar[slot][ii][cc] = *(const v4i *)(Aw + ((uint32_t)ii * (uint32_t)rbs + so + cc * 1024 + lo));
+                ar[la][ii][cc] = *(const v4i *)(Aw + ((uint32_t)ii * (uint32_t)rbs + so + cc * 1024 + lo));
             }
+            reduce_at(std::integral_constant<int, 32 * u + q>{});
             __builtin_amdgcn_sched_barrier(0);
         });
         cutc_static_for<0, 16>([&](auto q_c) {
             constexpr int q = decltype(q_c)::value;
             constexpr int i = q / 4, j = q % 4;
-            acc[i][j] = mfma16<F16>(ar[u][i][1], bf1[j], acc[i][j]);
+            acc[i][j] = mfma16<F16>(ar[sa][i][1], bf1[j], acc[i][j]);
             if constexpr (q < 4)
-                bf0[q] = *(const v4i *)&sB[(u + 1) % D][q * PK_TILE + lo];
+                bf0[q] = *(const v4i *)&sB[nb][q * PK_TILE + lo];
+            reduce_at(std::integral_constant<int, 32 * u + 16 + q>{});
             __builtin_amdgcn_sched_barrier(0);
         });
     };
 
     // prologue: steps 0 .. L - 1 in flight; DMA of the first BAR + 1 complete
-    cutc_static_for<0, L>([&](auto s_c) {
-        constexpr int s = decltype(s_c)::value;
-        issue(s_c, (s < KT ? s : KT - 1));
-    });
+    cutc_static_for<0, L>([&](auto s_c) { issue(s_c); });
     PK_WAIT_VM(WAIT_NEXT);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 4; j++)
         bf0[j] = *(const v4i *)&sB[0][j * PK_TILE + lo];
 
-    for (int64_t k0 = 0; k0 < KT; k0 += D * RW) {
-        const int64_t k1 = (KT - k0 < D * RW) ? KT : k0 + D * RW;
-        for (int64_t kb = k0; kb < k1; kb += D)
-            cutc_static_for<0, D>([&](auto u_c) { step(u_c, kb + decltype(u_c)::value); });
-        // one tile at a time, so that few accumulators sit in VGPRs (the
-        // loads of the next steps stay in flight)
-        cutc_static_for<0, 16>([&](auto q_c) {
-            constexpr int q = decltype(q_c)::value;
-#pragma unroll
-            for (int r = 0; r < 16; r++)
-                acc[q / 4][q % 4][r] = fmod_small(acc[q / 4][q % 4][r], mf, imf);
-            __builtin_amdgcn_sched_barrier(0);
-        });
-    }
+    for (int64_t kw = 0; kw < KT; kw += W)
+        cutc_static_for<0, W>([&](auto u_c) { step(u_c, kw + decltype(u_c)::value); });
     PK_WAIT_VM(0); // the repeated loads past the last step
 
     // result element r of lane l: A row 8 (r/4) + 4 (l/32) + r%4, B row l%32
@@ -808,6 +828,12 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     }
 }
 
+// 16-bit GEMM pipeline: A ring of 4 slots, B LDS ring of 6 stages, loads 3
+// steps ahead, a barrier every second step, windows of 24 steps (768 K; a
+// 16-bit plane's KT = kpad / 32 is a multiple of 24)
+#define CRT16_SHAPE 4, 6, 3, 2, 24
+static_assert(CUTC_PK_KALIGN / 32 % 24 == 0, "16-bit K steps per window");
+
 extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
                                 int64_t kpad, int8_t *C, int accumulate, hipStream_t st)
 {
@@ -824,18 +850,18 @@ extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, c
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
     const dim3 grid((unsigned)(ta * tb));
     for (int i = 0; i < c->n; i++) {
-        /* 6-stage rings, loads 4 steps ahead, a barrier every second step;
-         * 16-bit: a reduction every 5 groups of 6 steps (960 K) */
+        /* int8: 6-stage rings, loads 4 steps ahead, a barrier every second
+         * step; 16-bit: CRT16_SHAPE */
         int8_t *Ci = C + (int64_t)i * M * N;
         if (c->kind == CUTC_CRT_I8)
             hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040>), grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT,
                                (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
         else if (c->kind == CUTC_CRT_BF16)
-            hipLaunchKernelGGL((crt16_gemm_kernel<false, 6, 4, 2, 5>), grid, dim3(256), 0, st, A + i * pa, B + i * pb,
-                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
+            hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE>), grid, dim3(256), 0, st, A + i * pa,
+                               B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
         else
-            hipLaunchKernelGGL((crt16_gemm_kernel<true, 6, 4, 2, 5>), grid, dim3(256), 0, st, A + i * pa, B + i * pb,
-                               KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
+            hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE>), grid, dim3(256), 0, st, A + i * pa,
+                               B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
     }
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
