@@ -279,14 +279,17 @@ void oracle_jast_bh(int64_t ii0, int64_t n1_eff, int64_t jj0, int64_t n2_eff, in
 void oracle_long_range(int64_t jj0, int64_t n2_eff, int64_t ld, int64_t n2, int n_ao, const double *wr2,
                        const double *aos2, double *A)
 {
-#pragma omp parallel for schedule(static)
-    for (int64_t i2 = 0; i2 < n2_eff; i2++) {
-        double w = wr2[jj0 + i2];
-        for (int i = 0; i < n_ao; i++) {
-            double vi = aos2[jj0 + i2 + n2 * i];
-            for (int j = 0; j < n_ao; j++) {
-                double vj = aos2[jj0 + i2 + n2 * j];
-                A[i2 + ld * ((int64_t)j + (int64_t)n_ao * i)] = w * vi * vj;
+    /* every element is the same product w * vi * vj; the column loop is
+     * outside so that the stores run along i2, contiguous in A */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < n_ao; i++) {
+        for (int j = 0; j < n_ao; j++) {
+            double *col = A + ld * ((int64_t)j + (int64_t)n_ao * i);
+            const double *ai = aos2 + jj0 + n2 * (int64_t)i, *aj = aos2 + jj0 + n2 * (int64_t)j;
+            const double *w = wr2 + jj0;
+            for (int64_t i2 = 0; i2 < n2_eff; i2++) {
+                double vi = ai[i2], vj = aj[i2];
+                col[i2] = w[i2] * vi * vj;
             }
         }
     }
