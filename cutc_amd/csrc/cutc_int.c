@@ -225,11 +225,14 @@ static int program_from_device(int n_nuc, int size_bh, const double *dc, const i
     return rc;
 }
 
-/* A Jastrow program with its device copies (monomials and classes). */
+/* A Jastrow program with its device copies (monomials and classes).
+ * borrowed: host and device arrays live in static host memory and the
+ * device scratch (the allocation-free entry point) and are not freed. */
 typedef struct {
     cutc_jprog h;
     cutc_jmono *dmono;
     int *dcls;
+    int borrowed;
 } jprog_t;
 
 /* Device copies of a program's monomials and classes. */
@@ -265,17 +268,72 @@ static void dfree(void *p);
 
 static void jprog_free(jprog_t *jp)
 {
-    dfree(jp->dmono);
-    dfree(jp->dcls);
-    cutc_jast_program_free(&jp->h);
+    if (!jp->borrowed) {
+        dfree(jp->dmono);
+        dfree(jp->dcls);
+        cutc_jast_program_free(&jp->h);
+    }
     memset(jp, 0, sizeof *jp);
+}
+
+/* jprog_load without allocating: the Boys-Handy tables are copied into
+ * static host arrays, expanded by cutc_jast_program_static, and the
+ * program's monomials and classes copied to the front of the device
+ * region base (*used: bytes taken there, 256-byte multiples). */
+static double g_nat_c[CUTC_JPROG_STATIC_TERMS];
+static int g_nat_h[3 * CUTC_JPROG_STATIC_TERMS];
+
+static int jprog_load_static(int n_nuc, int size_bh, const double *dc, const int *dm, const int *dn,
+                             const int *dno, int need, char *base, size_t cap, size_t *used, jprog_t *jp)
+{
+    memset(jp, 0, sizeof *jp);
+    jp->borrowed = 1;
+    *used = 0;
+    if (!need)
+        return CUTC_OK;
+    const int64_t cnt = (int64_t)n_nuc * size_bh;
+    if (n_nuc < 0 || size_bh < 0 || cnt > CUTC_JPROG_STATIC_TERMS)
+        return CUTC_ERR_ARG;
+    if (cnt > 0 &&
+        (hipMemcpy(g_nat_c, dc, sizeof(double) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(g_nat_h, dm, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(g_nat_h + cnt, dn, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess ||
+         hipMemcpy(g_nat_h + 2 * cnt, dno, sizeof(int) * cnt, hipMemcpyDeviceToHost) != hipSuccess))
+        return CUTC_ERR_HIP;
+    int rc = check_exponents(g_nat_h, g_nat_h + cnt, g_nat_h + 2 * cnt, cnt);
+    if (rc == CUTC_OK)
+        rc = cutc_jast_program_static(n_nuc, size_bh, g_nat_c, g_nat_h, g_nat_h + cnt, g_nat_h + 2 * cnt, &jp->h);
+    if (rc != CUTC_OK)
+        return rc;
+    const size_t bm = (sizeof(cutc_jmono) * (jp->h.n_mono + 1) + 255) & ~(size_t)255;
+    const size_t bc = (sizeof(int) * (2 * jp->h.n_cls + 2) + 255) & ~(size_t)255;
+    if (bm + bc > cap)
+        return CUTC_ERR_NOMEM;
+    jp->dmono = (cutc_jmono *)base;
+    jp->dcls = (int *)(base + bm);
+    *used = bm + bc;
+    if (jp->h.n_mono > 0)
+        rc = hip_rc(hipMemcpy(jp->dmono, jp->h.mono, sizeof(cutc_jmono) * jp->h.n_mono, hipMemcpyHostToDevice));
+    if (rc == CUTC_OK && jp->h.n_cls > 0)
+        rc = hip_rc(hipMemcpy(jp->dcls, jp->h.cls, sizeof(int) * 2 * jp->h.n_cls, hipMemcpyHostToDevice));
+    return rc;
 }
 
 static size_t arena_size_current(void);
 
+/* Set by the allocation-free entry point while it runs (under the library
+ * lock): the stages carve their workspace from this device region instead
+ * of the arena, and the memory plan is made against its size. */
+static struct {
+    char *base;
+    size_t size;
+} g_ws_fixed;
+
 /* free device memory plus this device's cached workspace, minus a margin */
 static size_t device_budget(const cutc_options *o)
 {
+    if (g_ws_fixed.base)
+        return (o && o->mem_limit && o->mem_limit < g_ws_fixed.size) ? o->mem_limit : g_ws_fixed.size;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess)
         return 0;
@@ -328,6 +386,13 @@ static int ws_begin(ws_t *w, size_t need)
 {
     int dev = 0;
     memset(w, 0, sizeof *w);
+    if (g_ws_fixed.base) {
+        if (need > g_ws_fixed.size)
+            return CUTC_ERR_NOMEM;
+        w->base = g_ws_fixed.base;
+        w->cap = g_ws_fixed.size;
+        return CUTC_OK;
+    }
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
         return CUTC_ERR_HIP;
     if (g_arena[dev].size < need && g_ws_frozen)
@@ -1018,7 +1083,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
                        const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
                        const double *aos1, const double *aos2, const double *c_bh, const int *m_bh, const int *n_bh,
                        const int *o_bh, double *X, double *out, hipStream_t st, cutc_stats *stats,
-                       const tracer_t *hook)
+                       const tracer_t *hook, const jprog_t *pre)
 {
     int rc = check_sizes(n1, n2, n_ao, n_nuc, size_bh);
     if (rc)
@@ -1038,7 +1103,10 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
     if ((stages & 2) && n1 > 0 && n_ao > 0 && (!wr1 || !aos1))
         return CUTC_ERR_ARG;
     jprog_t prog;
-    rc = jprog_load(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, (stages & 1) && n1 > 0 && n2 > 0 && n_ao > 0, &prog);
+    if (pre)
+        prog = *pre;
+    else
+        rc = jprog_load(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, (stages & 1) && n1 > 0 && n2 > 0 && n_ao > 0, &prog);
     if (rc) {
         jprog_free(&prog);
         return rc;
@@ -1113,7 +1181,54 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
     }
     pthread_mutex_lock(&g_lock);
     int rc = device_path(opt, 3, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
-                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats, NULL);
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, stats, NULL, NULL);
+    pthread_mutex_unlock(&g_lock);
+    cutc_set_last_error(rc);
+    return rc;
+}
+
+/* cutc_int_device without allocating: the Jastrow program goes through
+ * static host arrays into the front of the device scratch (scratch.hip), the
+ * stages carve their workspace from the rest of it and plan their r1 chunks
+ * against its size, and no timing events are made.  Calls on one GPU share
+ * its scratch (serialised by the library lock; each returns after its work
+ * has completed). */
+int cutc_int_device_noalloc(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
+                            const double *r1, const double *wr1, const double *r2, const double *wr2,
+                            const double *rn, const double *aos_data1, const double *aos_data2, const double *c_bh,
+                            const int *m_bh, const int *n_bh, const int *o_bh, double *int2_grad1_u12_ao,
+                            double *int_2e_ao, void *stream)
+{
+    cutc_options o;
+    if (opt)
+        o = *opt;
+    else
+        cutc_default_options(&o);
+    o.verbose = 0; /* printing times needs events */
+    int rc = check_sizes(n_grid1, n_grid2, n_ao, n_nuc, size_bh);
+    const int need = n_grid1 > 0 && n_grid2 > 0 && n_ao > 0;
+    if (rc == CUTC_OK && need && (int64_t)n_nuc * size_bh > 0 && (!c_bh || !m_bh || !n_bh || !o_bh))
+        rc = CUTC_ERR_ARG;
+    if (rc != CUTC_OK) {
+        cutc_set_last_error(rc);
+        return rc;
+    }
+    pthread_mutex_lock(&g_lock);
+    char *base = NULL;
+    size_t bytes = 0, used = 0;
+    jprog_t jp;
+    memset(&jp, 0, sizeof jp);
+    rc = cutc_scratch((void **)&base, &bytes);
+    if (rc == CUTC_OK)
+        rc = jprog_load_static(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, need, base, bytes, &used, &jp);
+    if (rc == CUTC_OK) {
+        g_ws_fixed.base = base + used;
+        g_ws_fixed.size = bytes - used;
+        rc = device_path(&o, 3, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos_data1, aos_data2,
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, int_2e_ao, (hipStream_t)stream, NULL, NULL, &jp);
+        g_ws_fixed.base = NULL;
+        g_ws_fixed.size = 0;
+    }
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
     return rc;
@@ -1477,7 +1592,7 @@ void get_int2_grad1_u12_ao(cutc_dim3 dimGrid, cutc_dim3 dimBlock, int n_grid1, i
     opt.verbose = 1; /* the reference prints its plan and stage times */
     pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 1, n_grid1, n_grid2, n_ao, n_nuc, size_bh, r1, NULL, r2, wr2, rn, NULL, aos_data2,
-                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL, NULL);
+                         c_bh, m_bh, n_bh, o_bh, int2_grad1_u12_ao, NULL, NULL, NULL, NULL, NULL);
     release_arenas(); /* the reference keeps no device memory between calls */
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
@@ -1493,7 +1608,7 @@ void get_int_2e_ao(int n_grid1, int n_ao, double *wr1, double *aos_data1, double
     opt.verbose = 1; /* the reference prints its stage times */
     pthread_mutex_lock(&g_lock);
     int rc = device_path(&opt, 2, n_grid1, 0, n_ao, 0, 0, NULL, wr1, NULL, NULL, NULL, aos_data1, NULL, NULL, NULL,
-                         NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL, NULL);
+                         NULL, NULL, int2_grad1_u12_ao, tc_int_2e_ao, NULL, NULL, NULL, NULL);
     release_arenas();
     pthread_mutex_unlock(&g_lock);
     cutc_set_last_error(rc);
@@ -1634,7 +1749,7 @@ static int host_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nu
     TRYH(hipEventRecord(e1, NULL));
     const double t_in = now_ms();
     TRY(device_path(opt, 3, n1, n2, n_ao, n_nuc, size_bh, r1, wr1, r2, wr2, rn, aos1, aos2, c, m, n, o, X, out, NULL,
-                    stats, &hook));
+                    stats, &hook, NULL));
     const double t_built = now_ms();
     TRYH(hipEventRecord(e2, NULL));
     if (dl.xf) {

@@ -42,25 +42,19 @@ static int find_or_add_class(int *cls, int *n_cls, int j, int k, int cap)
     return (*n_cls)++;
 }
 
-int cutc_jast_program(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh,
-                      cutc_jprog *prog)
+/* raw monomials: A, i, j, k, is_r, coef */
+typedef struct {
+    int A, i, j, k, is_r;
+    double coef;
+} raw_t;
+
+/* The expansion into caller memory: raw (4 nt + 1 entries) and cls (2 (4 nt
+ * + 1) ints) are work space, prog->mono (4 nt + 1 entries) and prog->cls
+ * (2 (4 nt + 1) ints) must already point at the output arrays. */
+static int program_core(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh,
+                        const int *o_bh, raw_t *raw, int *cls, cutc_jprog *prog)
 {
-    memset(prog, 0, sizeof *prog);
     const int64_t nt = (int64_t)n_nuc * size_bh;
-    if (n_nuc < 0 || size_bh < 0)
-        return CUTC_ERR_ARG;
-    /* raw monomials: A, i, j, k, is_r, coef */
-    typedef struct {
-        int A, i, j, k, is_r;
-        double coef;
-    } raw_t;
-    raw_t *raw = malloc(sizeof(raw_t) * (4 * nt + 1));
-    int *cls = malloc(sizeof(int) * 2 * (4 * nt + 1));
-    if (!raw || !cls) {
-        free(raw);
-        free(cls);
-        return CUTC_ERR_NOMEM;
-    }
     int64_t nr = 0;
     int kmax = 0, n_cls = 0;
     for (int A = 0; A < n_nuc; A++) {
@@ -84,19 +78,8 @@ int cutc_jast_program(int n_nuc, int size_bh, const double *c_bh, const int *m_b
         if (raw[t].j == 0 && raw[t].k > kmax)
             kmax = raw[t].k;
     for (int64_t t = 0; t < nr; t++)
-        if (raw[t].j > 0 && find_or_add_class(cls, &n_cls, raw[t].j, raw[t].k, (int)(4 * nt + 1)) < 0) {
-            free(raw);
-            free(cls);
+        if (raw[t].j > 0 && find_or_add_class(cls, &n_cls, raw[t].j, raw[t].k, (int)(4 * nt + 1)) < 0)
             return CUTC_ERR_ARG;
-        }
-    prog->mono = malloc(sizeof(cutc_jmono) * (nr + 1));
-    prog->cls = malloc(sizeof(int) * 2 * (n_cls + 1));
-    if (!prog->mono || !prog->cls) {
-        free(raw);
-        free(cls);
-        cutc_jast_program_free(prog);
-        return CUTC_ERR_NOMEM;
-    }
     const int base = 4 * (kmax + 1);
     for (int64_t t = 0; t < nr; t++) {
         cutc_jmono *mo = &prog->mono[t];
@@ -114,9 +97,47 @@ int cutc_jast_program(int n_nuc, int size_bh, const double *c_bh, const int *m_b
     prog->n_cls = n_cls;
     prog->kmax = kmax;
     prog->ts = base + 4 * (int64_t)n_cls * n_nuc;
+    return CUTC_OK;
+}
+
+int cutc_jast_program(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh,
+                      cutc_jprog *prog)
+{
+    memset(prog, 0, sizeof *prog);
+    if (n_nuc < 0 || size_bh < 0)
+        return CUTC_ERR_ARG;
+    const int64_t nt = (int64_t)n_nuc * size_bh;
+    raw_t *raw = malloc(sizeof(raw_t) * (4 * nt + 1));
+    int *cls = malloc(sizeof(int) * 2 * (4 * nt + 1));
+    prog->mono = malloc(sizeof(cutc_jmono) * (4 * nt + 1));
+    prog->cls = malloc(sizeof(int) * 2 * (4 * nt + 1));
+    int rc = (raw && cls && prog->mono && prog->cls) ? CUTC_OK : CUTC_ERR_NOMEM;
+    if (rc == CUTC_OK)
+        rc = program_core(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, raw, cls, prog);
     free(raw);
     free(cls);
-    return CUTC_OK;
+    if (rc != CUTC_OK)
+        cutc_jast_program_free(prog);
+    return rc;
+}
+
+/* Host work space of cutc_jast_program_static: terms of up to
+ * CUTC_JPROG_STATIC_TERMS (n_nuc x size_bh).  One set for the process; the
+ * library's callers hold the library lock while they use it. */
+static raw_t g_raw[4 * CUTC_JPROG_STATIC_TERMS + 1];
+static int g_cls_work[2 * (4 * CUTC_JPROG_STATIC_TERMS + 1)];
+static cutc_jmono g_mono[4 * CUTC_JPROG_STATIC_TERMS + 1];
+static int g_cls[2 * (4 * CUTC_JPROG_STATIC_TERMS + 1)];
+
+int cutc_jast_program_static(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh,
+                             const int *o_bh, cutc_jprog *prog)
+{
+    memset(prog, 0, sizeof *prog);
+    if (n_nuc < 0 || size_bh < 0 || (int64_t)n_nuc * size_bh > CUTC_JPROG_STATIC_TERMS)
+        return CUTC_ERR_ARG;
+    prog->mono = g_mono;
+    prog->cls = g_cls;
+    return program_core(n_nuc, size_bh, c_bh, m_bh, n_bh, o_bh, g_raw, g_cls_work, prog);
 }
 
 void cutc_jast_program_free(cutc_jprog *prog)

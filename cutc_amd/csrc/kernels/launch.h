@@ -30,6 +30,13 @@ typedef struct cutc_jprog {
 int cutc_jast_program(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh, const int *o_bh,
                       cutc_jprog *prog);
 void cutc_jast_program_free(cutc_jprog *prog);
+/* The same into process-wide static host arrays (no allocation) for tables
+ * of at most CUTC_JPROG_STATIC_TERMS terms (n_nuc x size_bh), CUTC_ERR_ARG
+ * beyond; the program stays valid until the next call and is not freed.
+ * Callers hold the library lock. */
+#define CUTC_JPROG_STATIC_TERMS 4096
+int cutc_jast_program_static(int n_nuc, int size_bh, const double *c_bh, const int *m_bh, const int *n_bh,
+                             const int *o_bh, cutc_jprog *prog);
 
 /* Per-r1 table rows for r1 points [ii0, ii0 + n1) into tab (n1 x ts). */
 int cutc_launch_jast_prep(const double *r1, int64_t ii0, int64_t n1, const double *rn, int n_nuc,
@@ -64,6 +71,13 @@ int cutc_launch_nonherm_ref(int64_t n1, int n_ao, const double *wr1, const doubl
 int cutc_launch_aos(int64_t n, const double *r, const double *rn, int n_nuc, int n_ao, const int *nucl,
                     const int *power, const int *nprim, int pmax, const double *expo, const double *coef, double *out,
                     hipStream_t st);
+
+/* Device scratch of the allocation-free entry point (scratch.hip): the
+ * device address and size of this GPU's file-scope array, looked up once
+ * per device (with that device current) and kept.  Callers hold the
+ * library lock. */
+#define CUTC_SCRATCH_BYTES CUTC_NOALLOC_SCRATCH_BYTES
+int cutc_scratch(void **ptr, size_t *bytes);
 
 /* dst[0..n) = value. */
 int cutc_launch_fill(double *dst, int64_t n, double value, hipStream_t st);

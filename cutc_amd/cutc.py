@@ -112,8 +112,9 @@ HEADER_SYMBOLS = [
     "int_long_range", "int_short_range_herm", "int_short_range_nonherm", "trans_inplace", "cutc_get_last_error",
     "cutc_error_string", "cutc_default_options", "cutc_int_device", "cutc_int_ex", "cutc_make_plan",
     "cutc_release_workspace", "cutc_int_streamed_device", "cutc_int_batch_device", "cutc_no", "cutc_no_device",
-    "cutc_ao_to_mo", "cutc_ao_to_mo_device", "cutc_aos", "cutc_aos_device",
+    "cutc_ao_to_mo", "cutc_ao_to_mo_device", "cutc_aos", "cutc_aos_device", "cutc_int_device_noalloc",
 ]
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 _vp = ctypes.c_void_p
 _ci = ctypes.c_int
@@ -147,6 +148,8 @@ def lib():
         L.cutc_int_device.restype = _ci
         L.cutc_int_device.argtypes = ([ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [_vp]
                                       + [ctypes.POINTER(Stats)])
+        L.cutc_int_device_noalloc.restype = _ci
+        L.cutc_int_device_noalloc.argtypes = [ctypes.POINTER(Options)] + [_ci] * 5 + [_vp] * 13 + [_vp]
         L.get_int2_grad1_u12_ao.restype = None
         L.get_int2_grad1_u12_ao.argtypes = [Dim3, Dim3] + [_ci] * 5 + [_vp] * 10
         L.get_int_2e_ao.restype = None
@@ -341,6 +344,19 @@ def cutc_int_device(ds, X, I, opt: Options | None = None, stream=None):
                                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st))
     _check(rc, "cutc_int_device")
     return st.as_dict()
+
+
+def cutc_int_device_noalloc(ds, X, I, opt: Options | None = None, stream=None):
+    """cutc_int_device with every scratch buffer in the library's fixed
+    device array (no allocation); no statistics."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    rc = lib().cutc_int_device_noalloc(ctypes.byref(opt or options()), ds.n1, ds.n2, ds.n_ao, ds.n_nuc, ds.size_bh,
+                                       _tp(ds.r1), _tp(ds.wr1), _tp(ds.r2), _tp(ds.wr2), _tp(ds.rn), _tp(ds.aos1),
+                                       _tp(ds.aos2), _tp(ds.c_bh), _tp(ds.m_bh), _tp(ds.n_bh), _tp(ds.o_bh), _tp(X),
+                                       _tp(I), ctypes.c_void_p(stream.cuda_stream))
+    _check(rc, "cutc_int_device_noalloc")
 
 
 def cutc_int_streamed_device(ds, I, opt: Options | None = None, stream=None):

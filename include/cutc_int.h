@@ -162,6 +162,27 @@ int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao,
                     const int *n_bh, const int *o_bh, double *int2_grad1_u12_ao, double *int_2e_ao, void *stream,
                     cutc_stats *stats);
 
+/* cutc_int_device without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * layouts and results as cutc_int_device, minus the statistics.  All scratch
+ * (the Jastrow program and every stage's workspace) lives in a fixed device
+ * array of CUTC_NOALLOC_SCRATCH_BYTES (256 MiB) per GPU, held from the
+ * library's first use of that GPU to the end of the process; the r1 chunks
+ * are planned to fit it.  Returns CUTC_ERR_NOMEM when a single r1 row of a
+ * stage does not fit (e.g. n_ao = 24 with 16000 r2 points fits in every
+ * precision; n_ao = 200 with 200000 r2 points does not), CUTC_ERR_ARG beyond
+ * 4096 Jastrow terms (n_nuc x size_bh).  Calls on one GPU share the array:
+ * the library serialises its own calls and each returns after its work is
+ * complete, so calls must not be issued so that they overlap on one GPU by
+ * other means.  Nothing in the array needs to start at zero. */
+#define CUTC_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int cutc_int_device_noalloc(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
+                            const double *r1, const double *wr1, const double *r2, const double *wr2,
+                            const double *rn, const double *aos_data1, const double *aos_data2, const double *c_bh,
+                            const int *m_bh, const int *n_bh, const int *o_bh, double *int2_grad1_u12_ao,
+                            double *int_2e_ao, void *stream);
+
 /* cutc_int_device and cutc_int_ex keep their device workspace (one arena
  * per device, grown on demand) between calls, so repeated builds allocate
  * nothing; this frees it.  cutc_int, the get_* functions and the host-array
