@@ -63,7 +63,7 @@ int cutc_aos_device(int n_grid, int n_nuc, int n_ao, int prim_max, const double 
 int cutc_aos(int n_grid, int n_nuc, int n_ao, int prim_max, double *h_r, double *h_rn, int *h_ao_nucl,
              int *h_ao_power, int *h_ao_prim_num, double *h_ao_expo, double *h_ao_coef, double *h_aos_data)
 {
-    int rc = CUTC_OK;
+    int rc = CUTC_OK, locked = 0;
     void *d[8] = {NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
     cutc_xfer *xf = NULL;
     if (n_grid < 0 || n_nuc < 0 || n_ao < 0 || prim_max < 0 || n_ao > 65535) {
@@ -84,6 +84,9 @@ int cutc_aos(int n_grid, int n_nuc, int n_ao, int prim_max, double *h_r, double 
         const size_t sz[8] = {8 * 3 * ng, 8 * 3 * (size_t)n_nuc, 4 * na, 4 * 3 * na, 4 * na, 8 * pm * na,
                               8 * pm * na, 8 * ng * na * 4};
         const void *hin[7] = {h_r, h_rn, h_ao_nucl, h_ao_power, h_ao_prim_num, h_ao_expo, h_ao_coef};
+        /* the library lock from the first allocation to the last free */
+        cutc_internal_lock();
+        locked = 1;
         for (int i = 0; i < 8; i++)
             if (sz[i] > 0 && (rc = hrc(hipMalloc(&d[i], sz[i]))) != CUTC_OK)
                 goto out;
@@ -106,6 +109,8 @@ out:
     for (int i = 0; i < 8; i++)
         if (d[i])
             hipFree(d[i]);
+    if (locked)
+        cutc_internal_unlock();
     cutc_set_last_error(rc);
     return rc;
 }

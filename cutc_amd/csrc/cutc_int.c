@@ -502,6 +502,8 @@ static int hbuf_get(size_t need, char **base)
     return CUTC_OK;
 }
 
+void cutc_internal_release_workspace(void) { release_arenas(); }
+
 void cutc_release_workspace(void)
 {
     pthread_mutex_lock(&g_lock);

@@ -173,6 +173,8 @@ int cutc_ao_to_mo(int n_grid1, int n_ao, int n_mo, double *h_mo_l_coef, double *
             cutc_set_last_error(CUTC_ERR_ARG);
             return CUTC_ERR_ARG;
         }
+    /* the library lock from the first allocation to the last free */
+    cutc_internal_lock();
     for (int i = 0; i < 4; i++)
         if (sz[i] > 0 && (rc = hrc(hipMalloc((void **)&d[i], 8 * sz[i]))) != CUTC_OK)
             goto out;
@@ -187,7 +189,7 @@ int cutc_ao_to_mo(int n_grid1, int n_ao, int n_mo, double *h_mo_l_coef, double *
                           (rc = cutc_xfer_wait(xf, t)) != CUTC_OK))
             goto out;
     }
-    rc = cutc_ao_to_mo_device(n_grid1, n_ao, n_mo, d[0], d[1], d[2], d[3], 0, 0, NULL, NULL);
+    rc = mo_path(n_grid1, n_ao, n_mo, d[0], d[1], d[2], d[3], 0, 0, NULL, NULL);
     if (rc == CUTC_OK && sz[3] > 0) {
         rc = cutc_xfer_d2h(xf, h_int2_grad1_u12, d[3], 8 * sz[3], NULL);
         if (rc == CUTC_OK)
@@ -198,7 +200,8 @@ out:
     for (int i = 0; i < 4; i++)
         if (d[i])
             hipFree(d[i]);
-    cutc_release_workspace();
+    cutc_internal_release_workspace();
+    cutc_internal_unlock();
     cutc_set_last_error(rc);
     return rc;
 }

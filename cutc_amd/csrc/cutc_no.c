@@ -335,9 +335,12 @@ int cutc_no(int n_grid1, int n_mo, int ne_a, int ne_b, double *h_wr1, double *h_
     const double *h_in[4] = {h_wr1, h_mos_l_in_r, h_mos_r_in_r, h_int2_grad1_u12};
     for (int i = 0; i < 4; i++)
         if (sz[i] > 0 && !h_in[i]) {
-            rc = CUTC_ERR_ARG;
-            goto out;
+            cutc_set_last_error(CUTC_ERR_ARG);
+            return CUTC_ERR_ARG;
         }
+    /* the library lock from the first allocation to the last free: other
+     * callers plan their workspace against the same free memory */
+    cutc_internal_lock();
     for (int i = 0; i < 7; i++)
         if (sz[i] > 0 && (rc = hrc(hipMalloc((void **)&d[i], 8 * sz[i]))) != CUTC_OK)
             goto out;
@@ -365,10 +368,8 @@ int cutc_no(int n_grid1, int n_mo, int ne_a, int ne_b, double *h_wr1, double *h_
     }
     {
         no_hook hook = {wait_upload, &up};
-        cutc_internal_lock();
         rc = no_path(n_grid1, n_mo, ne_a, ne_b, d[0], d[1], d[2], d[3], d[4], d[5], d[6], 0, up.xf ? up.part : 0,
                      NULL, &stt, up.xf ? &hook : NULL);
-        cutc_internal_unlock();
     }
     if (rc == CUTC_OK) {
         double *h_out[3] = {h_no_2e, h_no_1e, h_no_0e};
@@ -389,7 +390,8 @@ out:
     for (int i = 0; i < 7; i++)
         if (d[i])
             hipFree(d[i]);
-    cutc_release_workspace();
+    cutc_internal_release_workspace();
+    cutc_internal_unlock();
     cutc_set_last_error(rc);
     return rc;
 }

@@ -13,6 +13,9 @@ void cutc_internal_unlock(void);
  * capped by mem_limit when that is nonzero */
 size_t cutc_internal_budget(size_t mem_limit);
 /* make the arena at least need bytes; *base: its start */
+/* free the cached workspace and host-array buffers of every device (the
+ * caller holds the library lock) */
+void cutc_internal_release_workspace(void);
 int cutc_internal_workspace(size_t need, char **base);
 /* host copy threads of the transfer pipeline (CUTC_HOST_COPY_THREADS or the
  * CPU count) */
