@@ -172,6 +172,177 @@ __global__ __launch_bounds__(JAST_THREADS) void jast_pair_kernel(
     }
 }
 
+// rbar(r12) = d / (1 + d) and its gradient (dx,dy,dz) / (d (1 + d)^2) with
+// one division: q = 1 / (d (1 + d)), then 1 / (1 + d) = d q and
+// 1 / (d (1 + d)^2) = d q^2; squared distances <= 1e-15 give zeros
+// (reference: src/jast_bh.cu:139-156).
+__device__ static inline void rbar_grad_1div(double dx, double dy, double dz, double &f, double &gx, double &gy,
+                                             double &gz)
+{
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > 1e-15) {
+        const double d = sqrt(d2);
+        const double q = 1.0 / (d * (1.0 + d));
+        const double t1 = d * q, t2 = t1 * q;
+        f = d * t1;
+        gx = t2 * dx;
+        gy = t2 * dy;
+        gz = t2 * dz;
+    } else {
+        f = gx = gy = gz = 0.0;
+    }
+}
+
+// The pair kernel with the block's r1 table rows staged in LDS and NMAX the
+// nucleus count rounded up to a multiple of 4.  The table entries are read
+// as LDS broadcasts, which bound the kernel, so every thread takes two r2
+// points (i2 and i2 + 256 of a 512-point block) and each entry read serves
+// both.  ONE: the program has a single class (j, k), whose f2A^j is formed
+// once per r2 point (with k = 0 that power is the weight itself).  A block
+// stages JAST_LDS_R1B rows at a time and walks JAST_ROWS_PER_BLOCK rows, so
+// that an r2 point's nucleus factors are formed once per that many pairs.
+// The sums are those of jast_pair_kernel, in the same order.
+#define JAST_LDS_R1B 16
+#define JAST_ROWS_PER_BLOCK 128
+#define JAST_PPT 2 /* r2 points per thread */
+template <int NMAX, bool ONE>
+__global__ __launch_bounds__(JAST_THREADS) void jast_pair_lds_kernel(
+    const double *__restrict__ r1, const double *__restrict__ r2, const double *__restrict__ rn, int n_nuc,
+    const double *__restrict__ tab, int64_t ts, const int *__restrict__ cls, int n_cls, int kmax,
+    double *__restrict__ G, int64_t ii0, int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, int64_t ldg,
+    int64_t plane)
+{
+    extern __shared__ double s_tab[]; // JAST_LDS_R1B rows x ts
+    const int j1 = ONE ? cls[0] : 0, k1 = ONE ? cls[1] : 0;
+    int64_t i2v[JAST_PPT];
+    bool act[JAST_PPT], live[JAST_PPT];
+    double px[JAST_PPT], py[JAST_PPT], pz[JAST_PPT];
+    double fj[JAST_PPT][NMAX]; // f2A, or f2A^j for a single class
+#pragma unroll
+    for (int h = 0; h < JAST_PPT; h++) {
+        i2v[h] = (int64_t)blockIdx.x * JAST_PPT * JAST_THREADS + h * JAST_THREADS + threadIdx.x;
+        act[h] = i2v[h] < n2_fill;
+        live[h] = i2v[h] < n2;
+        px[h] = py[h] = pz[h] = 0.0;
+        if (live[h]) {
+            const double *p = r2 + 3 * (jj0 + i2v[h]);
+            px[h] = p[0];
+            py[h] = p[1];
+            pz[h] = p[2];
+        }
+#pragma unroll
+        for (int A = 0; A < NMAX; A++) {
+            fj[h][A] = 0.0;
+            if (A < n_nuc) {
+                const double f = cutc_rbar(px[h] - rn[3 * A], py[h] - rn[3 * A + 1], pz[h] - rn[3 * A + 2]);
+                fj[h][A] = ONE ? ipow_u(f, j1) : f;
+            }
+        }
+    }
+    const int base = 4 * (kmax + 1);
+    for (int64_t i1_0 = (int64_t)blockIdx.y * JAST_LDS_R1B; i1_0 < n1; i1_0 += (int64_t)gridDim.y * JAST_LDS_R1B) {
+        const int nr = (n1 - i1_0 < JAST_LDS_R1B) ? (int)(n1 - i1_0) : JAST_LDS_R1B;
+        __syncthreads(); // the previous rows are no longer read
+        for (int64_t t = threadIdx.x; t < nr * ts; t += JAST_THREADS)
+            s_tab[t] = tab[i1_0 * ts + t];
+        __syncthreads();
+        for (int r = 0; r < nr; r++) {
+            const double *p = r1 + 3 * (ii0 + i1_0 + r);
+            const double *row = s_tab + r * ts;
+            double ux[JAST_PPT], uy[JAST_PPT], uz[JAST_PPT], s[JAST_PPT], g[JAST_PPT], gx[JAST_PPT], gy[JAST_PPT],
+                gz[JAST_PPT];
+#pragma unroll
+            for (int h = 0; h < JAST_PPT; h++) {
+                rbar_grad_1div(p[0] - px[h], p[1] - py[h], p[2] - pz[h], g[h], gx[h], gy[h], gz[h]);
+                ux[h] = uy[h] = uz[h] = s[h] = 0.0;
+                double gp = 1.0;
+                for (int k = 0; k <= kmax; k++) {
+                    ux[h] += gp * row[4 * k];
+                    uy[h] += gp * row[4 * k + 1];
+                    uz[h] += gp * row[4 * k + 2];
+                    s[h] += gp * row[4 * k + 3];
+                    gp *= g[h];
+                }
+            }
+            if constexpr (ONE) {
+                double gk[JAST_PPT];
+#pragma unroll
+                for (int h = 0; h < JAST_PPT; h++)
+                    gk[h] = ipow_u(g[h], k1);
+#pragma unroll
+                for (int A = 0; A < NMAX; A++) {
+                    if (A < n_nuc) {
+                        const double *qa = row + base + 4 * A;
+                        const double q0 = qa[0], q1 = qa[1], q2 = qa[2], q3 = qa[3];
+#pragma unroll
+                        for (int h = 0; h < JAST_PPT; h++) {
+                            const double w = k1 == 0 ? fj[h][A] : fj[h][A] * gk[h];
+                            ux[h] += w * q0;
+                            uy[h] += w * q1;
+                            uz[h] += w * q2;
+                            s[h] += w * q3;
+                        }
+                    }
+                }
+            } else {
+                for (int c = 0; c < n_cls; c++) {
+                    const int j = cls[2 * c], k = cls[2 * c + 1];
+                    double gk[JAST_PPT];
+#pragma unroll
+                    for (int h = 0; h < JAST_PPT; h++)
+                        gk[h] = ipow_u(g[h], k);
+                    const double *qc = row + base + 4 * (int64_t)c * n_nuc;
+#pragma unroll
+                    for (int A = 0; A < NMAX; A++) {
+                        if (A < n_nuc) {
+                            const double q0 = qc[4 * A], q1 = qc[4 * A + 1], q2 = qc[4 * A + 2], q3 = qc[4 * A + 3];
+#pragma unroll
+                            for (int h = 0; h < JAST_PPT; h++) {
+                                const double w = ipow_u(fj[h][A], j) * gk[h];
+                                ux[h] += w * q0;
+                                uy[h] += w * q1;
+                                uz[h] += w * q2;
+                                s[h] += w * q3;
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < JAST_PPT; h++) {
+                if (!act[h])
+                    continue;
+                double vx = ux[h] + s[h] * gx[h], vy = uy[h] + s[h] * gy[h], vz = uz[h] + s[h] * gz[h];
+                if (!live[h])
+                    vx = vy = vz = 0.0;
+                double *o = G + (i1_0 + r) * ldg + i2v[h];
+                o[0] = vx;
+                o[plane] = vy;
+                o[2 * plane] = vz;
+                o[3 * plane] = -0.5 * (vx * vx + vy * vy + vz * vz);
+            }
+        }
+    }
+}
+
+template <int NMAX, bool ONE>
+static int launch_pairs_lds(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab,
+                            int64_t ts, const int *cls, int n_cls, int kmax, double *G, int64_t ii0, int64_t n1,
+                            int64_t jj0, int64_t n2, int64_t n2_fill, int64_t ldg, int64_t plane, hipStream_t st)
+{
+    int64_t gx = (n2_fill + JAST_PPT * JAST_THREADS - 1) / (JAST_PPT * JAST_THREADS);
+    int64_t gy = (n1 + JAST_ROWS_PER_BLOCK - 1) / JAST_ROWS_PER_BLOCK;
+    if (gx > 2147483647)
+        return CUTC_ERR_ARG;
+    if (gy > 65535)
+        gy = 65535;
+    const size_t lds = sizeof(double) * JAST_LDS_R1B * (size_t)ts;
+    hipLaunchKernelGGL((jast_pair_lds_kernel<NMAX, ONE>), dim3((unsigned)gx, (unsigned)gy),
+                       dim3(JAST_THREADS), lds, st, r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, G, ii0, n1, jj0, n2,
+                       n2_fill, ldg, plane);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
 template <int NMAX, int R1B>
 static int launch_pairs(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab,
                         int64_t ts, const int *cls, int n_cls, int kmax, const double *f2tab, int64_t ld_f2,
@@ -197,6 +368,28 @@ extern "C" int cutc_launch_jast_pairs(const double *r1, const double *r2, const 
 {
     if (n1 <= 0 || n2_fill <= 0)
         return 0;
+    /* up to 32 nuclei and a table block that fits 48 KB of LDS: the staged
+     * kernel for the nucleus count rounded up to 4 */
+    if (n_nuc <= 32 && sizeof(double) * JAST_LDS_R1B * ts <= 48 * 1024) {
+        const int one = n_cls == 1;
+#define LL(NM)                                                                                                       \
+    return one ? launch_pairs_lds<NM, true>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, G, ii0, n1, jj0, n2,       \
+                                            n2_fill, ldg, plane, st)                                                  \
+               : launch_pairs_lds<NM, false>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, G, ii0, n1, jj0, n2,      \
+                                             n2_fill, ldg, plane, st)
+        switch ((n_nuc + 3) / 4) {
+        case 0:
+        case 1: LL(4);
+        case 2: LL(8);
+        case 3: LL(12);
+        case 4: LL(16);
+        case 5: LL(20);
+        case 6: LL(24);
+        case 7: LL(28);
+        default: LL(32);
+        }
+#undef LL
+    }
 #define LP(NM, RB)                                                                                                   \
     launch_pairs<NM, RB>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, f2tab, n2_fill, G, ii0, n1, jj0, n2, n2_fill,  \
                          ldg, plane, st)
