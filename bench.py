@@ -87,6 +87,9 @@ def parse():
                          "transform of int2_grad1_u12 (cutc_ao_to_mo), n_ao=n_mo=200, n_grid1=60000; ao: AO values and "
                          "gradients on both grids of the cube20 workload (cutc_aos_device), 3 primitives per AO")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="batch64: grid sizes times this factor")
+    ap.add_argument("--batch-streams", type=int, default=0,
+                    help="batch64: concurrent graph branches (0: the library default, 2); FP64 is also timed on one "
+                         "branch")
     ap.add_argument("--variants", default="",
                     help="further split builds compared with the same FP64 build, e.g. bf16:3,bf16:4,fp16:4")
     ap.add_argument("--variant-warmup", type=int, default=1, help="untimed builds before each variant's timed one")
@@ -224,8 +227,9 @@ def batch_bench(args, torch, cutc, synth):
                       "n_ao_range": [min(s.n_ao for s in sys_), max(s.n_ao for s in sys_)]},
            "w_ref_flops": W, "sequential_fp64_s": seq_s, "sequential_fp64_eff_tflops": W / seq_s / 1e12,
            "batch": {}}
-    for prec in BATCH_PRECISIONS:
-        opt = cutc.options(prec)
+    runs = [("fp64", 1)] + [(p, args.batch_streams) for p in BATCH_PRECISIONS]
+    for prec, nstreams in runs:
+        opt = cutc.options(prec, streams=nstreams)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         st = cutc.cutc_int_batch_device(dss, Ib, opt, stream)
@@ -238,10 +242,12 @@ def batch_bench(args, torch, cutc, synth):
         cutc.release_workspace()
         res = {"graph_launch_s": st["ms_launch"] / 1e3, "call_wall_s": wall, "setup_ms": st["ms_setup"],
                "autotune_ms": st["ms_tune"], "graph_nodes": st["n_nodes"], "systems_split_k": st["n_split"],
+               "branches": st["n_branches"],
                "eff_tflops": W / (st["ms_launch"] / 1e3) / 1e12, "speedup_vs_sequential_fp64": seq_s / wall,
                "max_rel_frob_vs_sequential_fp64": max(errs)}
         log(args, f"batch {prec}: {res}")
-        out["batch"][f"{prec}x{opt.slices}" if prec != "fp64" else prec] = res
+        label = f"{prec}x{opt.slices}" if prec != "fp64" else prec
+        out["batch"][label + ("_1branch" if nstreams == 1 and prec == "fp64" else "")] = res
     best = out["batch"]["fp64"]
     out.update({"value": best["eff_tflops"], "ms_per_step": best["graph_launch_s"] * 1e3, "dtype": "f64"})
     print(json.dumps(out), flush=True)

@@ -1407,12 +1407,23 @@ static int check_system(const cutc_system *y)
     return CUTC_OK;
 }
 
+/* executed contraction work of a system (FP64-equivalent flops), to balance
+ * the branches of a batch */
+static double system_work(const cutc_system *y)
+{
+    const double P = 0.5 * (double)y->n_ao * (y->n_ao + 1);
+    return 2.0 * P * 4.0 * y->n_grid1 * (double)y->n_grid2 + 2.0 * P * P * 4.0 * y->n_grid1;
+}
+
+#define CUTC_BATCH_MAX_BRANCHES 4
+
 static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys, hipStream_t st,
                       cutc_batch_stats *bst)
 {
     const cutc_options ropt = resolved_options(opt);
     opt = &ropt;
-    if (!valid_precision(opt) || nsys < 0 || (nsys > 0 && !sys))
+    if (!valid_precision(opt) || nsys < 0 || (nsys > 0 && !sys) || opt->streams < 0 ||
+        opt->streams > CUTC_BATCH_MAX_BRANCHES)
         return CUTC_ERR_ARG;
     for (int i = 0; i < nsys; i++)
         if (check_system(&sys[i]))
@@ -1421,24 +1432,28 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
         memset(bst, 0, sizeof *bst);
     if (nsys == 0)
         return CUTC_OK;
-    int rc = CUTC_OK;
+    int rc = CUTC_OK, nb = 1, capturing = 0;
     jprog_t *jp = (jprog_t *)calloc((size_t)nsys, sizeof(jprog_t));
     cutc_plan *pl = (cutc_plan *)calloc((size_t)nsys, sizeof(cutc_plan));
     int64_t *nc = (int64_t *)calloc((size_t)nsys, sizeof(int64_t));
     int *split2 = (int *)calloc((size_t)nsys, sizeof(int));
-    double *Xc = NULL;
+    int *branch = (int *)calloc((size_t)nsys, sizeof(int));
+    double *Xc[CUTC_BATCH_MAX_BRANCHES] = {NULL};
+    hipStream_t bs[CUTC_BATCH_MAX_BRANCHES] = {NULL};
+    hipEvent_t ef = NULL, ej[CUTC_BATCH_MAX_BRANCHES] = {NULL};
+    tracer_t tr[CUTC_BATCH_MAX_BRANCHES];
     hipGraph_t graph = NULL;
     hipGraphExec_t exec = NULL;
     hipEvent_t e0 = NULL, e1 = NULL;
-    hipStream_t cs = NULL;
-    if (!jp || !pl || !nc || !split2) {
+    memset(tr, 0, sizeof tr);
+    if (!jp || !pl || !nc || !split2 || !branch) {
         rc = CUTC_ERR_NOMEM;
         goto done;
     }
     {
-        /* X chunk: opt->n1_chunk when set, else as many r1 points as keep the
-         * shared chunk buffer within a quarter of the device budget (whole
-         * systems, usually: fewer chunks mean fewer step-2 passes) */
+        /* X chunk: opt->n1_chunk when set, else as many r1 points as keep a
+         * chunk buffer within a quarter of the device budget (whole systems,
+         * usually: fewer chunks mean fewer step-2 passes) */
         int64_t cmax = opt->n1_chunk;
         if (cmax <= 0) {
             const size_t quarter = device_budget(opt) / 4;
@@ -1463,11 +1478,9 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             if (b > xbytes)
                 xbytes = b;
         }
-        if (xbytes)
-            TRYH(hipMalloc((void **)&Xc, xbytes));
         const double t0 = now_ms();
-        /* plans against the memory left beside the chunk buffer; the workspace
-         * is grown once to the largest stage of the batch */
+        /* plans against the device budget; the workspace of a branch is the
+         * largest stage of the batch */
         const size_t budget = device_budget(opt);
         size_t need = 0;
         for (int i = 0; i < nsys; i++) {
@@ -1492,35 +1505,76 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             if (b > need)
                 need = b;
         }
+        need = ws_round(need + need / 64 + ((size_t)16 << 20));
+        /* branches: opt->streams (default 2), as many as the device holds a
+         * workspace region and a chunk buffer for */
+        nb = opt->streams > 0 ? opt->streams : 2;
+        if (nb > nsys)
+            nb = nsys;
+        while (nb > 1 && (size_t)nb * (need + ws_round(xbytes)) > budget)
+            nb--;
+        /* each system goes to the branch with the least work so far */
+        double load[CUTC_BATCH_MAX_BRANCHES] = {0.0};
+        for (int i = 0; i < nsys; i++) {
+            int b = 0;
+            for (int r = 1; r < nb; r++)
+                if (load[r] < load[b])
+                    b = r;
+            branch[i] = b;
+            load[b] += system_work(&sys[i]);
+        }
+        for (int r = 0; r < nb && xbytes; r++)
+            TRYH(hipMalloc((void **)&Xc[r], xbytes));
         ws_t ws;
-        TRY(ws_begin(&ws, need + need / 64 + ((size_t)16 << 20)));
-        /* record on a private stream (the legacy default stream cannot be
-         * captured), ordered after the caller's stream */
-        TRYH(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        TRY(ws_begin(&ws, (size_t)nb * need));
+        /* record on private streams (the legacy default stream cannot be
+         * captured), the first ordered after the caller's stream, the others
+         * forked from it inside the capture and joined back at the end */
+        for (int r = 0; r < nb; r++)
+            TRYH(hipStreamCreateWithFlags(&bs[r], hipStreamNonBlocking));
         TRYH(hipEventCreate(&e0));
         TRYH(hipEventCreate(&e1));
+        TRYH(hipEventCreateWithFlags(&ef, hipEventDisableTiming));
+        for (int r = 1; r < nb; r++)
+            TRYH(hipEventCreateWithFlags(&ej[r], hipEventDisableTiming));
         TRYH(hipEventRecord(e0, st));
-        TRYH(hipStreamWaitEvent(cs, e0, 0));
-        tracer_t tr;
-        tr_init(&tr, cs, 0);
-        tr.capture = 1;
+        TRYH(hipStreamWaitEvent(bs[0], e0, 0));
+        for (int r = 0; r < nb; r++) {
+            tr_init(&tr[r], bs[r], 0);
+            tr[r].capture = 1;
+        }
         g_ws_frozen = 1;
-        TRYH(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        TRYH(hipStreamBeginCapture(bs[0], hipStreamCaptureModeThreadLocal));
+        capturing = 1;
+        TRYH(hipEventRecord(ef, bs[0]));
+        for (int r = 1; r < nb; r++)
+            TRYH(hipStreamWaitEvent(bs[r], ef, 0));
         for (int i = 0; i < nsys && rc == CUTC_OK; i++) {
             const cutc_system *y = &sys[i];
+            const int b = branch[i];
             const int64_t nao2 = (int64_t)y->n_ao * y->n_ao;
             if (y->n_ao == 0)
                 continue;
+            /* the branch's own workspace region, fixed while its launches are recorded */
+            g_ws_fixed.base = ws.base + (size_t)b * need;
+            g_ws_fixed.size = need;
             if (y->n_grid1 == 0)
-                rc = cutc_launch_fill(y->int_2e_ao, nao2 * nao2, 0.0, cs);
+                rc = cutc_launch_fill(y->int_2e_ao, nao2 * nao2, 0.0, bs[b]);
             else
                 rc = streamed_chunks(opt, &pl[i], y->n_grid1, y->n_grid2, y->n_ao, y->n_nuc, y->r1, y->wr1, y->r2,
-                                     y->wr2, y->rn, y->aos_data1, y->aos_data2, &jp[i], y->int_2e_ao, Xc, nc[i],
-                                     split2[i], &tr, NULL);
+                                     y->wr2, y->rn, y->aos_data1, y->aos_data2, &jp[i], y->int_2e_ao, Xc[b], nc[i],
+                                     split2[i], &tr[b], NULL);
+            g_ws_fixed.base = NULL;
+            g_ws_fixed.size = 0;
         }
-        const int rc_end = hip_rc(hipStreamEndCapture(cs, &graph));
+        for (int r = 1; r < nb && rc == CUTC_OK; r++) {
+            rc = hip_rc(hipEventRecord(ej[r], bs[r]));
+            if (rc == CUTC_OK)
+                rc = hip_rc(hipStreamWaitEvent(bs[0], ej[r], 0));
+        }
+        const int rc_end = hip_rc(hipStreamEndCapture(bs[0], &graph));
+        capturing = 0;
         g_ws_frozen = 0;
-        tr_free(&tr);
         TRY(rc);
         TRY(rc_end);
         TRYH(hipGraphInstantiate(&exec, graph, NULL, NULL, 0));
@@ -1530,24 +1584,39 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             if (hipGraphGetNodes(graph, NULL, &nn) == hipSuccess)
                 bst->n_nodes = (int64_t)nn;
         }
-        TRYH(hipEventRecord(e0, cs));
-        TRYH(hipGraphLaunch(exec, cs));
-        TRYH(hipEventRecord(e1, cs));
-        TRYH(hipStreamSynchronize(cs));
+        TRYH(hipEventRecord(e0, bs[0]));
+        TRYH(hipGraphLaunch(exec, bs[0]));
+        TRYH(hipEventRecord(e1, bs[0]));
+        TRYH(hipStreamSynchronize(bs[0]));
         float ms = 0.0f;
         TRYH(hipEventElapsedTime(&ms, e0, e1));
         if (bst) {
             bst->ms_launch = ms;
             bst->n_systems = nsys;
+            bst->n_branches = nb;
             for (int i = 0; i < nsys; i++)
                 bst->n_split += split2[i] > 1;
         }
     }
 done:
     g_ws_frozen = 0;
-    if (cs) {
-        hipStreamSynchronize(cs);
-        hipStreamDestroy(cs);
+    g_ws_fixed.base = NULL;
+    g_ws_fixed.size = 0;
+    if (capturing) { /* an error while recording: end the capture, drop its graph */
+        hipGraph_t g = NULL;
+        if (hipStreamEndCapture(bs[0], &g) == hipSuccess && g)
+            hipGraphDestroy(g);
+        (void)hipGetLastError();
+    }
+    for (int r = 0; r < CUTC_BATCH_MAX_BRANCHES; r++) {
+        if (bs[r]) {
+            hipStreamSynchronize(bs[r]);
+            hipStreamDestroy(bs[r]);
+        }
+        if (ej[r])
+            hipEventDestroy(ej[r]);
+        tr_free(&tr[r]);
+        dfree(Xc[r]);
     }
     if (exec)
         hipGraphExecDestroy(exec);
@@ -1557,7 +1626,8 @@ done:
         hipEventDestroy(e0);
     if (e1)
         hipEventDestroy(e1);
-    dfree(Xc);
+    if (ef)
+        hipEventDestroy(ef);
     if (jp)
         for (int i = 0; i < nsys; i++)
             jprog_free(&jp[i]);
@@ -1565,6 +1635,7 @@ done:
     free(pl);
     free(nc);
     free(split2);
+    free(branch);
     return rc;
 }
 

@@ -50,7 +50,8 @@ class Dim3(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("slices", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("n1_chunk", ctypes.c_int), ("mem_limit", ctypes.c_size_t), ("host_copy", ctypes.c_int)]
+                ("n1_chunk", ctypes.c_int), ("mem_limit", ctypes.c_size_t), ("host_copy", ctypes.c_int),
+                ("streams", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -77,7 +78,8 @@ class System(ctypes.Structure):
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("ms_setup", ctypes.c_double), ("ms_tune", ctypes.c_double), ("ms_launch", ctypes.c_double),
-                ("n_nodes", ctypes.c_int64), ("n_systems", ctypes.c_int), ("n_split", ctypes.c_int)]
+                ("n_nodes", ctypes.c_int64), ("n_systems", ctypes.c_int), ("n_split", ctypes.c_int),
+                ("n_branches", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -204,7 +206,7 @@ def _check(rc, where):
         raise CutcError(rc, where)
 
 
-def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0, host_copy=0) -> Options:
+def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0, host_copy=0, streams=0) -> Options:
     """slices: bf16/fp16 digit slices per operand, or the number of moduli
     for int8 / bf16crt / fp16crt (None: the precision's default,
     DEFAULT_SLICES).  host_copy (host entry points): 0 = pipelined download
@@ -217,6 +219,7 @@ def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=
     o.n1_chunk = n1_chunk
     o.mem_limit = mem_limit
     o.host_copy = int(host_copy)
+    o.streams = int(streams)
     return o
 
 

@@ -124,6 +124,9 @@ typedef struct cutc_options {
                          each r1 chunk of X while the build runs (pinned staging,
                          copy stream, host copy threads); 1 = blocking copies
                          after the build, as the reference                      */
+    int streams;      /* cutc_int_batch_device: concurrent branches the systems
+                         are recorded on (1..4, 0 = 2; fewer when the device
+                         cannot hold a workspace and chunk buffer per branch) */
 } cutc_options;
 
 typedef struct cutc_stats {
@@ -225,13 +228,17 @@ typedef struct cutc_batch_stats {
     int64_t n_nodes;  /* nodes recorded in the graph */
     int n_systems;
     int n_split;      /* systems whose FP64 step-2 contraction the autotuner split along K */
+    int n_branches;   /* concurrent branches the systems were recorded on */
 } cutc_batch_stats;
 
 /* Batched build of independent systems of any sizes: each is built streamed
  * (X in r1 chunks of opt->n1_chunk points, by default whole systems as far
  * as a quarter of the device memory allows) into its own
  * int_2e_ao; the launches of the whole batch are recorded into one hipGraph
- * and launched once on `stream`.  Before recording, an autotuner times the
+ * and launched once on `stream`.  The systems are spread over opt->streams
+ * concurrent branches of the graph (default 2), each with its own workspace
+ * and chunk buffer, every system on the branch with the least work so far,
+ * so that one system's memory-bound kernels overlap another's contractions.  Before recording, an autotuner times the
  * K splits of the FP64 step-2 contraction for every small pair count (cached
  * per shape for the process).  No reference counterpart (the reference
  * builds one system per call). */

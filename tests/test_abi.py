@@ -175,6 +175,16 @@ def test_argument_errors_before_any_device_call():
     assert L.cutc_get_last_error() == cutc.CUTC_ERR_ARG
 
 
+def test_batch_branch_count_checked_before_any_device_call():
+    """cutc_int_batch_device refuses more than 4 concurrent branches (or a
+    negative count) with -2 before touching the device"""
+    L = cutc.lib()
+    st = cutc.BatchStats()
+    for streams in (5, -1):
+        o = cutc.options("fp64", streams=streams)
+        assert L.cutc_int_batch_device(ctypes.byref(o), 0, None, None, ctypes.byref(st)) == cutc.CUTC_ERR_ARG
+
+
 def test_one_hip_runtime_per_process():
     """The wrapper loads torch before the library, so a process that uses both
     maps a single libamdhip64 (torch asks for "libamdhip64.so", the library

@@ -30,17 +30,21 @@ def systems():
             synth.make_system(7, 55, 5000, 1800, 1105, name="b4")]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "int8", "bf16", "bf16crt"])
-def test_batch_matches_single_calls(torch, precision):
+@pytest.mark.parametrize("precision,streams", [("fp64", 1), ("fp64", 0), ("fp64", 3), ("int8", 0), ("bf16", 2),
+                                               ("bf16crt", 2)])
+def test_batch_matches_single_calls(torch, precision, streams):
+    """streams 0: the default of 2 concurrent graph branches"""
     sys_ = systems()
-    print(f"batch {precision}: {[s.shape_str() for s in sys_]}", flush=True)
+    print(f"batch {precision} streams={streams}: {[s.shape_str() for s in sys_]}", flush=True)
     dss = [cutc.DeviceSystem.upload(s) for s in sys_]
     Ib = [torch.full((s.n_ao,) * 4, float("nan"), dtype=torch.float64, device="cuda") for s in sys_]
-    opt = cutc.options(precision, n1_chunk=2048)
+    opt = cutc.options(precision, n1_chunk=2048, streams=streams)
     st = cutc.cutc_int_batch_device(dss, Ib, opt)
     torch.cuda.synchronize()
-    print(f"  graph nodes {st['n_nodes']}, launch {st['ms_launch']:.1f} ms, setup {st['ms_setup']:.1f} ms", flush=True)
+    print(f"  graph nodes {st['n_nodes']}, branches {st['n_branches']}, launch {st['ms_launch']:.1f} ms, "
+          f"setup {st['ms_setup']:.1f} ms", flush=True)
     assert st["n_systems"] == len(sys_) and st["n_nodes"] > len(sys_)
+    assert st["n_branches"] == (streams or 2)
     for s, ds, I in zip(sys_, dss, Ib):
         Is = torch.full_like(I, float("nan"))
         cutc.cutc_int_streamed_device(ds, Is, cutc.options(precision, n1_chunk=2048))
