@@ -125,7 +125,7 @@ typedef struct cutc_options {
                          copy stream, host copy threads); 1 = blocking copies
                          after the build, as the reference                      */
     int streams;      /* cutc_int_batch_device: concurrent branches the systems
-                         are recorded on (1..4, 0 = 2; fewer when the device
+                         are recorded on (1..4, 0 = 1; fewer when the device
                          cannot hold a workspace and chunk buffer per branch) */
 } cutc_options;
 
@@ -235,10 +235,11 @@ typedef struct cutc_batch_stats {
  * (X in r1 chunks of opt->n1_chunk points, by default whole systems as far
  * as a quarter of the device memory allows) into its own
  * int_2e_ao; the launches of the whole batch are recorded into one hipGraph
- * and launched once on `stream`.  The systems are spread over opt->streams
- * concurrent branches of the graph (default 2), each with its own workspace
- * and chunk buffer, every system on the branch with the least work so far,
- * so that one system's memory-bound kernels overlap another's contractions.  Before recording, an autotuner times the
+ * and launched once on `stream`.  opt->streams > 1 spreads the systems over
+ * that many concurrent branches of the graph, each with its own workspace
+ * and chunk buffer, every system on the branch with the least work so far
+ * (for batches of small systems, whose launches leave the device idle; the
+ * default is one branch).  Before recording, an autotuner times the
  * K splits of the FP64 step-2 contraction for every small pair count (cached
  * per shape for the process).  No reference counterpart (the reference
  * builds one system per call). */
