@@ -141,6 +141,41 @@ def test_triatomic_full_host_path(torch):
     assert abs(dd - ddo) <= TOL * abs(ddo)
 
 
+REFERENCE_LINES = [  # src/cutc_int.c, src/build_int2_grad1_u12_ao.cu, src/build_int_2e_ao.cu, in their order
+    " Computing TC-Integrals With CuTC",  # cutc_int.c
+    " free  memory = ", " total memory = ", "n_grid1_pass = ", "n_grid1_rest = ", "n1_pass = ",
+    "n_grid2_pass = ", "n_grid2_rest = ", "n2_pass = ",  # build_int2_grad1_u12_ao.cu:60-61,81-95
+    "Ellapsed time for DGEMM to build int2_grad1_u12_ao = ", "Ellapsed time for int_long_range kernel = ",
+    "Ellapsed time for cutc_int_bh kernel = ",  # build_int2_grad1_u12_ao.cu:342-344
+    "Ellapsed time for get_int2_grad1_u12_ao = ",  # cutc_int.c:153
+    "Ellapsed time for DGEMM to build int_2e_ao = ", "Ellapsed time for int_short_range_herm kernel = ",
+    "Ellapsed time for int_short_range_nonherm kernel = ", "Ellapsed time for addT = ",  # build_int_2e_ao.cu:139-142
+    "Ellapsed time for get_int_2e_ao = ",  # cutc_int.c:198
+    "Ellapsed time for Device <-> Host transf = ", "Ellapsed (total) time on GPU for cutc_int = ",  # cutc_int.c:230,236
+]
+
+
+def test_reference_stdout_lines(torch, capfd):
+    """cutc_int prints the reference's progress and stage-time lines, with
+    the same labels and in the same order, each with a number"""
+    s = synth.triatomic(n1=300, n2=2000)
+    capfd.readouterr()
+    cutc.cutc_int(s)
+    out = capfd.readouterr().out.splitlines()
+    pos = 0
+    for label in REFERENCE_LINES:
+        while pos < len(out) and not out[pos].startswith(label):
+            pos += 1
+        assert pos < len(out), f"missing or out of order: {label!r}\n" + "\n".join(out)
+        rest = out[pos][len(label):].split()
+        if rest:
+            float(rest[0])
+        pos += 1
+    # the four values of the chunk plan for one r1 chunk of all points
+    txt = "\n".join(out)
+    assert "n_grid1_pass = 300" in txt and "n1_pass = 1" in txt and "n_grid2_pass = 2000" in txt
+
+
 def test_chunking_and_reference_stage_functions(torch):
     s = with_terms(synth.make_system(6, 33, 300, 1500, 77, name="chunk"), ALL_TERMS)
     X0, I0, st0 = run_device(torch, s)
