@@ -28,7 +28,9 @@ for int_2e_ao and the D D contraction.
 value = effective TFLOP/s of the timed (split) build = W_ref / seconds, with
 the reference formulation's work W_ref = 8 n_ao^2 n1 n2 + 8 n_ao^4 n1 (full
 AO pairs, the metric's definition).  The build executes only the packed
-k <= l pairs: roofline fractions are computed on the executed work.
+k <= l pairs, and the modular stage-1 contraction only the (256-pair tile,
+K step) tiles with a non-zero residue (stats k_density): roofline
+fractions are computed on the executed work.
 
 Prints one JSON line.
 """
@@ -690,7 +692,10 @@ def main():
                 "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12}
     else:
         launches = last["n_gemm1"] * opt.slices  # one launch per modulus and r1 chunk
-        ops = 2.0 * opt.slices * P * 4 * s.n1 * s.n2
+        # the list-driven contraction runs k_density of the (256-pair tile,
+        # K step) tiles: the executed work is the dense work times that
+        dens = last["k_density"] if last["k_density"] > 0 else 1.0
+        ops = 2.0 * opt.slices * P * 4 * s.n1 * s.n2 * dens
         achieved = ops / (last["ms_gemm1"] / 1e3) / 1e12
         i8 = args.precision == "int8"
         roof = {"kernel": ("crt_gemm_pk_kernel" if i8 else "crt16_gemm_kernel") +
@@ -698,7 +703,9 @@ def main():
                 "bound": "mfma", "achieved": achieved, "peak": PEAK[args.precision],
                 "unit": "TOPS (int8)" if i8 else "TFLOP/s", "frac": achieved / PEAK[args.precision],
                 "traffic": traffic, "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
-                "work": f"2 x {opt.slices} moduli x {P} pairs x 4 n1 x n2 {'int8 ops' if i8 else 'flops'}",
+                "k_density": dens,
+                "work": f"2 x {opt.slices} moduli x {P} pairs x 4 n1 x n2 x k_density "
+                        f"{'int8 ops' if i8 else 'flops'} (the listed tiles)",
                 "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12,
                 "traffic_note": "HBM-side bytes per launch (L2 fills x2 + write-backs), profiles/cube20_traffic.txt"}
 
@@ -751,7 +758,8 @@ def main():
                 cutc.release_workspace()
                 b = bst[-1]
                 berr = errors_vs_fp64()
-                bops = 2.0 * bopt.slices * P * 4 * s.n1 * s.n2
+                bdens = b["k_density"] if b["k_density"] > 0 else 1.0
+                bops = 2.0 * bopt.slices * P * 4 * s.n1 * s.n2 * bdens
                 blaunch = b["n_gemm1"] * bopt.slices
                 out["bf16_split"] = {
                     "precision": f"bf16crt x{bopt.slices} moduli (bf16 MFMA, FP32 accumulation kept exact, CRT "
@@ -766,7 +774,9 @@ def main():
                                  "achieved": bops / (b["ms_gemm1"] / 1e3) / 1e12, "peak": PEAK["bf16"],
                                  "unit": "TFLOP/s", "frac": bops / (b["ms_gemm1"] / 1e3) / 1e12 / PEAK["bf16"],
                                  "launches": blaunch, "avg_launch_ms": b["ms_gemm1"] / max(1, blaunch),
-                                 "work": f"2 x {bopt.slices} moduli x {P} pairs x 4 n1 x n2 bf16 flops"}}
+                                 "k_density": bdens,
+                                 "work": f"2 x {bopt.slices} moduli x {P} pairs x 4 n1 x n2 x k_density bf16 "
+                                         "flops (the listed tiles)"}}
                 log(args, f"bf16 leg: {bms / 1e3:.2f} s, {fp64['build_s'] / (bms / 1e3):.2f}x FP64, "
                           f"max rel frob {berr['max_rel_frob']:.2e}")
         # further split variants against the same FP64 build (one warm-up build each)

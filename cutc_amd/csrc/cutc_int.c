@@ -597,7 +597,13 @@ done:
  * the modular (CRT) scheme, pair residues quantised on the fly from w2 and
  * chi, each G chunk quantised after the Jastrow block, one int8 (or bf16 /
  * fp16) GEMM per modulus, CRT rebuild scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
- * pairs and of each G chunk, one slice GEMM into FP64, scattered into X. */
+ * pairs and of each G chunk, one slice GEMM into FP64, scattered into X.
+ * Modular with one K pass: block-sparse.  The r2 points are reordered
+ * (nearest nucleus, then distance; the Jastrow, pair weights and AO values
+ * all use the permuted copies, so the contraction over r2 is unchanged),
+ * the pair rows sorted by the r2 segments where their quantised values are
+ * non-zero, and every GEMM runs, per 256-pair tile, only the K steps where
+ * that tile has a non-zero residue (opt->dense: all of them). */
 static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                         const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
                         const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats)
@@ -610,8 +616,12 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     const int nmod = opt->slices;
     int *pairs = NULL;
     double *G = NULL, *f2tab = NULL, *tab = NULL;
-    int *rexpA = NULL, *rexpG = NULL;
+    int *rexpA = NULL, *rexpG = NULL, *rexpN = NULL, *kcount = NULL;
     int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
+    uint8_t *occ = NULL;
+    uint32_t *klist = NULL, *svals = NULL;
+    uint64_t *skeys = NULL;
+    double *r2p = NULL, *wr2p = NULL, *aos2p = NULL;
     int rc = CUTC_OK, ev;
     const int slice = !is_crt(opt->precision), f16 = opt->precision == CUTC_PREC_FP16;
     const int emode = slice ? CUTC_ROWEXP_MAX : CUTC_ROWEXP_NORM;
@@ -629,14 +639,24 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     const size_t bG = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(4 * n1c) * ldk
                             : rb * nmod * CUTC_PK_ROWS_A(4 * n1c) * kc;
     const size_t bC = slice ? sizeof(double) * 4 * n1c * P : (size_t)nmod * 4 * n1c * P;
+    /* block-sparse state (the planner's sparse_bytes) */
+    const int sparse = !slice && kpass == 1 && n2 > 0 && P > 0;
+    const int kind = slice ? 0 : crt_kind(opt->precision);
+    const int64_t tiles = CUTC_PK_ROWS_B(P) / CUTC_SP_ROWS, KT = sparse ? ldk / CUTC_CRT_KSTEP(kind) : 0;
+    const int64_t nsort = sparse ? cutc_sort_len(n2 > P ? n2 : P) : 0;
 
     ws_t ws;
     {
-        const size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(int) * P) +
-                            ws_round(sizeof(int) * 4 * n1c) + ws_round(bA) + ws_round(bG) + ws_round(bC) +
-                            ws_round(sizeof(double) * 4 * n1c * ldk) +
-                            (n_nuc > 32 ? ws_round(sizeof(double) * n_nuc * ldk) : 0) +
-                            ws_round(sizeof(double) * (prog->ts * n1c + 1));
+        size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(int) * P) +
+                      ws_round(sizeof(int) * 4 * n1c) + ws_round(bA) + ws_round(bG) + ws_round(bC) +
+                      ws_round(sizeof(double) * 4 * n1c * ldk) +
+                      (n_nuc > 32 ? ws_round(sizeof(double) * n_nuc * ldk) : 0) +
+                      ws_round(sizeof(double) * (prog->ts * n1c + 1));
+        if (sparse)
+            need += ws_round(sizeof(double) * 3 * n2) + ws_round(sizeof(double) * n2) +
+                    ws_round(sizeof(double) * n_ao * n2) + ws_round(sizeof(uint64_t) * nsort) +
+                    ws_round(sizeof(uint32_t) * nsort) + ws_round(sizeof(int) * P) + ws_round(tiles * KT) +
+                    ws_round(sizeof(uint32_t) * tiles * KT) + ws_round(sizeof(int) * tiles);
         TRY(ws_begin(&ws, need));
         if (stats)
             stats->workspace_bytes = need;
@@ -651,16 +671,49 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     if (n_nuc > 32)
         WS_TAKE(f2tab, sizeof(double) * n_nuc * ldk);
     WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
+    if (sparse) {
+        WS_TAKE(r2p, sizeof(double) * 3 * n2);
+        WS_TAKE(wr2p, sizeof(double) * n2);
+        WS_TAKE(aos2p, sizeof(double) * n_ao * n2);
+        WS_TAKE(skeys, sizeof(uint64_t) * nsort);
+        WS_TAKE(svals, sizeof(uint32_t) * nsort);
+        WS_TAKE(rexpN, sizeof(int) * P);
+        WS_TAKE(occ, tiles * KT);
+        WS_TAKE(klist, sizeof(uint32_t) * tiles * KT);
+        WS_TAKE(kcount, sizeof(int) * tiles);
+    }
 
-    TRY(cutc_launch_pair_table(n_ao, pairs, st));
     ev = tr_begin(tr, CAT_SPLIT);
-    rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpA, st);
-    if (rc == CUTC_OK && kpass == 1)
-        rc = slice ? cutc_oz_slice_pairw(nmod, f16, wr2, aos2, n2, n_ao, ldk, CUTC_OZ_ROWS(P), rexpA, (uint16_t *)Apl,
-                                         st)
-                   : cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rexpA, Apl, st);
+    if (sparse) {
+        /* r2 order, then everything of stage 1 reads the permuted copies */
+        rc = cutc_order_r2(r2, wr2, aos2, n2, n_ao, rn, n_nuc, skeys, svals, nsort, r2p, wr2p, aos2p, st);
+        r2 = r2p;
+        wr2 = wr2p;
+        aos2 = aos2p;
+        if (rc == CUTC_OK)
+            rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpN, st);
+        if (rc == CUTC_OK)
+            rc = cutc_order_pairs(wr2, aos2, n2, n_ao, crt.bits, rexpN, skeys, svals, cutc_sort_len(P), pairs, rexpA,
+                                  st);
+        if (rc == CUTC_OK)
+            rc = cutc_zero_u8(occ, tiles * KT, st);
+        if (rc == CUTC_OK)
+            rc = cutc_crt_quant_pairw_pk_occ(&crt, wr2, aos2, n2, n2, n_ao, ldk, rexpA, pairs, Apl, occ, st);
+        if (rc == CUTC_OK)
+            rc = cutc_klists(occ, tiles, KT, CUTC_CRT_LIST_ALIGN(kind), klist, kcount, st);
+    } else {
+        rc = cutc_launch_pair_table(n_ao, pairs, st);
+        if (rc == CUTC_OK)
+            rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpA, st);
+        if (rc == CUTC_OK && kpass == 1)
+            rc = slice ? cutc_oz_slice_pairw(nmod, f16, wr2, aos2, n2, n_ao, ldk, CUTC_OZ_ROWS(P), rexpA,
+                                             (uint16_t *)Apl, st)
+                       : cutc_crt_quant_pairw_pk(&crt, wr2, aos2, n2, n_ao, ldk, rexpA, Apl, st);
+    }
     tr_end(tr, ev);
     TRY(rc);
+    const uint32_t *kl = (sparse && !opt->dense) ? klist : NULL;
+    const int *kc_ = (sparse && !opt->dense) ? kcount : NULL;
 
     int nchunks = 0;
     for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
@@ -685,7 +738,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             ev = tr_begin(tr, CAT_GEMM1);
             rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
                                       rexpA, (double *)Cpl, st)
-                       : cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, ldk, Cpl, 0, st);
+                       : cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, ldk, kl, kc_, Cpl, 0, st);
             tr_end(tr, ev);
             TRY(rc);
         }
@@ -720,6 +773,24 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     }
     if (!tr->capture)
         TRYH(hipStreamSynchronize(st));
+    if (stats && !slice && !tr->capture) {
+        /* executed fraction of the (256-pair tile, K step) tiles */
+        double d = 1.0;
+        if (kl && tiles > 0) {
+            int *h = (int *)malloc(sizeof(int) * tiles);
+            if (!h)
+                TRY(CUTC_ERR_NOMEM);
+            rc = hipMemcpy(h, kcount, sizeof(int) * tiles, hipMemcpyDeviceToHost) == hipSuccess ? CUTC_OK
+                                                                                              : CUTC_ERR_HIP;
+            double sum = 0.0;
+            for (int64_t i = 0; i < tiles; i++)
+                sum += h[i];
+            free(h);
+            TRY(rc);
+            d = sum / ((double)tiles * (double)KT);
+        }
+        stats->k_density = d;
+    }
 done:
     if (rc != CUTC_OK && !tr->capture)
         hipStreamSynchronize(st);

@@ -366,26 +366,41 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// pair rows p = (k, l) of A[p][r2] = (w chi_l) chi_k, computed on the fly
+// pair rows p = (k, l) of A[p][r2] = (w chi_l) chi_k, computed on the fly;
+// (k, l) from the table ptab (2 ints per row) or, without one, the packed
+// pair index.  occ (optional): byte [tb][kt] set to 1 when a row of the
+// CUTC_SP_ROWS-row B tile tb has a non-zero quantised value in K step kt (64 K for
+// int8, 32 for the 16-bit kinds); the caller clears it first.  A wave's 64
+// threads cover 32 rows x 32 K inside one step of either size, so one
+// ballot per wave decides the flag.
 template <int KIND>
 __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const double *__restrict__ wr2,
                                                              const double *__restrict__ aos2, int64_t ld2,
                                                              int64_t n2, int n_ao, int64_t KT, int64_t plane,
-                                                             const int *__restrict__ rexp, int8_t *planes)
+                                                             const int *__restrict__ rexp,
+                                                             const int *__restrict__ ptab, int8_t *planes,
+                                                             uint8_t *occ)
 {
     const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
     const int64_t rb = blockIdx.y, p = rb * 32 + r;
     const int64_t k0 = (int64_t)blockIdx.x * 128 + 16 * j;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     double q[16];
+    bool nz = false;
     if (p < P) {
         int k, l;
-        pair_of(p, k, l);
+        if (ptab) {
+            k = ptab[2 * p];
+            l = ptab[2 * p + 1];
+        } else {
+            pair_of(p, k, l);
+        }
         const int sh = c.bits - rexp[p];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int64_t i = k0 + e;
             q[e] = (i < n2) ? rint(ldexp(wr2[i] * aos2[i + ld2 * l] * aos2[i + ld2 * k], sh)) : 0.0;
+            nz |= q[e] != 0.0;
         }
     } else {
 #pragma unroll
@@ -393,6 +408,11 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
             q[e] = 0.0;
     }
     store_residues<KIND>(c, q, planes, rb, r, k0, KT, plane);
+    if (occ) {
+        const uint64_t b = __ballot(nz);
+        if (b && (threadIdx.x & 63) == 0)
+            occ[(rb / (CUTC_SP_ROWS / 32)) * KT + k0 / (KIND == CUTC_CRT_I8 ? 64 : 32)] = 1;
+    }
 }
 
 extern "C" int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2,
@@ -407,6 +427,13 @@ extern "C" int cutc_crt_quant_pairw_pk_range(const cutc_crt *c, const double *wr
                                              int64_t n2, int n_ao, int64_t kpad, const int *rexp, int8_t *planes,
                                              hipStream_t st)
 {
+    return cutc_crt_quant_pairw_pk_occ(c, wr2, aos2, ld2, n2, n_ao, kpad, rexp, NULL, planes, NULL, st);
+}
+
+extern "C" int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2, const double *aos2, int64_t ld2,
+                                           int64_t n2, int n_ao, int64_t kpad, const int *rexp, const int *ptab,
+                                           int8_t *planes, uint8_t *occ, hipStream_t st)
+{
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     if (P <= 0 || kpad <= 0)
         return 0;
@@ -416,7 +443,7 @@ extern "C" int cutc_crt_quant_pairw_pk_range(const cutc_crt *c, const double *wr
     const dim3 grid((unsigned)(kpad / 128), (unsigned)((P + 31) / 32));
 #define QP(K_)                                                                                                         \
     hipLaunchKernelGGL(quant_pairw_pk_kernel<K_>, grid, dim3(256), 0, st, *c, wr2, aos2, ld2, n2, n_ao,                \
-                       pk_steps(K_, kpad), plane, rexp, planes)
+                       pk_steps(K_, kpad), plane, rexp, ptab, planes, occ)
     CRT_KIND_DISPATCH(c->kind, QP);
 #undef QP
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
@@ -502,13 +529,15 @@ __device__ static inline void pk_dma16(const int8_t *g, int8_t *l)
     asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(g) : "memory");
 }
 
-// Block tile 512 A rows x 128 B rows, 4 waves of 128 x 128 (4 x 4 MFMA
-// 32x32x32 i8, 256 accumulator registers).  Each wave streams its own A rows
-// straight from the packed planes into a D-slot register ring (one coalesced
-// 1 KB load per fragment); the block's B rows, shared by the 4 waves, go
-// global -> LDS (pk_dma16) into a D-stage ring.  During K step k (64 bytes,
-// 32 MFMAs per wave) every wave issues the 2 DMA and 8 A loads of step k + L
-// and reads the c = 0 fragments of step k + 1.  A barrier opens every BAR-th
+// Block tile (512 / WB) A rows x 128 WB B rows, 4 waves of 128 x 128 (4 x 4
+// MFMA 32x32x32 i8, 256 accumulator registers); WB = 1: the waves split the
+// A rows, WB = 2: 2 x 2 waves, each A half read by two waves and each B half
+// shared by two.  Each wave streams its own A rows straight from the packed
+// planes into a D-slot register ring (one coalesced 1 KB load per
+// fragment); the block's B rows go global -> LDS (pk_dma16, WB row blocks
+// per wave) into a D-stage ring.  During K step k (64 bytes, 32 MFMAs per
+// wave) every wave issues the 2 WB DMA and 8 A loads of step k + L and
+// reads the c = 0 fragments of step k + 1.  A barrier opens every BAR-th
 // step, after the wait for the DMA of the next BAR steps.  The stage written
 // in step k belongs to step k + L - D, which every wave has finished once the
 // last barrier (at k - BAR + 1 or later) has passed: D >= L + BAR.  Loads
@@ -517,31 +546,53 @@ __device__ static inline void pk_dma16(const int8_t *g, int8_t *l)
 // SEG 64 2^14 < 2^31), each a full pipeline (prologue, steady state, drain),
 // so that the int32 sums can be shrunk modulo m between segments while no
 // fragment registers are live.
-template <int D, int L, int BAR, int SEG>
+// SP: the block runs only the kcount[tb] steps listed in klist[tb KT ..]
+// (a multiple of D; the steps whose B tile has a non-zero residue, padded
+// with a step whose B tile is zero), each segment's byte offsets staged in
+// LDS (sK) and read one step ahead, so the offset of step k + L is in a
+// register when step k issues its loads.  WB = 2 halves the A bytes per
+// MFMA, which bounds the list-driven GEMM: blocks of different B tiles run
+// different steps, so their A tiles are not shared through L2.
+template <int D, int L, int BAR, int SEG, bool SP, int WB>
 __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__restrict__ A,
                                                              const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                              int tiles_b, int64_t M, int64_t N, int mod,
-                                                             double inv_mod, int8_t *__restrict__ C, int accum)
+                                                             double inv_mod, int8_t *__restrict__ C, int accum,
+                                                             const uint32_t *__restrict__ klist,
+                                                             const int *__restrict__ kcount)
 {
     static_assert(SEG % D == 0 && D % BAR == 0 && (int64_t)SEG * 64 * 16384 < 2147483648LL, "segment");
-    static_assert(D >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
-    __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * PK_TILE];
+    static_assert(D >= L + BAR && L >= BAR + 1 && (8 + 2 * WB) * L <= 63 && (WB == 1 || WB == 2), "pipeline shape");
+    __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * WB * PK_TILE];
+    __shared__ uint32_t sK[SP ? SEG : 1];
 
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
     int ta, tb;
     cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
+    const int64_t NS = SP ? (int64_t)kcount[tb] : KT; // steps run
+    const uint32_t *kl = SP ? klist + (int64_t)tb * KT : nullptr;
+    int64_t kbase = 0;                                 // first step of the current segment
+    uint32_t so_nx = 0;                                // SP: offset of the step L + 1 ahead
+    auto off = [&](int64_t s) -> uint32_t {
+        if constexpr (SP)
+            return sK[s - kbase];
+        else
+            return (uint32_t)s * PK_TILE;
+    };
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t rbs = KT * PK_TILE; // bytes per row block
-    const int8_t *Aw = A + ((int64_t)ta * 16 + wave * 4) * rbs;
-    const int8_t *Bw = B + ((int64_t)tb * 4 + wave) * rbs;
+    const int wr = WB == 1 ? wave : wave >> 1, wc = WB == 1 ? 0 : wave & 1; // A and B quarter / half
+    const int8_t *Aw = A + ((int64_t)ta * (16 / WB) + wr * 4) * rbs;
+    const int8_t *Bw = B + ((int64_t)tb * 4 * WB + wave * WB) * rbs; // this wave's DMA row blocks
     const uint32_t lo = (uint32_t)lane * 16;
     const float inv_modf = (float)inv_mod;
     // at a barrier step b the DMA of step b + BAR (issued during step
-    // b + BAR - L, loads in issue order: 2 DMA then 8 A per step) is complete
-    // once no more than its own 8 A loads and the 10 loads of each of the
-    // L - BAR - 1 steps in between remain
-    constexpr int WAIT_NEXT = 8 + 10 * (L - BAR - 1);
+    // b + BAR - L, loads in issue order: 2 WB DMA then 8 A per step) is
+    // complete once no more than its own 8 A loads and the 8 + 2 WB loads of
+    // each of the L - BAR - 1 steps in between remain
+    constexpr int NLD = 8 + 2 * WB;
+    constexpr int WAIT_NEXT = 8 + NLD * (L - BAR - 1);
 
     v4i ar[D][4][2];
     v4i bf0[4], bf1[4];
@@ -554,9 +605,12 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
 
     auto issue = [&](auto slot_c, int64_t s) {
         constexpr int slot = decltype(slot_c)::value;
-        const uint32_t so = (uint32_t)s * PK_TILE;
-        pk_dma16(Bw + so + lo, &sB[slot][wave * PK_TILE]);
-        pk_dma16(Bw + so + 1024 + lo, &sB[slot][wave * PK_TILE + 1024]);
+        const uint32_t so = off(s);
+#pragma unroll
+        for (int d = 0; d < WB; d++) {
+            pk_dma16(Bw + ((uint32_t)d * (uint32_t)rbs + so + lo), &sB[slot][(wave * WB + d) * PK_TILE]);
+            pk_dma16(Bw + ((uint32_t)d * (uint32_t)rbs + so + 1024 + lo), &sB[slot][(wave * WB + d) * PK_TILE + 1024]);
+        }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -568,8 +622,8 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
 
     // One K step.  With one wave per SIMD every instruction issues in order,
     // so the loads are spread between the MFMAs (one after each of the first
-    // ten c = 0 MFMAs, the next step's fragment reads after the first four
-    // c = 1 MFMAs) instead of stalling the matrix pipe behind a burst.
+    // 8 + 2 WB c = 0 MFMAs, the next step's fragment reads after the first
+    // four c = 1 MFMAs) instead of stalling the matrix pipe behind a burst.
     auto step = [&](auto u_c, int64_t k, int64_t k1) {
         constexpr int u = decltype(u_c)::value;
         constexpr int slot = (u + L) % D;
@@ -579,17 +633,23 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            bf1[j] = *(const v4i *)&sB[u][j * PK_TILE + 1024 + lo];
+            bf1[j] = *(const v4i *)&sB[u][(wc * 4 + j) * PK_TILE + 1024 + lo];
         __builtin_amdgcn_sched_barrier(0);
-        const uint32_t so = (uint32_t)(k + L < k1 ? k + L : k1 - 1) * PK_TILE;
+        uint32_t so;
+        if constexpr (SP)
+            so = so_nx;
+        else
+            so = off(k + L < k1 ? k + L : k1 - 1);
         cutc_static_for<0, 16>([&](auto q_c) {
             constexpr int q = decltype(q_c)::value;
             constexpr int i = q / 4, j = q % 4;
             acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][0], bf0[j], acc[i][j], 0, 0, 0);
-            if constexpr (q < 2) {
-                pk_dma16(Bw + so + q * 1024 + lo, &sB[slot][wave * PK_TILE + q * 1024]);
-            } else if constexpr (q < 10) {
-                constexpr int ii = (q - 2) / 2, cc = (q - 2) % 2;
+            if constexpr (q < 2 * WB) {
+                constexpr int d = q / 2, h = q % 2;
+                pk_dma16(Bw + ((uint32_t)d * (uint32_t)rbs + so + h * 1024 + lo),
+                         &sB[slot][(wave * WB + d) * PK_TILE + h * 1024]);
+            } else if constexpr (q < NLD) {
+                constexpr int ii = (q - 2 * WB) / 2, cc = (q - 2 * WB) % 2;
                 ar[slot][ii][cc] = *(const v4i *)(Aw + ((uint32_t)ii * (uint32_t)rbs + so + cc * 1024 + lo));
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -598,14 +658,24 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
             constexpr int q = decltype(q_c)::value;
             constexpr int i = q / 4, j = q % 4;
             acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[u][i][1], bf1[j], acc[i][j], 0, 0, 0);
+            if constexpr (SP && q == 0)
+                so_nx = off(k + 1 + L < k1 ? k + 1 + L : k1 - 1);
             if constexpr (q < 4)
-                bf0[q] = *(const v4i *)&sB[(u + 1) % D][q * PK_TILE + lo];
+                bf0[q] = *(const v4i *)&sB[(u + 1) % D][(wc * 4 + q) * PK_TILE + lo];
             __builtin_amdgcn_sched_barrier(0);
         });
     };
 
-    for (int64_t k0 = 0; k0 < KT; k0 += SEG) {
-        const int64_t k1 = (KT - k0 < SEG) ? KT : k0 + SEG;
+    for (int64_t k0 = 0; k0 < NS; k0 += SEG) {
+        const int64_t k1 = (NS - k0 < SEG) ? NS : k0 + SEG;
+        if constexpr (SP) {
+            // the previous segment's last reads of sK are behind its drain barrier
+            kbase = k0;
+            for (int i = threadIdx.x; i < k1 - k0; i += 256)
+                sK[i] = kl[k0 + i] * PK_TILE;
+            __syncthreads();
+            so_nx = off(k0 + L < k1 ? k0 + L : k1 - 1);
+        }
         // prologue: steps k0 .. k0 + L - 1 in flight; DMA of the first BAR + 1 complete
         cutc_static_for<0, L>([&](auto s_c) {
             constexpr int s = decltype(s_c)::value;
@@ -615,7 +685,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            bf0[j] = *(const v4i *)&sB[0][j * PK_TILE + lo];
+            bf0[j] = *(const v4i *)&sB[0][(wc * 4 + j) * PK_TILE + lo];
 
         for (int64_t kb = k0; kb < k1; kb += D)
             cutc_static_for<0, D>([&](auto u_c) { step(u_c, kb + decltype(u_c)::value, k1); });
@@ -624,7 +694,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
         // segment's prologue reuses their LDS stages
         PK_WAIT_VM(0);
         __syncthreads();
-        if (k1 < KT) {
+        if (k1 < NS) {
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -641,12 +711,12 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     for (int i = 0; i < 4; i++) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int64_t col = (int64_t)tb * 128 + 32 * j + li;
+            const int64_t col = (int64_t)tb * 128 * WB + wc * 128 + 32 * j + li;
             if (col >= N)
                 continue;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int64_t row = (int64_t)ta * 512 + wave * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
+                const int64_t row = (int64_t)ta * (512 / WB) + wr * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
                 if (row < M)
                     C[row * N + col] =
                         (int8_t)reduce_mod(acc[i][j][r] + (accum ? (int)C[row * N + col] : 0), mod, inv_mod);
@@ -699,29 +769,50 @@ __host__ __device__ constexpr int crt16_red_elem(int pos) { return 2 * ((pos - 1
 // exact result), two accumulators per MFMA slot while the tile is idle
 // (crt16_red_tile), so that at most two values pass through VGPRs at a time
 // and nothing goes to scratch.  KT is a multiple of W.
-template <bool F16, int NA, int DB, int L, int BAR, int W>
+// SP: the kcount[tb] listed steps only (a multiple of W), as in
+// crt_gemm_pk_kernel, in segments of SEG steps (a multiple of W), each a
+// full pipeline with its byte offsets staged in LDS.  WB: the block shape,
+// as in crt_gemm_pk_kernel.
+template <bool F16, int NA, int DB, int L, int BAR, int W, bool SP, int WB>
 __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__restrict__ A,
                                                             const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                             int tiles_b, int64_t M, int64_t N, int mod,
-                                                            double inv_mod, int8_t *__restrict__ C, int accum)
+                                                            double inv_mod, int8_t *__restrict__ C, int accum,
+                                                            const uint32_t *__restrict__ klist,
+                                                            const int *__restrict__ kcount)
 {
+    constexpr int SEG = 85 * W;
     static_assert(W * 32 * 127 * 127 + 383 < (1 << 24), "exact FP32 sums between reductions");
     static_assert(W % NA == 0 && W % DB == 0 && W % BAR == 0, "compile-time ring slots in a window");
-    static_assert(NA >= L + 1 && DB >= L + BAR && L >= BAR + 1 && 10 * L <= 63, "pipeline shape");
+    static_assert(NA >= L + 1 && DB >= L + BAR && L >= BAR + 1 && (8 + 2 * WB) * L <= 63 && (WB == 1 || WB == 2),
+                  "pipeline shape");
     static_assert(33 * 15 + 25 < 32 * W, "every tile reduced in each window");
-    __shared__ __attribute__((aligned(16))) int8_t sB[DB][4 * PK_TILE];
+    __shared__ __attribute__((aligned(16))) int8_t sB[DB][4 * WB * PK_TILE];
+    __shared__ uint32_t sK[SP ? SEG : 1];
 
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
     int ta, tb;
     cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
+    const int64_t NS = SP ? (int64_t)kcount[tb] : KT; // steps run
+    const uint32_t *kl = SP ? klist + (int64_t)tb * KT : nullptr;
+    int64_t kbase = 0;                                 // first step of the current segment
+    uint32_t so_nx = 0;                                // SP: offset of the step L + 1 ahead
+    auto off = [&](int64_t s) -> uint32_t {
+        if constexpr (SP)
+            return sK[s - kbase];
+        else
+            return (uint32_t)s * PK_TILE;
+    };
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t rbs = KT * PK_TILE; // bytes per row block
-    const int8_t *Aw = A + ((int64_t)ta * 16 + wave * 4) * rbs;
-    const int8_t *Bw = B + ((int64_t)tb * 4 + wave) * rbs;
+    const int wr = WB == 1 ? wave : wave >> 1, wc = WB == 1 ? 0 : wave & 1; // A and B quarter / half
+    const int8_t *Aw = A + ((int64_t)ta * (16 / WB) + wr * 4) * rbs;
+    const int8_t *Bw = B + ((int64_t)tb * 4 * WB + wave * WB) * rbs; // this wave's DMA row blocks
     const uint32_t lo = (uint32_t)lane * 16;
     const float mf = (float)mod, imf = (float)inv_mod;
-    constexpr int WAIT_NEXT = 8 + 10 * (L - BAR - 1);
+    constexpr int NLD = 8 + 2 * WB; // loads per step: 2 WB DMA, 8 A
+    constexpr int WAIT_NEXT = 8 + NLD * (L - BAR - 1);
 
     v4i ar[NA][4][2];
     v4i bf0[4], bf1[4];
@@ -732,12 +823,16 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
         for (int j = 0; j < 4; j++)
             acc[i][j] = v16f{};
 
-    // loads of step s (< KT) into A slot s % NA and LDS stage s % DB
-    auto issue = [&](auto s_c) {
+    // loads of segment step s (the last step's again past k1) into A slot
+    // s % NA and LDS stage s % DB
+    auto issue = [&](auto s_c, int64_t k0, int64_t k1) {
         constexpr int s = decltype(s_c)::value, sa = s % NA, sb = s % DB;
-        const uint32_t so = (uint32_t)(s < KT ? s : KT - 1) * PK_TILE;
-        pk_dma16(Bw + so + lo, &sB[sb][wave * PK_TILE]);
-        pk_dma16(Bw + so + 1024 + lo, &sB[sb][wave * PK_TILE + 1024]);
+        const uint32_t so = off(k0 + s < k1 ? k0 + s : k1 - 1);
+#pragma unroll
+        for (int d = 0; d < WB; d++) {
+            pk_dma16(Bw + ((uint32_t)d * (uint32_t)rbs + so + lo), &sB[sb][(wave * WB + d) * PK_TILE]);
+            pk_dma16(Bw + ((uint32_t)d * (uint32_t)rbs + so + 1024 + lo), &sB[sb][(wave * WB + d) * PK_TILE + 1024]);
+        }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -760,7 +855,7 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     // step u of a window, K step k: during it every wave issues the 2 DMA
     // and 8 A loads of step k + L (the last step's again past the end) and
     // reads the c = 0 fragments of step k + 1
-    auto step = [&](auto u_c, int64_t k) {
+    auto step = [&](auto u_c, int64_t k, int64_t k1) {
         constexpr int u = decltype(u_c)::value, sa = u % NA, sb = u % DB;
         constexpr int la = (u + L) % NA, lb = (u + L) % DB, nb = (u + 1) % DB;
         if constexpr (u % BAR == 0) {
@@ -769,17 +864,23 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            bf1[j] = *(const v4i *)&sB[sb][j * PK_TILE + 1024 + lo];
+            bf1[j] = *(const v4i *)&sB[sb][(wc * 4 + j) * PK_TILE + 1024 + lo];
         __builtin_amdgcn_sched_barrier(0);
-        const uint32_t so = (uint32_t)(k + L < KT ? k + L : KT - 1) * PK_TILE;
+        uint32_t so;
+        if constexpr (SP)
+            so = so_nx;
+        else
+            so = off(k + L < k1 ? k + L : k1 - 1);
         cutc_static_for<0, 16>([&](auto q_c) {
             constexpr int q = decltype(q_c)::value;
             constexpr int i = q / 4, j = q % 4;
             acc[i][j] = mfma16<F16>(ar[sa][i][0], bf0[j], acc[i][j]);
-            if constexpr (q < 2) {
-                pk_dma16(Bw + so + q * 1024 + lo, &sB[lb][wave * PK_TILE + q * 1024]);
-            } else if constexpr (q < 10) {
-                constexpr int ii = (q - 2) / 2, cc = (q - 2) % 2;
+            if constexpr (q < 2 * WB) {
+                constexpr int d = q / 2, h = q % 2;
+                pk_dma16(Bw + ((uint32_t)d * (uint32_t)rbs + so + h * 1024 + lo),
+                         &sB[lb][(wave * WB + d) * PK_TILE + h * 1024]);
+            } else if constexpr (q < NLD) {
+                constexpr int ii = (q - 2 * WB) / 2, cc = (q - 2 * WB) % 2;
                 ar[la][ii][cc] = *(const v4i *)(Aw + ((uint32_t)ii * (uint32_t)rbs + so + cc * 1024 + lo));
             }
             reduce_at(std::integral_constant<int, 32 * u + q>{});
@@ -789,24 +890,41 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
             constexpr int q = decltype(q_c)::value;
             constexpr int i = q / 4, j = q % 4;
             acc[i][j] = mfma16<F16>(ar[sa][i][1], bf1[j], acc[i][j]);
+            if constexpr (SP && q == 0)
+                so_nx = off(k + 1 + L < k1 ? k + 1 + L : k1 - 1);
             if constexpr (q < 4)
-                bf0[q] = *(const v4i *)&sB[nb][q * PK_TILE + lo];
+                bf0[q] = *(const v4i *)&sB[nb][(wc * 4 + q) * PK_TILE + lo];
             reduce_at(std::integral_constant<int, 32 * u + 16 + q>{});
             __builtin_amdgcn_sched_barrier(0);
         });
     };
 
-    // prologue: steps 0 .. L - 1 in flight; DMA of the first BAR + 1 complete
-    cutc_static_for<0, L>([&](auto s_c) { issue(s_c); });
-    PK_WAIT_VM(WAIT_NEXT);
-    __syncthreads();
+    // dense: one segment of KT steps; SP: segments of SEG listed steps
+    const int64_t segl = SP ? SEG : NS;
+    for (int64_t k0 = 0; k0 < NS; k0 += segl) {
+        const int64_t k1 = (NS - k0 < segl) ? NS : k0 + segl;
+        if constexpr (SP) {
+            // the previous segment's last reads of sK are behind its drain barrier
+            kbase = k0;
+            for (int i = threadIdx.x; i < k1 - k0; i += 256)
+                sK[i] = kl[k0 + i] * PK_TILE;
+            __syncthreads();
+            so_nx = off(k0 + L < k1 ? k0 + L : k1 - 1);
+        }
+        // prologue: steps k0 .. k0 + L - 1 in flight; DMA of the first BAR + 1 complete
+        cutc_static_for<0, L>([&](auto s_c) { issue(s_c, k0, k1); });
+        PK_WAIT_VM(WAIT_NEXT);
+        __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-        bf0[j] = *(const v4i *)&sB[0][j * PK_TILE + lo];
+        for (int j = 0; j < 4; j++)
+            bf0[j] = *(const v4i *)&sB[0][(wc * 4 + j) * PK_TILE + lo];
 
-    for (int64_t kw = 0; kw < KT; kw += W)
-        cutc_static_for<0, W>([&](auto u_c) { step(u_c, kw + decltype(u_c)::value); });
-    PK_WAIT_VM(0); // the repeated loads past the last step
+        for (int64_t kw = k0; kw < k1; kw += W)
+            cutc_static_for<0, W>([&](auto u_c) { step(u_c, kw + decltype(u_c)::value, k1); });
+        PK_WAIT_VM(0); // the repeated loads past the last step
+        if constexpr (SP)
+            __syncthreads(); // before the next segment's offsets and LDS stages
+    }
 
     // result element r of lane l: A row 8 (r/4) + 4 (l/32) + r%4, B row l%32
     const int li = lane & 31, lg = lane >> 5;
@@ -814,12 +932,12 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     for (int i = 0; i < 4; i++) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int64_t col = (int64_t)tb * 128 + 32 * j + li;
+            const int64_t col = (int64_t)tb * 128 * WB + wc * 128 + 32 * j + li;
             if (col >= N)
                 continue;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int64_t row = (int64_t)ta * 512 + wave * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
+                const int64_t row = (int64_t)ta * (512 / WB) + wr * 128 + 32 * i + 8 * (r >> 2) + 4 * lg + (r & 3);
                 if (row < M)
                     C[row * N + col] = (int8_t)reduce_mod(
                         (int)fmod_small(acc[i][j][r], mf, imf) + (accum ? (int)C[row * N + col] : 0), mod, inv_mod);
@@ -837,14 +955,24 @@ static_assert(CUTC_PK_KALIGN / 32 % 24 == 0, "16-bit K steps per window");
 extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
                                 int64_t kpad, int8_t *C, int accumulate, hipStream_t st)
 {
+    return cutc_crt_gemm_pk_list(c, A, M, B, N, kpad, NULL, NULL, C, accumulate, st);
+}
+
+extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
+                                     int64_t kpad, const uint32_t *klist, const int *kcount, int8_t *C,
+                                     int accumulate, hipStream_t st)
+{
     if (M <= 0 || N <= 0)
         return 0;
-    if (kpad <= 0 || kpad % CUTC_PK_KALIGN || c->kind < CUTC_CRT_I8 || c->kind > CUTC_CRT_FP16)
+    if (kpad <= 0 || kpad % CUTC_PK_KALIGN || c->kind < CUTC_CRT_I8 || c->kind > CUTC_CRT_FP16 ||
+        (!klist) != (!kcount))
         return CUTC_ERR_ARG;
     const int64_t KT = pk_steps(c->kind, kpad), rb = CUTC_CRT_RBYTES(c->kind);
     if (4 * KT * PK_TILE > 4294967295LL) /* 32-bit offsets inside a wave's four row blocks */
         return CUTC_ERR_ARG;
-    const int64_t ta = CUTC_PK_ROWS_A(M) / 512, tb = CUTC_PK_ROWS_B(N) / 128;
+    const int sp = klist != NULL;
+    /* list-driven: 256 x 256 blocks (list per CUTC_SP_ROWS B rows); dense: 512 x 128 */
+    const int64_t ta = CUTC_PK_ROWS_A(M) / (sp ? 256 : 512), tb = CUTC_PK_ROWS_B(N) / (sp ? CUTC_SP_ROWS : 128);
     if (ta * tb > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
@@ -853,15 +981,26 @@ extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, c
         /* int8: 6-stage rings, loads 4 steps ahead, a barrier every second
          * step; 16-bit: CRT16_SHAPE */
         int8_t *Ci = C + (int64_t)i * M * N;
-        if (c->kind == CUTC_CRT_I8)
-            hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040>), grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT,
-                               (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
-        else if (c->kind == CUTC_CRT_BF16)
-            hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE>), grid, dim3(256), 0, st, A + i * pa,
-                               B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
-        else
-            hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE>), grid, dim3(256), 0, st, A + i * pa,
-                               B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate);
+#define GEMM_ARGS                                                                                                      \
+    grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate, \
+        klist, kcount
+        if (c->kind == CUTC_CRT_I8) {
+            if (sp)
+                hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040, true, 2>), GEMM_ARGS);
+            else
+                hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040, false, 1>), GEMM_ARGS);
+        } else if (c->kind == CUTC_CRT_BF16) {
+            if (sp)
+                hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE, true, 2>), GEMM_ARGS);
+            else
+                hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE, false, 1>), GEMM_ARGS);
+        } else {
+            if (sp)
+                hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE, true, 2>), GEMM_ARGS);
+            else
+                hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE, false, 1>), GEMM_ARGS);
+        }
+#undef GEMM_ARGS
     }
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -193,7 +193,7 @@ int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, double *par
  * CUTC_CRT_RBYTES(kind); every entry point below takes the kind from c. */
 #define CUTC_PK_KALIGN 768
 #define CUTC_PK_ROWS_A(rows) ((((int64_t)(rows)) + 511) / 512 * 512)
-#define CUTC_PK_ROWS_B(rows) ((((int64_t)(rows)) + 127) / 128 * 128)
+#define CUTC_PK_ROWS_B(rows) ((((int64_t)(rows)) + 255) / 256 * 256)
 int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t ld, int64_t rows, int64_t K, int64_t kpad,
                         const int *rexp, int8_t *planes, hipStream_t st);
 int cutc_crt_quant_pairw_pk(const cutc_crt *c, const double *wr2, const double *aos2, int64_t n2, int n_ao,
@@ -213,6 +213,47 @@ int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t ro
  * accumulate: add the residues already in C (one K pass of several). */
 int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
                      int8_t *C, int accumulate, hipStream_t st);
+/* Block-sparse stage 1 (order.hip).  The r2 points are reordered (nearest
+ * nucleus, then distance to it) and the pair rows sorted by where their
+ * quantised values are non-zero, so that whole (CUTC_SP_ROWS-pair B tile,
+ * K step) tiles of the pair planes are zero; the quantiser flags the others
+ * (occ[tb][kt], one byte each, cleared first) and the GEMM runs, per B tile,
+ * only the listed steps (blocks of 256 A x CUTC_SP_ROWS B rows).  A zero
+ * tile adds 0 to every modular sum: the results are those of the dense
+ * GEMM, bit for bit.
+ * Steps: CUTC_CRT_KSTEP(kind) K each (kpad / KSTEP per plane); lists are
+ * padded to CUTC_CRT_LIST_ALIGN(kind) steps (the GEMM pipelines' unroll). */
+#define CUTC_CRT_KSTEP(kind) ((kind) == CUTC_CRT_I8 ? 64 : 32)
+#define CUTC_CRT_LIST_ALIGN(kind) ((kind) == CUTC_CRT_I8 ? 6 : 24)
+#define CUTC_PAIR_SEGS 16 /* r2 segments of the pair support keys */
+#define CUTC_SP_ROWS 256  /* pair rows per B tile of the step lists */
+/* the same GEMM over the steps klist[tb KT + s], s < kcount[tb], of B tile
+ * tb (KT = kpad / KSTEP); klist = kcount = NULL: every step */
+int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
+                          const uint32_t *klist, const int *kcount, int8_t *C, int accumulate, hipStream_t st);
+/* pair planes with (k, l) of row p from ptab (NULL: the packed pair index)
+ * and, with occ, the non-zero flags of every (B tile, step) */
+int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2, const double *aos2, int64_t ld2, int64_t n2,
+                                int n_ao, int64_t kpad, const int *rexp, const int *ptab, int8_t *planes,
+                                uint8_t *occ, hipStream_t st);
+/* sort length for n keys: the power of two >= max(n, 2048) */
+int64_t cutc_sort_len(int64_t n);
+/* ascending sort of n (key, value) pairs, n = cutc_sort_len(n) */
+int cutc_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, hipStream_t st);
+/* r2 order: perm in vals[0 .. n2) and the permuted copies r2p (n2 x 3),
+ * wr2p (n2) and aos2p (n_ao x n2) of r2, wr2 and the AO values aos2
+ * (n_ao x n2); keys / vals hold nsort = cutc_sort_len(n2) entries */
+int cutc_order_r2(const double *r2, const double *wr2, const double *aos2, int64_t n2, int n_ao, const double *rn,
+                  int n_nuc, uint64_t *keys, uint32_t *vals, int64_t nsort, double *r2p, double *wr2p,
+                  double *aos2p, hipStream_t st);
+/* pair order from the natural-order exponents rexp_nat: the sorted table
+ * pairs (2 ints per row) and exponents rexp; nsort = cutc_sort_len(P) */
+int cutc_order_pairs(const double *wr2, const double *aos2, int64_t n2, int n_ao, int bits, const int *rexp_nat,
+                     uint64_t *keys, uint32_t *vals, int64_t nsort, int *pairs, int *rexp, hipStream_t st);
+int cutc_zero_u8(uint8_t *p, int64_t n, hipStream_t st);
+/* per-tile step lists from the flags occ (tiles x KT), KT a multiple of align */
+int cutc_klists(const uint8_t *occ, int64_t tiles, int64_t KT, int align, uint32_t *list, int *count,
+                hipStream_t st);
 /* Slice (Ozaki scheme I) contractions, ozaki1.hip: S = 2..4 digit planes of
  * 16-bit elements (bf16 or fp16 bit patterns, f16 != 0), rows padded to
  * CUTC_OZ_ROWS, kpad a multiple of CUTC_PK_KALIGN.  Row exponents from

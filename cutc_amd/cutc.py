@@ -51,7 +51,7 @@ class Dim3(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("slices", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("n1_chunk", ctypes.c_int), ("mem_limit", ctypes.c_size_t), ("host_copy", ctypes.c_int),
-                ("streams", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("dense", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -62,7 +62,8 @@ class Stats(ctypes.Structure):
                 ("n1_chunks", ctypes.c_int), ("flops_exec", ctypes.c_double), ("workspace_bytes", ctypes.c_size_t),
                 ("ms_host_in", ctypes.c_double), ("ms_host_out", ctypes.c_double),
                 ("ms_long_range", ctypes.c_double), ("ms_short_range", ctypes.c_double),
-                ("ms_stage1", ctypes.c_double), ("ms_stage2", ctypes.c_double)]
+                ("ms_stage1", ctypes.c_double), ("ms_stage2", ctypes.c_double),
+                ("k_density", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -206,11 +207,14 @@ def _check(rc, where):
         raise CutcError(rc, where)
 
 
-def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0, host_copy=0, streams=0) -> Options:
+def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=0, host_copy=0, streams=0,
+            dense=0) -> Options:
     """slices: bf16/fp16 digit slices per operand, or the number of moduli
     for int8 / bf16crt / fp16crt (None: the precision's default,
     DEFAULT_SLICES).  host_copy (host entry points): 0 = pipelined download
-    during the build, 1 = blocking copies after it."""
+    during the build, 1 = blocking copies after it.  dense (modular
+    precisions): 1 = the stage-1 contraction runs every K step instead of
+    skipping the all-zero tiles of the pair operand (same results)."""
     o = Options()
     lib().cutc_default_options(ctypes.byref(o))
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -220,6 +224,7 @@ def options(precision="fp64", slices=None, verbose=False, n1_chunk=0, mem_limit=
     o.mem_limit = mem_limit
     o.host_copy = int(host_copy)
     o.streams = int(streams)
+    o.dense = int(dense)
     return o
 
 

@@ -127,6 +127,10 @@ typedef struct cutc_options {
     int streams;      /* cutc_int_batch_device: concurrent branches the systems
                          are recorded on (1..4, 0 = 1; fewer when the device
                          cannot hold a workspace and chunk buffer per branch) */
+    int dense;        /* modular precisions: 1 = the stage-1 contraction runs
+                         every K step; 0 = it skips the (256-pair, K-step)
+                         tiles whose residues are all zero.  Same results, bit
+                         for bit; 1 is for checking and timing. */
 } cutc_options;
 
 typedef struct cutc_stats {
@@ -150,6 +154,9 @@ typedef struct cutc_stats {
     double ms_short_range; /* of ms_integrands: stage-2 r1 integrands (herm and non-herm, one kernel) */
     double ms_stage1;      /* stage 1 as a whole (Jastrow, integrand, contraction, split kernels) */
     double ms_stage2;      /* stage 2 as a whole, with the symmetrisation */
+    double k_density;      /* modular stage 1: fraction of the (256-pair tile, K step)
+                              tiles the contraction ran (1 = every one; 0 when not
+                              measured: FP64, slices, graph capture) */
 } cutc_stats;
 
 /* FP64, slices = 0 (each precision's own default), everything else 0. */

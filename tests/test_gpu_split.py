@@ -163,7 +163,7 @@ def test_gemm_pk_matches_exact_products(torch, M, N, K):
     rb[:, :, K:] = 0
     ra[0, 0, :K] = -128
     rb[0, 0, :K] = -128
-    ma, nb = (M + 511) // 512 * 512, (N + 127) // 128 * 128
+    ma, nb = (M + 511) // 512 * 512, (N + 255) // 256 * 256
     A = torch.from_numpy(np.concatenate([pack(ra[i], ma) for i in range(n)])).cuda()
     B = torch.from_numpy(np.concatenate([pack(rb[i], nb) for i in range(n)])).cuda()
     C = torch.zeros(n * M * N, dtype=torch.int8, device="cuda")
@@ -231,7 +231,7 @@ def test_gemm16_matches_exact_products(torch, kind, M, N, K, accum):
     rb[0, 0, :K] = -127
     ra[1, 0, :K] = 127
     rb[1, 0, :K] = -127
-    ma, nb = (M + 511) // 512 * 512, (N + 127) // 128 * 128
+    ma, nb = (M + 511) // 512 * 512, (N + 255) // 256 * 256
     A = torch.from_numpy(np.concatenate([pack16(to16(ra[i], kind == 2), ma) for i in range(n)]).view(np.int16)).cuda()
     B = torch.from_numpy(np.concatenate([pack16(to16(rb[i], kind == 2), nb) for i in range(n)]).view(np.int16)).cuda()
     C0 = np.zeros((n, M, N), np.int64)
