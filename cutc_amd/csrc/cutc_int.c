@@ -738,7 +738,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             ev = tr_begin(tr, CAT_GEMM1);
             rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
                                       rexpA, (double *)Cpl, st)
-                       : cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, ldk, kl, kc_, Cpl, 0, st);
+                       : cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, ldk, kl, kc_, 0, Cpl, 0, st);
             tr_end(tr, ev);
             TRY(rc);
         }
@@ -905,24 +905,35 @@ static int tune_split2(int64_t P, int n_ao, int64_t K, size_t budget, hipStream_
     return best;
 }
 
-/* Stage 2 over the r1 points r0 .. r0 + cnt - 1: wr1 / aos1 are the caller's
- * full arrays (leading dimension n1), X holds the cnt points' four components
- * with component stride cnt.  accumulate = 0: the first contraction stores
- * into out, else everything adds; symm = 1: out = M + M^T at the end.
- * nsplit2 > 1: the FP64 contractions run split-K (for small n_pairs, whose
- * P x P tile grid leaves most of the GPU idle). */
+/* Stage 2 over the r1 points r0 .. r0 + cnt - 1: r1 / wr1 / aos1 are the
+ * caller's full arrays (leading dimension n1), X holds the cnt points' four
+ * components with component stride cnt.  accumulate = 0: the first
+ * contraction stores into out, else everything adds; symm = 1: out = M + M^T
+ * at the end.  nsplit2 > 1: the FP64 contractions run split-K (for small
+ * n_pairs, whose P x P tile grid leaves most of the GPU idle).
+ * Modular precisions with r1 given: block-sparse, as stage 1 with the roles
+ * of the operands swapped.  Each r1 chunk's points run in (nearest nucleus,
+ * distance) order (the integrands are formed at the permuted points and X is
+ * read through the same permutation), the H-side pairs are sorted by where
+ * their quantised H values are non-zero, and the contraction runs, per
+ * 256-pair H tile (the GEMM's A side), only the K steps where that tile has a
+ * non-zero residue (opt->dense: all of them). */
 static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t r0, int64_t cnt, int n_ao,
-                  const double *wr1, const double *aos1, const double *X, double *out, int accumulate, int symm,
-                  int nsplit2, tracer_t *tr, cutc_stats *stats)
+                  const double *r1, const double *rn, int n_nuc, const double *wr1, const double *aos1,
+                  const double *X, double *out, int accumulate, int symm, int nsplit2, tracer_t *tr,
+                  cutc_stats *stats)
 {
     const hipStream_t st = tr->st;
     const int64_t P = pl->n_pairs, n1c = pl->n1_chunk_2e;
     const int64_t nao2 = (int64_t)n_ao * n_ao;
-    int *pairs = NULL;
+    int *pairs = NULL, *pairsH = NULL, *kcount = NULL;
     double *Hp = NULL, *NHp = NULL;
     int *rexpA = NULL, *rexpB = NULL;
     double *npart = NULL, *work = NULL;
     int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
+    uint8_t *occ = NULL;
+    uint32_t *perm1 = NULL, *klist = NULL, *pvals = NULL;
+    uint64_t *skeys = NULL, *pkeys = NULL;
     int rc = CUTC_OK, ev;
     const int split = opt->precision != CUTC_PREC_FP64;
     const int slice = split && !is_crt(opt->precision), f16 = opt->precision == CUTC_PREC_FP16;
@@ -938,6 +949,13 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     const size_t bpl = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * 3 * kp_max
                              : rb * nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
     const size_t bcp = slice ? sizeof(double) * P * P : (size_t)nmod * P * P;
+    /* block-sparse state: r1 order of a chunk, sort keys, the H-side pair
+     * table, occupancy flags, step lists and counts per 256-pair H tile */
+    const int sparse = split && !slice && r1 != NULL && (n_nuc == 0 || rn != NULL) && P > 0;
+    const int kind = slice ? 0 : crt_kind(opt->precision);
+    const int64_t tilesH = CUTC_PK_ROWS_A(P) / CUTC_SP_ROWS;
+    const int64_t KT2 = sparse ? 3 * kp_max / CUTC_CRT_KSTEP(kind) : 0;
+    const int64_t nsort = sparse ? cutc_sort_len(n1c) : 0, psort = sparse ? cutc_sort_len(P) : 0;
     {
         size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
                       ws_round(sizeof(double) * 3 * plane);
@@ -946,6 +964,11 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         if (split)
             need += 2 * ws_round(sizeof(int) * P) + ws_round(sizeof(double) * CUTC_CRT_KSPLIT * P) + 2 * ws_round(bpl) +
                     ws_round(bcp);
+        if (sparse)
+            need += ws_round(sizeof(uint64_t) * nsort) + ws_round(sizeof(uint32_t) * nsort) +
+                    ws_round(sizeof(uint64_t) * psort) + ws_round(sizeof(uint32_t) * psort) +
+                    ws_round(sizeof(int) * 2 * P) + ws_round(tilesH * KT2) + ws_round(sizeof(uint32_t) * tilesH * KT2) +
+                    ws_round(sizeof(int) * tilesH);
         TRY(ws_begin(&ws, need));
         if (stats && need > stats->workspace_bytes)
             stats->workspace_bytes = need;
@@ -963,18 +986,58 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         WS_TAKE(Bpl, bpl);
         WS_TAKE(Cpl, bcp);
     }
+    if (sparse) {
+        WS_TAKE(skeys, sizeof(uint64_t) * nsort);
+        WS_TAKE(perm1, sizeof(uint32_t) * nsort); /* the point sort's values: the permutation */
+        WS_TAKE(pkeys, sizeof(uint64_t) * psort);
+        WS_TAKE(pvals, sizeof(uint32_t) * psort);
+        WS_TAKE(pairsH, sizeof(int) * 2 * P);
+        WS_TAKE(occ, tilesH * KT2);
+        WS_TAKE(klist, sizeof(uint32_t) * tilesH * KT2);
+        WS_TAKE(kcount, sizeof(int) * tilesH);
+    }
     TRY(cutc_launch_pair_table(n_ao, pairs, st));
+    double dens_sum = 0.0, dens_den = 0.0;
 
     for (int64_t i0 = 0; i0 < cnt; i0 += n1c) {
         const int64_t n1e = (cnt - i0 < n1c) ? cnt - i0 : n1c;
         const int first = i0 == 0 && !accumulate;
         const int64_t pad0 = n1e * P, padn = ((n1e + 15) / 16 * 16 - n1e) * P;
         ev = tr_begin(tr, CAT_SHORTR);
-        rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, NHp, P, plane, st);
+        if (sparse) {
+            /* the chunk's r1 order, then the integrands at the permuted points */
+            rc = cutc_order_points(r1 + 3 * (r0 + i0), n1e, rn, n_nuc, skeys, perm1, cutc_sort_len(n1e), st);
+        }
+        if (rc == CUTC_OK)
+            rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, NHp, P, plane,
+                                           sparse ? perm1 : NULL, st);
         for (int m = 0; m < 4 && rc == CUTC_OK && padn > 0; m++)
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
         tr_end(tr, ev);
         TRY(rc);
+        if (sparse) {
+            /* H-side pair order by support (keys from H = w1 chi_a chi_b at the
+             * bits of the first part), then the integrands again in that order */
+            cutc_crt crt0;
+            cutc_colsrc sh;
+            memset(&sh, 0, sizeof sh);
+            sh.nsrc = 1;
+            sh.p[0] = Hp;
+            sh.ld = P;
+            sh.K = n1e;
+            ev = tr_begin(tr, CAT_SPLIT);
+            rc = cutc_crt_setup(n1e, nmod, kind, &crt0);
+            if (rc == CUTC_OK)
+                rc = cutc_rowexp_colk(&sh, P, emode, npart, rexpB, st);
+            if (rc == CUTC_OK)
+                rc = cutc_order_pairs_colk(&sh, n_ao, crt0.bits, rexpB, pkeys, pvals, psort, pairsH, st);
+            tr_end(tr, ev);
+            TRY(rc);
+            ev = tr_begin(tr, CAT_SHORTR);
+            rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairsH, Hp, NHp, P, plane, perm1, st);
+            tr_end(tr, ev);
+            TRY(rc);
+        }
         if (!split && nsplit2 > 1) {
             const int ns = cutc_gemm2_split_count(n1e, nsplit2);
             ev = tr_begin(tr, CAT_GEMM2);
@@ -1015,6 +1078,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 sa.K = sb.K = n1e;
                 sa.pairs = pairs;
                 sa.n_ao = n_ao;
+                sa.kperm = sparse ? perm1 : NULL; /* X rows of the permuted points */
                 for (int g = 0; g < nseg; g++) {
                     const int m = part == 0 ? 3 : g;
                     sa.p[g] = X + (m * cnt + i0) * nao2;
@@ -1027,22 +1091,46 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                                : cutc_crt_quant_colk_pk(&crt, &sa, P, kp, CUTC_PK_ROWS_B(P), rexpA, Apl, st);
                 if (rc == CUTC_OK)
                     rc = cutc_rowexp_colk(&sb, P, emode, npart, rexpB, st);
+                const int64_t kt = sparse ? nseg * kp / CUTC_CRT_KSTEP(kind) : 0;
+                if (rc == CUTC_OK && sparse)
+                    rc = cutc_zero_u8(occ, tilesH * kt, st);
                 if (rc == CUTC_OK)
                     rc = slice ? cutc_oz_slice_colk(nmod, f16, &sb, P, kp, CUTC_OZ_ROWS(P), rexpB, (uint16_t *)Bpl, st)
-                               : cutc_crt_quant_colk_pk(&crt, &sb, P, kp, CUTC_PK_ROWS_A(P), rexpB, Bpl, st);
+                               : cutc_crt_quant_colk_pk_occ(&crt, &sb, P, kp, CUTC_PK_ROWS_A(P), rexpB, Bpl,
+                                                            sparse ? occ : NULL, st);
+                if (rc == CUTC_OK && sparse)
+                    rc = cutc_klists(occ, tilesH, kt, CUTC_CRT_LIST_ALIGN(kind), klist, kcount, st);
                 tr_end(tr, ev);
                 TRY(rc);
+                const int lists = sparse && !opt->dense;
                 ev = tr_begin(tr, CAT_GEMM2);
                 rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Bpl, P, (const uint16_t *)Apl, P, nseg * kp, rexpB,
                                           rexpA, (double *)Cpl, st)
-                           : cutc_crt_gemm_pk(&crt, Bpl, P, Apl, P, nseg * kp, Cpl, 0, st);
+                           : cutc_crt_gemm_pk_list(&crt, Bpl, P, Apl, P, nseg * kp, lists ? klist : NULL,
+                                                   lists ? kcount : NULL, 1, Cpl, 0, st);
                 tr_end(tr, ev);
                 TRY(rc);
+                if (lists && stats && !tr->capture) {
+                    /* executed fraction of the (256-pair H tile, K step) tiles */
+                    int *h = (int *)malloc(sizeof(int) * tilesH);
+                    if (!h)
+                        TRY(CUTC_ERR_NOMEM);
+                    rc = hipMemcpyAsync(h, kcount, sizeof(int) * tilesH, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                                 hipStreamSynchronize(st) == hipSuccess
+                             ? CUTC_OK
+                             : CUTC_ERR_HIP;
+                    for (int64_t i = 0; i < tilesH && rc == CUTC_OK; i++)
+                        dens_sum += (double)h[i] * (double)nmod;
+                    free(h);
+                    TRY(rc);
+                    dens_den += (double)tilesH * (double)kt * (double)nmod;
+                }
                 ev = tr_begin(tr, CAT_SPLIT);
                 const double alpha = part == 0 ? 1.0 : -0.5;
                 const int mode = part == 0 ? (first ? 0 : 1) : 2;
                 rc = slice ? cutc_scatter_m((const double *)Cpl, 1, P, alpha, pairs, n_ao, out, mode, st)
-                           : cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, alpha, pairs, n_ao, out, mode, st);
+                           : cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, alpha, pairs, sparse ? pairsH : NULL,
+                                                n_ao, out, mode, st);
                 tr_end(tr, ev);
                 TRY(rc);
             }
@@ -1056,6 +1144,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         tr_end(tr, ev);
         TRY(rc);
     }
+    if (stats && dens_den > 0)
+        stats->k_density_2e = dens_sum / dens_den;
     if (!tr->capture)
         TRYH(hipStreamSynchronize(st));
 done:
@@ -1221,7 +1311,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             rc = cutc_launch_fill(out, (int64_t)n_ao * n_ao * n_ao * n_ao, 0.0, st);
             tr_end(&tr, ev);
         } else {
-            rc = stage2(opt, &pl, n1, 0, n1, n_ao, wr1, aos1, X, out, 0, 1,
+            rc = stage2(opt, &pl, n1, 0, n1, n_ao, r1, rn, n_nuc, wr1, aos1, X, out, 0, 1,
                         split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
         }
     }
@@ -1332,7 +1422,8 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
         tr_end(tr, evs);
         if (rc == CUTC_OK) {
             evs = tr_begin(tr, CAT_STAGE2);
-            rc = stage2(opt, pl, n1, r0, cnt, n_ao, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, nsplit2, tr, stats);
+            rc = stage2(opt, pl, n1, r0, cnt, n_ao, r1, rn, n_nuc, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, nsplit2,
+                        tr, stats);
             tr_end(tr, evs);
         }
     }

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void rowexp_colk_kernel(cutc_colsrc s, int64_t
     for (int seg = 0; seg < s.nsrc; seg++) {
         const double *base = s.p[seg] + col;
         for (int64_t k = k0; k < k1; k++)
-            v = rs_add<MODE>(v, base[k * s.ld]);
+            v = rs_add<MODE>(v, base[colsrc_k(s, k) * s.ld]);
     }
     partial[(int64_t)blockIdx.y * rows + r] = v;
 }
@@ -461,10 +461,14 @@ __device__ static inline int shrink_mod(int x, int m, float inv_m)
 // segment, read coalesced along the rows into LDS and written as 16-byte
 // fragments.  grid (ceil(rows / 32), kp / 128, nsrc); segment s occupies K
 // [s kp, (s + 1) kp), zero past the valid s.K.
+// occ (optional): the non-zero flags of the (CUTC_SP_ROWS-row tile, K step)
+// tiles, one ballot per wave (32 rows x 32 K inside one step), as in
+// quant_pairw_pk_kernel.
 template <int KIND>
 __global__ __launch_bounds__(256) void quant_colk_pk_kernel(cutc_crt c, cutc_colsrc s, int64_t rows, int64_t kp,
                                                             int64_t KT, int64_t plane,
-                                                            const int *__restrict__ rexp, int8_t *planes)
+                                                            const int *__restrict__ rexp, int8_t *planes,
+                                                            uint8_t *occ)
 {
     __shared__ double tile[128][33];
     const int64_t rb = blockIdx.x, r0 = rb * 32, k0 = (int64_t)blockIdx.y * 128;
@@ -477,22 +481,37 @@ __global__ __launch_bounds__(256) void quant_colk_pk_kernel(cutc_crt c, cutc_col
         const double *src = s.p[seg] + col;
         for (int kk = threadIdx.x >> 5; kk < 128; kk += 8) {
             const int64_t k = k0 + kk;
-            tile[kk][rr] = (rok && k < s.K) ? src[k * s.ld] : 0.0;
+            tile[kk][rr] = (rok && k < s.K) ? src[colsrc_k(s, k) * s.ld] : 0.0;
         }
     }
     __syncthreads();
     const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
     const int64_t row = r0 + r;
     double q[16];
+    bool nz = false;
     const int sh = (row < rows) ? c.bits - rexp[row] : 0;
 #pragma unroll
-    for (int e = 0; e < 16; e++)
+    for (int e = 0; e < 16; e++) {
         q[e] = rint(ldexp(tile[16 * j + e][r], sh));
+        nz |= q[e] != 0.0;
+    }
     store_residues<KIND>(c, q, planes, rb, r, seg * kp + k0 + 16 * j, KT, plane);
+    if (occ) {
+        const uint64_t b = __ballot(nz);
+        if (b && (threadIdx.x & 63) == 0)
+            occ[(rb / (CUTC_SP_ROWS / 32)) * KT + (seg * kp + k0 + 16 * j) / (KIND == CUTC_CRT_I8 ? 64 : 32)] = 1;
+    }
 }
 
 extern "C" int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp,
                                       int64_t plane_rows, const int *rexp, int8_t *planes, hipStream_t st)
+{
+    return cutc_crt_quant_colk_pk_occ(c, src, rows, kp, plane_rows, rexp, planes, NULL, st);
+}
+
+extern "C" int cutc_crt_quant_colk_pk_occ(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp,
+                                          int64_t plane_rows, const int *rexp, int8_t *planes, uint8_t *occ,
+                                          hipStream_t st)
 {
     if (rows <= 0 || kp <= 0)
         return 0;
@@ -503,7 +522,7 @@ extern "C" int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src,
     const dim3 grid((unsigned)((rows + 31) / 32), (unsigned)(kp / 128), src->nsrc);
 #define QC(K_)                                                                                                         \
     hipLaunchKernelGGL(quant_colk_pk_kernel<K_>, grid, dim3(256), 0, st, *c, *src, rows, kp, pk_steps(K_, ktot),       \
-                       plane_rows * ktot * CUTC_CRT_RBYTES(K_), rexp, planes)
+                       plane_rows * ktot * CUTC_CRT_RBYTES(K_), rexp, planes, occ)
     CRT_KIND_DISPATCH(c->kind, QC);
 #undef QC
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
@@ -552,8 +571,9 @@ __device__ static inline void pk_dma16(const int8_t *g, int8_t *l)
 // LDS (sK) and read one step ahead, so the offset of step k + L is in a
 // register when step k issues its loads.  WB = 2 halves the A bytes per
 // MFMA, which bounds the list-driven GEMM: blocks of different B tiles run
-// different steps, so their A tiles are not shared through L2.
-template <int D, int L, int BAR, int SEG, bool SP, int WB>
+// different steps, so their A tiles are not shared through L2.  LA: the
+// lists belong to the A tiles (klist[ta KT ..]) instead.
+template <int D, int L, int BAR, int SEG, bool SP, int WB, bool LA>
 __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__restrict__ A,
                                                              const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                              int tiles_b, int64_t M, int64_t N, int mod,
@@ -566,11 +586,18 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * WB * PK_TILE];
     __shared__ uint32_t sK[SP ? SEG : 1];
 
+    // consecutive blocks share the listed tile (4 tiles of the other side
+    // each): they run the same list in step and are replaced together, so
+    // the listed tile is read once per group through L2
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
     int ta, tb;
-    cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
-    const int64_t NS = SP ? (int64_t)kcount[tb] : KT; // steps run
-    const uint32_t *kl = SP ? klist + (int64_t)tb * KT : nullptr;
+    if constexpr (LA)
+        cutc_group_tile<4>(t, tiles_b, tiles_a, tb, ta);
+    else
+        cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
+    const int lt = LA ? ta : tb;                       // the tile whose list is run
+    const int64_t NS = SP ? (int64_t)kcount[lt] : KT;  // steps run
+    const uint32_t *kl = SP ? klist + (int64_t)lt * KT : nullptr;
     int64_t kbase = 0;                                 // first step of the current segment
     uint32_t so_nx = 0;                                // SP: offset of the step L + 1 ahead
     auto off = [&](int64_t s) -> uint32_t {
@@ -771,9 +798,9 @@ __host__ __device__ constexpr int crt16_red_elem(int pos) { return 2 * ((pos - 1
 // and nothing goes to scratch.  KT is a multiple of W.
 // SP: the kcount[tb] listed steps only (a multiple of W), as in
 // crt_gemm_pk_kernel, in segments of SEG steps (a multiple of W), each a
-// full pipeline with its byte offsets staged in LDS.  WB: the block shape,
-// as in crt_gemm_pk_kernel.
-template <bool F16, int NA, int DB, int L, int BAR, int W, bool SP, int WB>
+// full pipeline with its byte offsets staged in LDS.  WB, LA: the block
+// shape and the side of the lists, as in crt_gemm_pk_kernel.
+template <bool F16, int NA, int DB, int L, int BAR, int W, bool SP, int WB, bool LA>
 __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__restrict__ A,
                                                             const int8_t *__restrict__ B, int64_t KT, int tiles_a,
                                                             int tiles_b, int64_t M, int64_t N, int mod,
@@ -790,11 +817,18 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     __shared__ __attribute__((aligned(16))) int8_t sB[DB][4 * WB * PK_TILE];
     __shared__ uint32_t sK[SP ? SEG : 1];
 
+    // consecutive blocks share the listed tile (4 tiles of the other side
+    // each): they run the same list in step and are replaced together, so
+    // the listed tile is read once per group through L2
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
     int ta, tb;
-    cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
-    const int64_t NS = SP ? (int64_t)kcount[tb] : KT; // steps run
-    const uint32_t *kl = SP ? klist + (int64_t)tb * KT : nullptr;
+    if constexpr (LA)
+        cutc_group_tile<4>(t, tiles_b, tiles_a, tb, ta);
+    else
+        cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
+    const int lt = LA ? ta : tb;                       // the tile whose list is run
+    const int64_t NS = SP ? (int64_t)kcount[lt] : KT;  // steps run
+    const uint32_t *kl = SP ? klist + (int64_t)lt * KT : nullptr;
     int64_t kbase = 0;                                 // first step of the current segment
     uint32_t so_nx = 0;                                // SP: offset of the step L + 1 ahead
     auto off = [&](int64_t s) -> uint32_t {
@@ -950,16 +984,20 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
 // steps ahead, a barrier every second step, windows of 24 steps (768 K; a
 // 16-bit plane's KT = kpad / 32 is a multiple of 24)
 #define CRT16_SHAPE 4, 6, 3, 2, 24
+// (SP, WB, LA) of the three GEMM forms: dense, lists per B tile, lists per A tile
+#define GEMM_FALSE_1_FALSE false, 1, false
+#define GEMM_TRUE_2_FALSE true, 2, false
+#define GEMM_TRUE_2_TRUE true, 2, true
 static_assert(CUTC_PK_KALIGN / 32 % 24 == 0, "16-bit K steps per window");
 
 extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
                                 int64_t kpad, int8_t *C, int accumulate, hipStream_t st)
 {
-    return cutc_crt_gemm_pk_list(c, A, M, B, N, kpad, NULL, NULL, C, accumulate, st);
+    return cutc_crt_gemm_pk_list(c, A, M, B, N, kpad, NULL, NULL, 0, C, accumulate, st);
 }
 
 extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
-                                     int64_t kpad, const uint32_t *klist, const int *kcount, int8_t *C,
+                                     int64_t kpad, const uint32_t *klist, const int *kcount, int list_a, int8_t *C,
                                      int accumulate, hipStream_t st)
 {
     if (M <= 0 || N <= 0)
@@ -970,9 +1008,10 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
     const int64_t KT = pk_steps(c->kind, kpad), rb = CUTC_CRT_RBYTES(c->kind);
     if (4 * KT * PK_TILE > 4294967295LL) /* 32-bit offsets inside a wave's four row blocks */
         return CUTC_ERR_ARG;
-    const int sp = klist != NULL;
-    /* list-driven: 256 x 256 blocks (list per CUTC_SP_ROWS B rows); dense: 512 x 128 */
-    const int64_t ta = CUTC_PK_ROWS_A(M) / (sp ? 256 : 512), tb = CUTC_PK_ROWS_B(N) / (sp ? CUTC_SP_ROWS : 128);
+    const int sp = klist != NULL, la = sp && list_a;
+    /* list-driven: 256 x 256 blocks (CUTC_SP_ROWS rows per listed tile); dense: 512 x 128 */
+    static_assert(CUTC_SP_ROWS == 256, "list tiles of the 256 x 256 block");
+    const int64_t ta = CUTC_PK_ROWS_A(M) / (sp ? 256 : 512), tb = CUTC_PK_ROWS_B(N) / (sp ? 256 : 128);
     if (ta * tb > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
@@ -984,22 +1023,22 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
 #define GEMM_ARGS                                                                                                      \
     grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate, \
         klist, kcount
-        if (c->kind == CUTC_CRT_I8) {
-            if (sp)
-                hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040, true, 2>), GEMM_ARGS);
-            else
-                hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040, false, 1>), GEMM_ARGS);
-        } else if (c->kind == CUTC_CRT_BF16) {
-            if (sp)
-                hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE, true, 2>), GEMM_ARGS);
-            else
-                hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE, false, 1>), GEMM_ARGS);
-        } else {
-            if (sp)
-                hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE, true, 2>), GEMM_ARGS);
-            else
-                hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE, false, 1>), GEMM_ARGS);
-        }
+#define GEMM_SP(K8, K16)                                                                                               \
+    do {                                                                                                               \
+        if (c->kind == CUTC_CRT_I8)                                                                                    \
+            hipLaunchKernelGGL((crt_gemm_pk_kernel<6, 4, 2, 2040, K8>), GEMM_ARGS);                                    \
+        else if (c->kind == CUTC_CRT_BF16)                                                                             \
+            hipLaunchKernelGGL((crt16_gemm_kernel<false, CRT16_SHAPE, K16>), GEMM_ARGS);                               \
+        else                                                                                                           \
+            hipLaunchKernelGGL((crt16_gemm_kernel<true, CRT16_SHAPE, K16>), GEMM_ARGS);                                \
+    } while (0)
+        if (la)
+            GEMM_SP(GEMM_TRUE_2_TRUE, GEMM_TRUE_2_TRUE);
+        else if (sp)
+            GEMM_SP(GEMM_TRUE_2_FALSE, GEMM_TRUE_2_FALSE);
+        else
+            GEMM_SP(GEMM_FALSE_1_FALSE, GEMM_FALSE_1_FALSE);
+#undef GEMM_SP
 #undef GEMM_ARGS
     }
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
@@ -1063,12 +1102,14 @@ extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P,
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// thread per (p, q), p fastest; modes as the FP64 step-2 epilogue
+// thread per (p, q), p fastest; modes as the FP64 step-2 epilogue; the
+// rows q of the A side have their own pair table pairsB
 __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t *__restrict__ C,
                                                         int64_t P,
                                                         const int *__restrict__ rexpA,
                                                         const int *__restrict__ rexpB, double alpha,
-                                                        const int *__restrict__ pairs, int n_ao, double *M, int mode)
+                                                        const int *__restrict__ pairs,
+                                                        const int *__restrict__ pairsB, int n_ao, double *M, int mode)
 {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= P)
@@ -1080,7 +1121,7 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
     for (int64_t q = blockIdx.y; q < P; q += gridDim.y) {
     const double V = crt_value(c, C + q * P + p, P * P);
     const double v = alpha * ldexp(V, ea + rexpB[q]);
-    const int a = pairs[2 * q], b = pairs[2 * q + 1];
+    const int a = pairsB[2 * q], b = pairsB[2 * q + 1];
     double *o1 = M + n2 * (a + (int64_t)n_ao * b);
     double *o2 = M + n2 * (b + (int64_t)n_ao * a);
     if (mode == 0) {
@@ -1103,13 +1144,13 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
 }
 
 extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
-                                  const int *rexpB, double alpha, const int *pairs, int n_ao, double *M, int mode,
-                                  hipStream_t st)
+                                  const int *rexpB, double alpha, const int *pairs, const int *pairsB, int n_ao,
+                                  double *M, int mode, hipStream_t st)
 {
     if (P <= 0)
         return 0;
     const int64_t gy = P < 65535 ? P : 65535;
     hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, *c, C,
-                       P, rexpA, rexpB, alpha, pairs, n_ao, M, mode);
+                       P, rexpA, rexpB, alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, mode);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -81,7 +81,8 @@ __global__ __launch_bounds__(256) void r1_integrands_kernel(const double *__rest
                                                             const double *__restrict__ aos1, int64_t n1_ld,
                                                             int64_t ii0, int64_t n1, int n_ao,
                                                             const int *__restrict__ pairs, double *Hp, double *NHp,
-                                                            int64_t ldq, int64_t nh_plane)
+                                                            int64_t ldq, int64_t nh_plane,
+                                                            const uint32_t *__restrict__ perm)
 {
     extern __shared__ double s[];
     __shared__ double sw[R];
@@ -90,10 +91,12 @@ __global__ __launch_bounds__(256) void r1_integrands_kernel(const double *__rest
     for (int t = threadIdx.x; t < 4 * n_ao * R; t += 256) {
         int r = t % R, ca = t / R; // ca = c * n_ao + ao
         int ao = ca % n_ao, c = ca / n_ao;
-        s[t] = (r < nr) ? aos1[ii0 + r0 + r + n1_ld * (ao + (int64_t)n_ao * c)] : 0.0;
+        const int64_t pt = ii0 + (perm && r < nr ? (int64_t)perm[r0 + r] : r0 + r);
+        s[t] = (r < nr) ? aos1[pt + n1_ld * (ao + (int64_t)n_ao * c)] : 0.0;
     }
     if (threadIdx.x < R)
-        sw[threadIdx.x] = ((int)threadIdx.x < nr) ? wr1[ii0 + r0 + threadIdx.x] : 0.0;
+        sw[threadIdx.x] =
+            ((int)threadIdx.x < nr) ? wr1[ii0 + (perm ? (int64_t)perm[r0 + threadIdx.x] : r0 + threadIdx.x)] : 0.0;
     __syncthreads();
     const int64_t Q = (int64_t)n_ao * (n_ao + 1) / 2;
     const int64_t cs = (int64_t)n_ao * R;
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256) void r1_integrands_kernel(const double *__rest
 
 extern "C" int cutc_launch_r1_integrands(const double *wr1, const double *aos1, int64_t n1_ld, int64_t ii0,
                                          int64_t n1, int n_ao, const int *pairs, double *Hp, double *NHp,
-                                         int64_t ldq, int64_t nh_plane, hipStream_t st)
+                                         int64_t ldq, int64_t nh_plane, const uint32_t *perm, hipStream_t st)
 {
     if (n1 <= 0 || n_ao <= 0)
         return 0;
@@ -122,10 +125,10 @@ extern "C" int cutc_launch_r1_integrands(const double *wr1, const double *aos1, 
         int64_t nb = (n1 + 3) / 4;
         hipLaunchKernelGGL((r1_integrands_kernel<4>), dim3((unsigned)nb), dim3(256),
                            sizeof(double) * 4 * n_ao * 4, st, wr1, aos1, n1_ld, ii0, n1, n_ao, pairs, Hp, NHp, ldq,
-                           nh_plane);
+                           nh_plane, perm);
     } else if ((size_t)4 * n_ao * sizeof(double) <= 64 * 1024) {
         hipLaunchKernelGGL((r1_integrands_kernel<1>), dim3((unsigned)n1), dim3(256), sizeof(double) * 4 * n_ao, st,
-                           wr1, aos1, n1_ld, ii0, n1, n_ao, pairs, Hp, NHp, ldq, nh_plane);
+                           wr1, aos1, n1_ld, ii0, n1, n_ao, pairs, Hp, NHp, ldq, nh_plane, perm);
     } else {
         return CUTC_ERR_UNSUPPORTED;
     }

@@ -55,9 +55,10 @@ int cutc_launch_jast_pairs(const double *r1, const double *r2, const double *rn,
 int cutc_launch_pair_table(int n_ao, int *pairs, hipStream_t st);
 int cutc_launch_pair_weights(const double *wr2, const double *aos2, int64_t n2_ld, int64_t jj0, int64_t n2,
                              int64_t n2_fill, int n_ao, double *Apk, int64_t lda, hipStream_t st);
+/* perm (optional): row r of Hp / NHp is point ii0 + perm[r] instead of ii0 + r */
 int cutc_launch_r1_integrands(const double *wr1, const double *aos1, int64_t n1_ld, int64_t ii0, int64_t n1,
                               int n_ao, const int *pairs, double *Hp, double *NHp, int64_t ldq, int64_t nh_plane,
-                              hipStream_t st);
+                              const uint32_t *perm, hipStream_t st);
 int cutc_launch_long_range_ref(int64_t jj0, int64_t n2_eff, int64_t n2_tot, int64_t n2, int n_ao, const double *wr2,
                                const double *aos2, double *out, hipStream_t st);
 int cutc_launch_herm_ref(int64_t n1, int n_ao, const double *wr1, const double *aos1, double *out, hipStream_t st);
@@ -155,14 +156,16 @@ typedef struct cutc_crt {
 int cutc_crt_setup(int64_t K, int n, int kind, cutc_crt *c);
 
 /* column-major sources for the stage-2 operands: element (row r, k) of
- * segment s is p[s][k * ld + col(r)], col(r) = k_r + n_ao l_r through the
- * pair table when pairs != NULL, else r. */
+ * segment s is p[s][kp(k) * ld + col(r)], col(r) = k_r + n_ao l_r through the
+ * pair table when pairs != NULL, else r; kp(k) = kperm[k] when kperm != NULL
+ * (the r1 points of a chunk in another order), else k. */
 typedef struct cutc_colsrc {
     const double *p[3];
     int nsrc;
     int64_t ld, K;
     const int *pairs;
     int n_ao;
+    const uint32_t *kperm;
 } cutc_colsrc;
 
 /* Row exponents: CUTC_ROWEXP_NORM gives e with ||x_row||_2 < 2^e (the
@@ -207,6 +210,10 @@ int cutc_crt_quant_pairw_pk_range(const cutc_crt *c, const double *wr2, const do
  * CUTC_PK_KALIGN. */
 int cutc_crt_quant_colk_pk(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp, int64_t plane_rows,
                            const int *rexp, int8_t *planes, hipStream_t st);
+/* the same with the non-zero flags occ[tile][kt] of every (CUTC_SP_ROWS-row
+ * tile, K step) of the planes (cleared by the caller) */
+int cutc_crt_quant_colk_pk_occ(const cutc_crt *c, const cutc_colsrc *src, int64_t rows, int64_t kp,
+                               int64_t plane_rows, const int *rexp, int8_t *planes, uint8_t *occ, hipStream_t st);
 /* C[i][r][p] = (sum_k A[i][r][k] B[i][p][k]) mod m_i as int8 in
  * [-(m_i / 2), (m_i - 1) / 2] for r < M, p < N, packed planes as above; C
  * holds n M N bytes (row r of the A side is contiguous over p).
@@ -227,10 +234,12 @@ int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t
 #define CUTC_CRT_LIST_ALIGN(kind) ((kind) == CUTC_CRT_I8 ? 6 : 24)
 #define CUTC_PAIR_SEGS 16 /* r2 segments of the pair support keys */
 #define CUTC_SP_ROWS 256  /* pair rows per B tile of the step lists */
-/* the same GEMM over the steps klist[tb KT + s], s < kcount[tb], of B tile
- * tb (KT = kpad / KSTEP); klist = kcount = NULL: every step */
+/* the same GEMM over the steps klist[t KT + s], s < kcount[t], of tile t
+ * (KT = kpad / KSTEP) of the B side (list_a = 0) or of the A side
+ * (list_a = 1); klist = kcount = NULL: every step */
 int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
-                          const uint32_t *klist, const int *kcount, int8_t *C, int accumulate, hipStream_t st);
+                          const uint32_t *klist, const int *kcount, int list_a, int8_t *C, int accumulate,
+                          hipStream_t st);
 /* pair planes with (k, l) of row p from ptab (NULL: the packed pair index)
  * and, with occ, the non-zero flags of every (B tile, step) */
 int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2, const double *aos2, int64_t ld2, int64_t n2,
@@ -240,6 +249,15 @@ int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2, const doub
 int64_t cutc_sort_len(int64_t n);
 /* ascending sort of n (key, value) pairs, n = cutc_sort_len(n) */
 int cutc_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, hipStream_t st);
+/* point order (nearest nucleus, then distance) of the n points r (n x 3):
+ * the permutation in vals[0 .. n); keys / vals hold nsort = cutc_sort_len(n) */
+int cutc_order_points(const double *r, int64_t n, const double *rn, int n_nuc, uint64_t *keys, uint32_t *vals,
+                      int64_t nsort, hipStream_t st);
+/* pair order by support over a column source (rows = packed pairs, one
+ * segment): keys of the quantised rows rint(x 2^(bits - rexp[row])) over
+ * CUTC_PAIR_SEGS equal K segments, sorted; the sorted pair table in pairs */
+int cutc_order_pairs_colk(const cutc_colsrc *src, int n_ao, int bits, const int *rexp, uint64_t *keys,
+                          uint32_t *vals, int64_t nsort, int *pairs, hipStream_t st);
 /* r2 order: perm in vals[0 .. n2) and the permuted copies r2p (n2 x 3),
  * wr2p (n2) and aos2p (n_ao x n2) of r2, wr2 and the AO values aos2
  * (n_ao x n2); keys / vals hold nsort = cutc_sort_len(n2) entries */
@@ -280,9 +298,11 @@ int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N,
                        const int *rexpB, const int *pairs, int n_ao, double *X, int64_t n1, int64_t n1c,
                        int64_t i0, hipStream_t st);
 /* Rebuild alpha C into the n_ao^2 x n_ao^2 matrix M; mode as cutc_gemm2_f64. */
+/* pairsB (optional): the pair table of the rows of the GEMM's A side (rexpB),
+ * when it differs from pairs (the B side's, rexpA) */
 int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
-                       const int *rexpB, double alpha, const int *pairs, int n_ao, double *M, int mode,
-                       hipStream_t st);
+                       const int *rexpB, double alpha, const int *pairs, const int *pairsB, int n_ao, double *M,
+                       int mode, hipStream_t st);
 
 /* Dense TN GEMM (gemm_f64.hip): C = alpha A^T B + beta C, K-contiguous rows
  * (lda, ldb >= K).  K a multiple of 16 with even leading dimensions and

@@ -164,17 +164,27 @@ __global__ __launch_bounds__(256) void r2_gather_kernel(const uint32_t *__restri
     }
 }
 
+extern "C" int cutc_order_points(const double *r, int64_t n, const double *rn, int n_nuc, uint64_t *keys,
+                                 uint32_t *vals, int64_t nsort, hipStream_t st)
+{
+    if (n <= 0)
+        return 0;
+    if (nsort < n || nsort != cutc_sort_len(nsort) || n > 4294967295LL)
+        return CUTC_ERR_ARG;
+    hipLaunchKernelGGL(r2_key_kernel, dim3((unsigned)((nsort + 255) / 256)), dim3(256), 0, st, r, n, rn, n_nuc,
+                       nsort, keys, vals);
+    return cutc_sort_u64(keys, vals, nsort, st);
+}
+
 extern "C" int cutc_order_r2(const double *r2, const double *wr2, const double *aos2, int64_t n2, int n_ao,
                              const double *rn, int n_nuc, uint64_t *keys, uint32_t *vals, int64_t nsort,
                              double *r2p, double *wr2p, double *aos2p, hipStream_t st)
 {
     if (n2 <= 0)
         return 0;
-    if (nsort < n2 || nsort != cutc_sort_len(nsort) || n2 > 4294967295LL || n_ao < 0 || n_ao > 65534)
+    if (n_ao < 0 || n_ao > 65534)
         return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(r2_key_kernel, dim3((unsigned)((nsort + 255) / 256)), dim3(256), 0, st, r2, n2, rn, n_nuc,
-                       nsort, keys, vals);
-    int rc = cutc_sort_u64(keys, vals, nsort, st);
+    int rc = cutc_order_points(r2, n2, rn, n_nuc, keys, vals, nsort, st);
     if (rc)
         return rc;
     hipLaunchKernelGGL(r2_gather_kernel, dim3((unsigned)((n2 + 255) / 256), (unsigned)(n_ao + 1)), dim3(256), 0, st,
@@ -255,6 +265,68 @@ extern "C" int cutc_order_pairs(const double *wr2, const double *aos2, int64_t n
         return rc;
     hipLaunchKernelGGL(pair_perm_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, vals, P, rexp_nat,
                        pairs, rexp);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
+// Support keys of the rows of a one-segment column source (stage 2: rows =
+// packed pairs, K = the chunk's r1 points), as pair_key_kernel: block of 64
+// rows x 4 K lanes, coalesced along the rows; rows [rows, n) sort last.
+__global__ __launch_bounds__(256) void colk_key_kernel(cutc_colsrc s, int64_t rows, int bits,
+                                                       const int *__restrict__ rexp, int64_t n, uint64_t *keys,
+                                                       uint32_t *vals)
+{
+    __shared__ unsigned smask[64];
+    const int rl = threadIdx.x & 63, kk = threadIdx.x >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 64 + rl;
+    if (threadIdx.x < 64)
+        smask[threadIdx.x] = 0;
+    __syncthreads();
+    if (r < rows) {
+        const double *src = s.p[0] + colsrc_col(s, r);
+        const int sh = bits - rexp[r];
+        unsigned m = 0;
+        for (int64_t k = kk; k < s.K; k += 4) {
+            const double q = rint(ldexp(src[colsrc_k(s, k) * s.ld], sh));
+            if (q != 0.0)
+                m |= 1u << (int)(k * CUTC_PAIR_SEGS / s.K);
+        }
+        if (m)
+            atomicOr(&smask[rl], m);
+    }
+    __syncthreads();
+    if (kk == 0 && r < n) {
+        keys[r] = r < rows ? (((uint64_t)smask[rl] << 32) | (uint32_t)r) : ~(uint64_t)0;
+        vals[r] = (uint32_t)r;
+    }
+}
+
+// sorted position s <- natural pair order[s]: (k, l)
+__global__ __launch_bounds__(256) void pair_table_perm_kernel(const uint32_t *__restrict__ order, int64_t P,
+                                                              int *pairs)
+{
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= P)
+        return;
+    int k, l;
+    pair_of(order[s], k, l);
+    pairs[2 * s] = k;
+    pairs[2 * s + 1] = l;
+}
+
+extern "C" int cutc_order_pairs_colk(const cutc_colsrc *src, int n_ao, int bits, const int *rexp, uint64_t *keys,
+                                     uint32_t *vals, int64_t nsort, int *pairs, hipStream_t st)
+{
+    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    if (P <= 0)
+        return 0;
+    if (nsort < P || nsort != cutc_sort_len(nsort) || nsort > 2147483647LL || src->nsrc != 1 || src->K <= 0)
+        return CUTC_ERR_ARG;
+    hipLaunchKernelGGL(colk_key_kernel, dim3((unsigned)(nsort / 64)), dim3(256), 0, st, *src, P, bits, rexp, nsort,
+                       keys, vals);
+    int rc = cutc_sort_u64(keys, vals, nsort, st);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(pair_table_perm_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, vals, P, pairs);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void oz_slice_colk_kernel(cutc_colsrc s, int64
         const double *src = s.p[seg] + colsrc_col(s, rok ? r : 0);
         for (int kk = threadIdx.x >> 5; kk < 64; kk += 8) {
             const int64_t k = k0 + kk;
-            tile[kk][rr] = (rok && k < s.K) ? src[k * s.ld] : 0.0;
+            tile[kk][rr] = (rok && k < s.K) ? src[colsrc_k(s, k) * s.ld] : 0.0;
         }
     }
     __syncthreads();

@@ -21,3 +21,9 @@ __device__ static inline int64_t colsrc_col(const cutc_colsrc &s, int64_t r)
         return s.pairs[2 * r] + (int64_t)s.n_ao * s.pairs[2 * r + 1];
     return r;
 }
+
+// source row of K index k in a segment of a column-major stage-2 source
+__device__ static inline int64_t colsrc_k(const cutc_colsrc &s, int64_t k)
+{
+    return s.kperm ? (int64_t)s.kperm[k] : k;
+}

@@ -14,7 +14,8 @@
  *   stage 2:      Hp and NHp for an r1 chunk (4 planes x roundup(n1c2,16) x
  *                 n_pairs doubles) [INT8: + both operands' packed residues,
  *                 2 x nmod x 3 x n_pairs bytes per r1 row, + the GEMM residues
- *                 nmod x n_pairs^2 bytes]
+ *                 nmod x n_pairs^2 bytes, + the block-sparse orderings and
+ *                 step lists]
  * Chunks are balanced (equal sizes rounded up to a multiple of 64).  The
  * modular precisions may also cut K (r2) into passes of k_chunk points whose
  * residue products are added modulo each modulus, when the pair planes of the
@@ -113,9 +114,14 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
     /* stage 2 */
     int64_t fixed2 = 8 * P + 4096, per_r1b = 4 * 8 * P;
     if (crt) {
+        /* + the block-sparse state: the point sort (up to 24 bytes per r1
+         * row), the pair sort and H pair table, flags and step lists of every
+         * (256-pair H tile, 32-point step) over the 3 K segments */
+        const int64_t tiles = CUTC_PK_ROWS_A(P) / CUTC_SP_ROWS;
         fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + nmod * P * P +
-                  2 * rb * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1);
-        per_r1b += 2 * rb * nmod * 3 * CUTC_PK_ROWS_A(P);
+                  2 * rb * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1) + 12 * cutc_sort_len(P) + 8 * P +
+                  4 * tiles + 5 * tiles * 3 * CUTC_PK_KALIGN / 32 + 12 * 256;
+        per_r1b += 2 * rb * nmod * 3 * CUTC_PK_ROWS_A(P) + 24 + (5 * 3 * tiles + 31) / 32;
     } else if (slice) {
         fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + 8 * P * P +
                   2 * 2 * nmod * CUTC_OZ_ROWS(P) * 3 * (CUTC_PK_KALIGN - 1);

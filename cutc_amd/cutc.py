@@ -63,7 +63,7 @@ class Stats(ctypes.Structure):
                 ("ms_host_in", ctypes.c_double), ("ms_host_out", ctypes.c_double),
                 ("ms_long_range", ctypes.c_double), ("ms_short_range", ctypes.c_double),
                 ("ms_stage1", ctypes.c_double), ("ms_stage2", ctypes.c_double),
-                ("k_density", ctypes.c_double)]
+                ("k_density", ctypes.c_double), ("k_density_2e", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

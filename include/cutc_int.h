@@ -157,6 +157,7 @@ typedef struct cutc_stats {
     double k_density;      /* modular stage 1: fraction of the (256-pair tile, K step)
                               tiles the contraction ran (1 = every one; 0 when not
                               measured: FP64, slices, graph capture) */
+    double k_density_2e;   /* the same for the stage-2 contraction (256-pair H tiles) */
 } cutc_stats;
 
 /* FP64, slices = 0 (each precision's own default), everything else 0. */

@@ -1,10 +1,11 @@
-"""Block-sparse stage-1 modular contraction: the build that skips the
-all-zero (128-pair tile, K step) tiles of the pair operand equals the build
-that runs every step (options dense=1), bit for bit, in int8, bf16 and fp16
-modular arithmetic, while running only a fraction of the tiles
-(stats k_density).  Both orderings (r2 by nearest nucleus and distance,
-pairs by support) are applied in both builds, so X and int_2e_ao must agree
-exactly; each is also checked against the FP64 build (<= 1e-10).
+"""Block-sparse modular contractions: the build that skips the all-zero
+(256-pair tile, K step) tiles of the pair operand in stage 1 and of the H
+operand in stage 2 equals the build that runs every step (options
+dense=1), bit for bit, in int8, bf16 and fp16 modular arithmetic, while
+running only a fraction of the tiles (stats k_density, k_density_2e).  The
+orderings (r2 and each r1 chunk by nearest nucleus and distance, pairs by
+support) are applied in both builds, so X and int_2e_ao must agree exactly;
+each is also checked against the FP64 build (<= 1e-10).
 
 Systems: the 20-nucleus basis (n_ao = 200, 157 pair tiles) on 20000 r2
 points, several tiles with empty or short lists and padded ones; a
@@ -63,13 +64,15 @@ def test_sparse_equals_dense(torch, system, precision):
     Xs, Is, sts = build(torch, ds, precision, 0, n1_chunk=512)
     Xd, Id, std = build(torch, ds, precision, 1, n1_chunk=512)
     print(f"  k_density sparse {sts['k_density']:.3f} dense {std['k_density']:.3f}; "
-          f"gemm1 {sts['ms_gemm1']:.2f} / {std['ms_gemm1']:.2f} ms", flush=True)
+          f"k_density_2e {sts['k_density_2e']:.3f}; gemm1 {sts['ms_gemm1']:.2f} / {std['ms_gemm1']:.2f} ms; "
+          f"gemm2 {sts['ms_gemm2']:.2f} / {std['ms_gemm2']:.2f} ms", flush=True)
     assert torch.equal(Xs, Xd), "X differs between the list-driven and dense contraction"
     assert torch.equal(Is, Id), "int_2e_ao differs between the list-driven and dense contraction"
-    assert std["k_density"] == 1.0
-    assert 0.0 < sts["k_density"] <= 1.0
+    assert std["k_density"] == 1.0 and std["k_density_2e"] == 0.0
+    assert 0.0 < sts["k_density"] <= 1.0 and 0.0 < sts["k_density_2e"] <= 1.0
     if name == "cube20":
         assert sts["k_density"] < 0.5, sts["k_density"]
+        assert sts["k_density_2e"] < 0.6, sts["k_density_2e"]
     ex, ei = rel_dev(torch, Xs, X64), rel_dev(torch, Is, I64)
     print(f"  vs FP64 build: X {ex:.2e} int_2e_ao {ei:.2e}", flush=True)
     assert ex <= TOL and ei <= TOL
