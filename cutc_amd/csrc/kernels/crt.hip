@@ -47,6 +47,18 @@ __device__ static inline int residue(double q, int m, double inv_m)
     return r;
 }
 
+// Residue of the quantised value q (an integer-valued double, |q| <= 2^50)
+// modulo m in [-(m / 2), m / 2], for the operand planes.  The quotient is
+// rounded exactly: fl(q inv_m) is within (q / m) 2^-52 < 0.5 / m of q / m,
+// closer than any exact quotient of an odd m comes to a half-integer, so
+// rint picks the nearest integer and q - m rint(q / m) is exact.  The ends
+// +-m/2 are only reached by the even modulus 256 at exact ties, where +128
+// and -128 give the same byte; every other residue equals residue()'s.
+__device__ static inline int residue_q(double q, int m, double inv_m)
+{
+    return (int)fma(-rint(q * inv_m), (double)m, q);
+}
+
 // Row exponent e with ||x||_2 < 2^e from the computed sum of squares ss:
 // the sum carries a relative rounding error far below 2^-29 for the row
 // lengths here (at most ~2^20 terms in fixed-order sums), so ss is inflated
@@ -260,7 +272,7 @@ __device__ static inline void store_residues(const cutc_crt &c, const double (&q
                 unsigned u = 0;
 #pragma unroll
                 for (int e = 0; e < 4; e++)
-                    u |= ((unsigned)residue(q[4 * g + e], c.m[i], c.inv_m[i]) & 0xffu) << (8 * e);
+                    u |= ((unsigned)residue_q(q[4 * g + e], c.m[i], c.inv_m[i]) & 0xffu) << (8 * e);
                 w[g] = (int)u;
             }
             *(v4i *)(dst + i * plane) = v4i{w[0], w[1], w[2], w[3]};
@@ -272,8 +284,8 @@ __device__ static inline void store_residues(const cutc_crt &c, const double (&q
             int w[8];
 #pragma unroll
             for (int g = 0; g < 8; g++)
-                w[g] = (int)(half_bits<KIND>(residue(q[2 * g], c.m[i], c.inv_m[i])) |
-                             (half_bits<KIND>(residue(q[2 * g + 1], c.m[i], c.inv_m[i])) << 16));
+                w[g] = (int)(half_bits<KIND>(residue_q(q[2 * g], c.m[i], c.inv_m[i])) |
+                             (half_bits<KIND>(residue_q(q[2 * g + 1], c.m[i], c.inv_m[i])) << 16));
             *(v4i *)(dst0 + i * plane) = v4i{w[0], w[1], w[2], w[3]};
             *(v4i *)(dst1 + i * plane) = v4i{w[4], w[5], w[6], w[7]};
         }
