@@ -621,7 +621,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     uint8_t *occ = NULL;
     uint32_t *klist = NULL, *svals = NULL;
     uint64_t *skeys = NULL;
-    double *r2p = NULL, *wr2p = NULL, *aos2p = NULL;
+    double *r2p = NULL, *wr2p = NULL, *aos2p = NULL, *box = NULL;
+    void *cws = NULL;
     int rc = CUTC_OK, ev;
     const int slice = !is_crt(opt->precision), f16 = opt->precision == CUTC_PREC_FP16;
     const int emode = slice ? CUTC_ROWEXP_MAX : CUTC_ROWEXP_NORM;
@@ -643,7 +644,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     const int sparse = !slice && kpass == 1 && n2 > 0 && P > 0;
     const int kind = slice ? 0 : crt_kind(opt->precision);
     const int64_t tiles = CUTC_PK_ROWS_B(P) / CUTC_SP_ROWS, KT = sparse ? ldk / CUTC_CRT_KSTEP(kind) : 0;
-    const int64_t nsort = sparse ? cutc_sort_len(n2 > P ? n2 : P) : 0;
+    const int64_t nsort = sparse ? cutc_sort_len(n2) : 0;
 
     ws_t ws;
     {
@@ -656,7 +657,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             need += ws_round(sizeof(double) * 3 * n2) + ws_round(sizeof(double) * n2) +
                     ws_round(sizeof(double) * n_ao * n2) + ws_round(sizeof(uint64_t) * nsort) +
                     ws_round(sizeof(uint32_t) * nsort) + ws_round(sizeof(int) * P) + ws_round(tiles * KT) +
-                    ws_round(sizeof(uint32_t) * tiles * KT) + ws_round(sizeof(int) * tiles);
+                    ws_round(sizeof(uint32_t) * tiles * KT) + ws_round(sizeof(int) * tiles) +
+                    ws_round(sizeof(double) * 6) + ws_round(cutc_cluster_bytes(P, KT));
         TRY(ws_begin(&ws, need));
         if (stats)
             stats->workspace_bytes = need;
@@ -681,19 +683,21 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         WS_TAKE(occ, tiles * KT);
         WS_TAKE(klist, sizeof(uint32_t) * tiles * KT);
         WS_TAKE(kcount, sizeof(int) * tiles);
+        WS_TAKE(box, sizeof(double) * 6);
+        WS_TAKE(cws, cutc_cluster_bytes(P, KT));
     }
 
     ev = tr_begin(tr, CAT_SPLIT);
     if (sparse) {
         /* r2 order, then everything of stage 1 reads the permuted copies */
-        rc = cutc_order_r2(r2, wr2, aos2, n2, n_ao, rn, n_nuc, skeys, svals, nsort, r2p, wr2p, aos2p, st);
+        rc = cutc_order_r2(r2, wr2, aos2, n2, n_ao, box, skeys, svals, nsort, r2p, wr2p, aos2p, st);
         r2 = r2p;
         wr2 = wr2p;
         aos2 = aos2p;
         if (rc == CUTC_OK)
             rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpN, st);
         if (rc == CUTC_OK)
-            rc = cutc_order_pairs(wr2, aos2, n2, n_ao, crt.bits, rexpN, skeys, svals, cutc_sort_len(P), pairs, rexpA,
+            rc = cutc_order_pairs(wr2, aos2, n2, n_ao, crt.bits, rexpN, CUTC_CRT_KSTEP(kind), KT, cws, pairs, rexpA,
                                   st);
         if (rc == CUTC_OK)
             rc = cutc_zero_u8(occ, tiles * KT, st);
@@ -932,8 +936,10 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     double *npart = NULL, *work = NULL;
     int8_t *Apl = NULL, *Bpl = NULL, *Cpl = NULL;
     uint8_t *occ = NULL;
-    uint32_t *perm1 = NULL, *klist = NULL, *pvals = NULL;
-    uint64_t *skeys = NULL, *pkeys = NULL;
+    uint32_t *perm1 = NULL, *klist = NULL;
+    uint64_t *skeys = NULL;
+    double *box = NULL;
+    void *cws = NULL;
     int rc = CUTC_OK, ev;
     const int split = opt->precision != CUTC_PREC_FP64;
     const int slice = split && !is_crt(opt->precision), f16 = opt->precision == CUTC_PREC_FP16;
@@ -951,11 +957,12 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     const size_t bcp = slice ? sizeof(double) * P * P : (size_t)nmod * P * P;
     /* block-sparse state: r1 order of a chunk, sort keys, the H-side pair
      * table, occupancy flags, step lists and counts per 256-pair H tile */
-    const int sparse = split && !slice && r1 != NULL && (n_nuc == 0 || rn != NULL) && P > 0;
+    const int sparse = split && !slice && r1 != NULL && P > 0;
     const int kind = slice ? 0 : crt_kind(opt->precision);
     const int64_t tilesH = CUTC_PK_ROWS_A(P) / CUTC_SP_ROWS;
     const int64_t KT2 = sparse ? 3 * kp_max / CUTC_CRT_KSTEP(kind) : 0;
-    const int64_t nsort = sparse ? cutc_sort_len(n1c) : 0, psort = sparse ? cutc_sort_len(P) : 0;
+    const int64_t nsort = sparse ? cutc_sort_len(n1c) : 0;
+    const int64_t KTH = sparse ? kp_max / CUTC_CRT_KSTEP(kind) : 0; /* steps of the H segment */
     {
         size_t need = ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(double) * plane) +
                       ws_round(sizeof(double) * 3 * plane);
@@ -966,7 +973,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                     ws_round(bcp);
         if (sparse)
             need += ws_round(sizeof(uint64_t) * nsort) + ws_round(sizeof(uint32_t) * nsort) +
-                    ws_round(sizeof(uint64_t) * psort) + ws_round(sizeof(uint32_t) * psort) +
+                    ws_round(sizeof(double) * 6) + ws_round(cutc_cluster_bytes(P, KTH)) +
                     ws_round(sizeof(int) * 2 * P) + ws_round(tilesH * KT2) + ws_round(sizeof(uint32_t) * tilesH * KT2) +
                     ws_round(sizeof(int) * tilesH);
         TRY(ws_begin(&ws, need));
@@ -989,8 +996,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     if (sparse) {
         WS_TAKE(skeys, sizeof(uint64_t) * nsort);
         WS_TAKE(perm1, sizeof(uint32_t) * nsort); /* the point sort's values: the permutation */
-        WS_TAKE(pkeys, sizeof(uint64_t) * psort);
-        WS_TAKE(pvals, sizeof(uint32_t) * psort);
+        WS_TAKE(box, sizeof(double) * 6);
+        WS_TAKE(cws, cutc_cluster_bytes(P, KTH));
         WS_TAKE(pairsH, sizeof(int) * 2 * P);
         WS_TAKE(occ, tilesH * KT2);
         WS_TAKE(klist, sizeof(uint32_t) * tilesH * KT2);
@@ -1006,7 +1013,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
         ev = tr_begin(tr, CAT_SHORTR);
         if (sparse) {
             /* the chunk's r1 order, then the integrands at the permuted points */
-            rc = cutc_order_points(r1 + 3 * (r0 + i0), n1e, rn, n_nuc, skeys, perm1, cutc_sort_len(n1e), st);
+            rc = cutc_order_points(r1 + 3 * (r0 + i0), n1e, box, skeys, perm1, cutc_sort_len(n1e), st);
         }
         if (rc == CUTC_OK)
             rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, NHp, P, plane,
@@ -1030,7 +1037,8 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
             if (rc == CUTC_OK)
                 rc = cutc_rowexp_colk(&sh, P, emode, npart, rexpB, st);
             if (rc == CUTC_OK)
-                rc = cutc_order_pairs_colk(&sh, n_ao, crt0.bits, rexpB, pkeys, pvals, psort, pairsH, st);
+                rc = cutc_order_pairs_colk(&sh, n_ao, crt0.bits, rexpB, CUTC_CRT_KSTEP(kind),
+                                           (n1e + CUTC_CRT_KSTEP(kind) - 1) / CUTC_CRT_KSTEP(kind), cws, pairsH, st);
             tr_end(tr, ev);
             TRY(rc);
             ev = tr_begin(tr, CAT_SHORTR);

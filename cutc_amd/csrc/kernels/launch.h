@@ -232,7 +232,6 @@ int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t
  * padded to CUTC_CRT_LIST_ALIGN(kind) steps (the GEMM pipelines' unroll). */
 #define CUTC_CRT_KSTEP(kind) ((kind) == CUTC_CRT_I8 ? 64 : 32)
 #define CUTC_CRT_LIST_ALIGN(kind) ((kind) == CUTC_CRT_I8 ? 6 : 24)
-#define CUTC_PAIR_SEGS 16 /* r2 segments of the pair support keys */
 #define CUTC_SP_ROWS 256  /* pair rows per B tile of the step lists */
 /* the same GEMM over the steps klist[t KT + s], s < kcount[t], of tile t
  * (KT = kpad / KSTEP) of the B side (list_a = 0) or of the A side
@@ -249,25 +248,31 @@ int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2, const doub
 int64_t cutc_sort_len(int64_t n);
 /* ascending sort of n (key, value) pairs, n = cutc_sort_len(n) */
 int cutc_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, hipStream_t st);
-/* point order (nearest nucleus, then distance) of the n points r (n x 3):
- * the permutation in vals[0 .. n); keys / vals hold nsort = cutc_sort_len(n) */
-int cutc_order_points(const double *r, int64_t n, const double *rn, int n_nuc, uint64_t *keys, uint32_t *vals,
-                      int64_t nsort, hipStream_t st);
-/* pair order by support over a column source (rows = packed pairs, one
- * segment): keys of the quantised rows rint(x 2^(bits - rexp[row])) over
- * CUTC_PAIR_SEGS equal K segments, sorted; the sorted pair table in pairs */
-int cutc_order_pairs_colk(const cutc_colsrc *src, int n_ao, int bits, const int *rexp, uint64_t *keys,
-                          uint32_t *vals, int64_t nsort, int *pairs, hipStream_t st);
-/* r2 order: perm in vals[0 .. n2) and the permuted copies r2p (n2 x 3),
- * wr2p (n2) and aos2p (n_ao x n2) of r2, wr2 and the AO values aos2
- * (n_ao x n2); keys / vals hold nsort = cutc_sort_len(n2) entries */
-int cutc_order_r2(const double *r2, const double *wr2, const double *aos2, int64_t n2, int n_ao, const double *rn,
-                  int n_nuc, uint64_t *keys, uint32_t *vals, int64_t nsort, double *r2p, double *wr2p,
-                  double *aos2p, hipStream_t st);
-/* pair order from the natural-order exponents rexp_nat: the sorted table
- * pairs (2 ints per row) and exponents rexp; nsort = cutc_sort_len(P) */
+/* point order of the n points r (n x 3): Morton (Z-order) code in their
+ * bounding box (box: 6 doubles of scratch); the permutation in vals[0 .. n);
+ * keys / vals hold nsort = cutc_sort_len(n) */
+int cutc_order_points(const double *r, int64_t n, double *box, uint64_t *keys, uint32_t *vals, int64_t nsort,
+                      hipStream_t st);
+/* r2 order (cutc_order_points): perm in vals[0 .. n2) and the permuted
+ * copies r2p (n2 x 3), wr2p (n2) and aos2p (n_ao x n2) of r2, wr2 and the AO
+ * values aos2 (n_ao x n2) */
+int cutc_order_r2(const double *r2, const double *wr2, const double *aos2, int64_t n2, int n_ao, double *box,
+                  uint64_t *keys, uint32_t *vals, int64_t nsort, double *r2p, double *wr2p, double *aos2p,
+                  hipStream_t st);
+/* scratch bytes of the pair clustering of P rows over nsteps K steps */
+size_t cutc_cluster_bytes(int64_t P, int64_t nsteps);
+/* pair order: the support of every pair (the K steps of kstep points, 32 or
+ * 64, where rint(w chi_k chi_l 2^(bits - rexp_nat[p])) is non-zero, nsteps
+ * steps) and a greedy clustering of the pairs into tiles of CUTC_SP_ROWS
+ * with similar supports; the ordered pair table pairs (2 ints per row) and
+ * exponents rexp; ws: cutc_cluster_bytes(P, nsteps) bytes */
 int cutc_order_pairs(const double *wr2, const double *aos2, int64_t n2, int n_ao, int bits, const int *rexp_nat,
-                     uint64_t *keys, uint32_t *vals, int64_t nsort, int *pairs, int *rexp, hipStream_t st);
+                     int kstep, int64_t nsteps, void *ws, int *pairs, int *rexp, hipStream_t st);
+/* the same over a one-segment column source (rows = packed pairs, K = the
+ * chunk's points): supports of rint(x 2^(bits - rexp[row])); the ordered
+ * pair table in pairs */
+int cutc_order_pairs_colk(const cutc_colsrc *src, int n_ao, int bits, const int *rexp, int kstep, int64_t nsteps,
+                          void *ws, int *pairs, hipStream_t st);
 int cutc_zero_u8(uint8_t *p, int64_t n, hipStream_t st);
 /* per-tile step lists from the flags occ (tiles x KT), KT a multiple of align */
 int cutc_klists(const uint8_t *occ, int64_t tiles, int64_t KT, int align, uint32_t *list, int *count,

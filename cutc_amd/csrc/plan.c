@@ -45,9 +45,10 @@ static int64_t balanced(int64_t n, int64_t cmax, int64_t align)
  * tile, each region rounded up to 256 bytes. */
 static int64_t sparse_bytes(int64_t n2, int n_ao, int64_t P, int64_t ldk, int kind)
 {
-    const int64_t ns = cutc_sort_len(n2 > P ? n2 : P);
+    const int64_t ns = cutc_sort_len(n2);
     const int64_t tiles = CUTC_PK_ROWS_B(P) / CUTC_SP_ROWS, kt = ldk / CUTC_CRT_KSTEP(kind);
-    return 8 * ((int64_t)n_ao + 4) * n2 + 12 * ns + 4 * P + 5 * tiles * kt + 4 * tiles + 9 * 256;
+    return 8 * ((int64_t)n_ao + 4) * n2 + 12 * ns + 4 * P + 5 * tiles * kt + 4 * tiles +
+           (int64_t)cutc_cluster_bytes(P, kt) + 11 * 256;
 }
 
 int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, int slices, size_t budget,
@@ -118,10 +119,12 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
          * row), the pair sort and H pair table, flags and step lists of every
          * (256-pair H tile, 32-point step) over the 3 K segments */
         const int64_t tiles = CUTC_PK_ROWS_A(P) / CUTC_SP_ROWS;
+        const int64_t ks = rb == 1 ? 64 : 32; /* points per K step of the lists */
         fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + nmod * P * P +
-                  2 * rb * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1) + 12 * cutc_sort_len(P) + 8 * P +
-                  4 * tiles + 5 * tiles * 3 * CUTC_PK_KALIGN / 32 + 12 * 256;
-        per_r1b += 2 * rb * nmod * 3 * CUTC_PK_ROWS_A(P) + 24 + (5 * 3 * tiles + 31) / 32;
+                  2 * rb * nmod * CUTC_PK_ROWS_A(P) * 3 * (CUTC_PK_KALIGN - 1) + 8 * P +
+                  (int64_t)cutc_cluster_bytes(P, (CUTC_PK_KALIGN + 63 * ks) / ks) + 48 +
+                  4 * tiles + 5 * tiles * 3 * CUTC_PK_KALIGN / 32 + 14 * 256;
+        per_r1b += 2 * rb * nmod * 3 * CUTC_PK_ROWS_A(P) + 24 + (5 * 3 * tiles + 31) / 32 + (8 * P + 64 * ks - 1) / (64 * ks);
     } else if (slice) {
         fixed2 += 16 * P + 8 * CUTC_CRT_KSPLIT * P + 8 * P * P +
                   2 * 2 * nmod * CUTC_OZ_ROWS(P) * 3 * (CUTC_PK_KALIGN - 1);
