@@ -1058,13 +1058,19 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
 
 // ---------------------------------------------------------------- rebuild
 
-// V from the residues c_i of one element (see crt_host.c), as a double
+// V from the residues c_i of one element (see crt_host.c), as a double.
+// u_i = (c_i y_i) mod m_i in [0, m_i): |c_i y_i| < 2^15, so the float
+// quotient is within 2^-7 of the exact one and its floor is off by at most
+// one, which the two corrections undo (checked against the integer
+// remainder for every modulus, y and residue).
 __device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, int64_t stride)
 {
     double sh = 0.0, sm = 0.0, sl = 0.0;
     for (int i = 0; i < c.n; i++) {
-        const int ci = (int)C[(int64_t)i * stride];
-        int u = (ci * c.y[i]) % c.m[i];
+        const int x = (int)C[(int64_t)i * stride] * c.y[i];
+        int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
+        if (u >= c.m[i])
+            u -= c.m[i];
         if (u < 0)
             u += c.m[i];
         const double du = (double)u;
@@ -1079,20 +1085,33 @@ __device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, in
     return (vh * two80 + vm * two40) + vl;
 }
 
+// Block -> (row of C, block of 256 columns) of the rebuild kernels: one
+// row's npb blocks are consecutive in each XCD's contiguous range of the
+// grid (cutc_xcd_remap), so the scattered stores of a row (its whole output
+// line of n_ao^2 doubles) meet in one L2 and leave it as full lines; rows
+// past the grid repeat with stride gridDim.x / npb.
+__device__ static inline void rebuild_rowcol(int npb, int64_t &row, int64_t &p)
+{
+    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    row = t / npb;
+    p = (int64_t)(t % npb) * 256 + threadIdx.x;
+}
+
 // thread per (p, j), p fastest
 __global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t *__restrict__ C,
                                                         int64_t P, int64_t N, const int *__restrict__ rexpA,
                                                         const int *__restrict__ rexpB,
                                                         const int *__restrict__ pairs, int n_ao, double *X,
-                                                        int64_t n1, int64_t n1c, int64_t i0)
+                                                        int64_t n1, int64_t n1c, int64_t i0, int npb)
 {
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int64_t j0, p;
+    rebuild_rowcol(npb, j0, p);
     if (p >= P)
         return;
     const int k = pairs[2 * p], l = pairs[2 * p + 1];
     const int64_t n2 = (int64_t)n_ao * n_ao;
     const int ea = rexpA[p] - 2 * c.bits;
-    for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
+    for (int64_t j = j0; j < N; j += gridDim.x / npb) {
         const double V = crt_value(c, C + j * P + p, P * N);
         const double v = ldexp(V, ea + rexpB[j]);
         const int64_t m = j / n1c, i1 = j - m * n1c;
@@ -1108,9 +1127,11 @@ extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P,
 {
     if (P <= 0 || N <= 0)
         return 0;
-    const int64_t gy = N < 65535 ? N : 65535;
-    hipLaunchKernelGGL(rebuild_x_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, *c, C,
-                       P, N, rexpA, rexpB, pairs, n_ao, X, n1, n1c, i0);
+    const int64_t npb = (P + 255) / 256, rows = N < 1048576 / npb + 1 ? N : 1048576 / npb + 1;
+    if (npb > 1048576)
+        return CUTC_ERR_ARG;
+    hipLaunchKernelGGL(rebuild_x_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, N, rexpA, rexpB,
+                       pairs, n_ao, X, n1, n1c, i0, (int)npb);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -1121,16 +1142,18 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
                                                         const int *__restrict__ rexpA,
                                                         const int *__restrict__ rexpB, double alpha,
                                                         const int *__restrict__ pairs,
-                                                        const int *__restrict__ pairsB, int n_ao, double *M, int mode)
+                                                        const int *__restrict__ pairsB, int n_ao, double *M, int mode,
+                                                        int npb)
 {
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int64_t q0, p;
+    rebuild_rowcol(npb, q0, p);
     if (p >= P)
         return;
     const int k = pairs[2 * p], l = pairs[2 * p + 1];
     const int64_t n2 = (int64_t)n_ao * n_ao;
     const int64_t kl = k + (int64_t)n_ao * l, lk = l + (int64_t)n_ao * k;
     const int ea = rexpA[p] - 2 * c.bits;
-    for (int64_t q = blockIdx.y; q < P; q += gridDim.y) {
+    for (int64_t q = q0; q < P; q += gridDim.x / npb) {
     const double V = crt_value(c, C + q * P + p, P * P);
     const double v = alpha * ldexp(V, ea + rexpB[q]);
     const int a = pairsB[2 * q], b = pairsB[2 * q + 1];
@@ -1161,8 +1184,10 @@ extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P,
 {
     if (P <= 0)
         return 0;
-    const int64_t gy = P < 65535 ? P : 65535;
-    hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)gy), dim3(256), 0, st, *c, C,
-                       P, rexpA, rexpB, alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, mode);
+    const int64_t npb = (P + 255) / 256, rows = P < 1048576 / npb + 1 ? P : 1048576 / npb + 1;
+    if (npb > 1048576)
+        return CUTC_ERR_ARG;
+    hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, rexpA, rexpB,
+                       alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, mode, (int)npb);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
