@@ -56,7 +56,7 @@ BATCH_PRECISIONS = ("fp64", "bf16", "bf16crt", "int8")
 # n2 = 200000, the library's default moduli), from two rocprofv3 --pmc passes (FETCH_SIZE,
 # WRITE_SIZE) scaled by a 1 GiB copy kernel measured in the same run
 # (profiles/cube20_traffic.txt); not measured for other shapes.
-MEASURED_TRAFFIC = {"int8": 205.0e9, "fp64": 17.83e12,
+MEASURED_TRAFFIC = {"int8": 132.0e9, "fp64": 17.83e12,
                     # the transform's GEMM launch (n_ao = n_mo = 200, n_grid1 = 60000) and the AO kernel
                     # (260000 points, n_ao = 200), profiles/ao_to_mo_nao200_kernel_stats.txt and
                     # profiles/aos_cube20_kernel_stats.txt
