@@ -593,12 +593,30 @@ done:
     return rc;
 }
 
+/* sum of the n step counts of the lists (device), after the work queued on st */
+static int kcount_sum(const int *kcount, int64_t n, hipStream_t st, double *sum)
+{
+    int *h = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    if (!h)
+        return CUTC_ERR_NOMEM;
+    int rc = hipMemcpyAsync(h, kcount, sizeof(int) * n, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                     hipStreamSynchronize(st) == hipSuccess
+                 ? CUTC_OK
+                 : CUTC_ERR_HIP;
+    double s = 0.0;
+    for (int64_t i = 0; i < n && rc == CUTC_OK; i++)
+        s += h[i];
+    free(h);
+    *sum = s;
+    return rc;
+}
+
 /* Stage 1 with a split-precision contraction.  INT8 / BF16_CRT / FP16_CRT:
  * the modular (CRT) scheme, pair residues quantised on the fly from w2 and
  * chi, each G chunk quantised after the Jastrow block, one int8 (or bf16 /
  * fp16) GEMM per modulus, CRT rebuild scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
  * pairs and of each G chunk, one slice GEMM into FP64, scattered into X.
- * Modular with one K pass: block-sparse.  The r2 points are reordered
+ * Modular: block-sparse.  The r2 points are reordered
  * (nearest nucleus, then distance; the Jastrow, pair weights and AO values
  * all use the permuted copies, so the contraction over r2 is unchanged),
  * the pair rows sorted by the r2 segments where their quantised values are
@@ -641,9 +659,11 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
                             : rb * nmod * CUTC_PK_ROWS_A(4 * n1c) * kc;
     const size_t bC = slice ? sizeof(double) * 4 * n1c * P : (size_t)nmod * 4 * n1c * P;
     /* block-sparse state (the planner's sparse_bytes) */
-    const int sparse = !slice && kpass == 1 && n2 > 0 && P > 0;
+    const int sparse = !slice && n2 > 0 && P > 0;
     const int kind = slice ? 0 : crt_kind(opt->precision);
+    /* KT: the steps of all r2 points (pair supports); KTp: of one K pass (flags and lists) */
     const int64_t tiles = CUTC_PK_ROWS_B(P) / CUTC_SP_ROWS, KT = sparse ? ldk / CUTC_CRT_KSTEP(kind) : 0;
+    const int64_t KTp = sparse ? kc / CUTC_CRT_KSTEP(kind) : 0;
     const int64_t nsort = sparse ? cutc_sort_len(n2) : 0;
 
     ws_t ws;
@@ -656,8 +676,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         if (sparse)
             need += ws_round(sizeof(double) * 3 * n2) + ws_round(sizeof(double) * n2) +
                     ws_round(sizeof(double) * n_ao * n2) + ws_round(sizeof(uint64_t) * nsort) +
-                    ws_round(sizeof(uint32_t) * nsort) + ws_round(sizeof(int) * P) + ws_round(tiles * KT) +
-                    ws_round(sizeof(uint32_t) * tiles * KT) + ws_round(sizeof(int) * tiles) +
+                    ws_round(sizeof(uint32_t) * nsort) + ws_round(sizeof(int) * P) + ws_round(tiles * KTp) +
+                    ws_round(sizeof(uint32_t) * tiles * KTp) + ws_round(sizeof(int) * tiles) +
                     ws_round(sizeof(double) * 6) + ws_round(cutc_cluster_bytes(P, KT));
         TRY(ws_begin(&ws, need));
         if (stats)
@@ -680,8 +700,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         WS_TAKE(skeys, sizeof(uint64_t) * nsort);
         WS_TAKE(svals, sizeof(uint32_t) * nsort);
         WS_TAKE(rexpN, sizeof(int) * P);
-        WS_TAKE(occ, tiles * KT);
-        WS_TAKE(klist, sizeof(uint32_t) * tiles * KT);
+        WS_TAKE(occ, tiles * KTp);
+        WS_TAKE(klist, sizeof(uint32_t) * tiles * KTp);
         WS_TAKE(kcount, sizeof(int) * tiles);
         WS_TAKE(box, sizeof(double) * 6);
         WS_TAKE(cws, cutc_cluster_bytes(P, KT));
@@ -699,11 +719,11 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         if (rc == CUTC_OK)
             rc = cutc_order_pairs(wr2, aos2, n2, n_ao, crt.bits, rexpN, CUTC_CRT_KSTEP(kind), KT, cws, pairs, rexpA,
                                   st);
-        if (rc == CUTC_OK)
+        if (rc == CUTC_OK && kpass == 1)
             rc = cutc_zero_u8(occ, tiles * KT, st);
-        if (rc == CUTC_OK)
+        if (rc == CUTC_OK && kpass == 1)
             rc = cutc_crt_quant_pairw_pk_occ(&crt, wr2, aos2, n2, n2, n_ao, ldk, rexpA, pairs, Apl, occ, st);
-        if (rc == CUTC_OK)
+        if (rc == CUTC_OK && kpass == 1)
             rc = cutc_klists(occ, tiles, KT, CUTC_CRT_LIST_ALIGN(kind), klist, kcount, st);
     } else {
         rc = cutc_launch_pair_table(n_ao, pairs, st);
@@ -718,6 +738,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     TRY(rc);
     const uint32_t *kl = (sparse && !opt->dense) ? klist : NULL;
     const int *kc_ = (sparse && !opt->dense) ? kcount : NULL;
+    double dens_sum = 0.0, dens_den = 0.0; /* executed tiles of the passes of the first chunk */
 
     int nchunks = 0;
     for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
@@ -747,19 +768,35 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             TRY(rc);
         }
         /* several K passes (modular only): the residue planes of both
-         * operands for r2 points [k0, k0 + kn), products added into Cpl */
+         * operands for r2 points [k0, k0 + kn) (block-sparse: with the flags
+         * and step lists of the pass), products added into Cpl */
         for (int64_t kp = 0; kp < kpass && kpass > 1; kp++) {
             const int64_t k0 = kp * kc, kn = (n2 - k0 < kc) ? n2 - k0 : kc;
             ev = tr_begin(tr, CAT_SPLIT);
-            rc = cutc_crt_quant_pairw_pk_range(&crt, wr2 + k0, aos2 + k0, n2, kn, n_ao, kc, rexpA, Apl, st);
+            if (sparse) {
+                rc = cutc_zero_u8(occ, tiles * KTp, st);
+                if (rc == CUTC_OK)
+                    rc = cutc_crt_quant_pairw_pk_occ(&crt, wr2 + k0, aos2 + k0, n2, kn, n_ao, kc, rexpA, pairs, Apl,
+                                                     occ, st);
+                if (rc == CUTC_OK)
+                    rc = cutc_klists(occ, tiles, KTp, CUTC_CRT_LIST_ALIGN(kind), klist, kcount, st);
+            } else {
+                rc = cutc_crt_quant_pairw_pk_range(&crt, wr2 + k0, aos2 + k0, n2, kn, n_ao, kc, rexpA, Apl, st);
+            }
             if (rc == CUTC_OK)
                 rc = cutc_crt_quant_k_pk(&crt, G + k0, ldk, 4 * n1e, kn, kc, rexpG, Gpl, st);
             tr_end(tr, ev);
             TRY(rc);
             ev = tr_begin(tr, CAT_GEMM1);
-            rc = cutc_crt_gemm_pk(&crt, Gpl, 4 * n1e, Apl, P, kc, Cpl, kp > 0, st);
+            rc = cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, kc, kl, kc_, 0, Cpl, kp > 0, st);
             tr_end(tr, ev);
             TRY(rc);
+            if (kl && i0 == 0 && stats && !tr->capture) {
+                double sum = 0.0;
+                TRY(kcount_sum(kcount, tiles, st, &sum));
+                dens_sum += sum;
+                dens_den += (double)tiles * (double)KTp;
+            }
         }
         ev = tr_begin(tr, CAT_SPLIT);
         rc = slice ? cutc_oz_scatter_x((const double *)Cpl, P, 4 * n1e, pairs, n_ao, X, n1, n1e, i0, st)
@@ -780,18 +817,12 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     if (stats && !slice && !tr->capture) {
         /* executed fraction of the (256-pair tile, K step) tiles */
         double d = 1.0;
-        if (kl && tiles > 0) {
-            int *h = (int *)malloc(sizeof(int) * tiles);
-            if (!h)
-                TRY(CUTC_ERR_NOMEM);
-            rc = hipMemcpy(h, kcount, sizeof(int) * tiles, hipMemcpyDeviceToHost) == hipSuccess ? CUTC_OK
-                                                                                              : CUTC_ERR_HIP;
+        if (kl && tiles > 0 && kpass == 1) {
             double sum = 0.0;
-            for (int64_t i = 0; i < tiles; i++)
-                sum += h[i];
-            free(h);
-            TRY(rc);
+            TRY(kcount_sum(kcount, tiles, st, &sum));
             d = sum / ((double)tiles * (double)KT);
+        } else if (kl && dens_den > 0) {
+            d = dens_sum / dens_den;
         }
         stats->k_density = d;
     }

@@ -9,8 +9,8 @@
  *   INT8 stage 1: pairs + row maxima + pair residues (nmod x n_pairs x ldk
  *                 bytes, fragment-packed) + per r1 row: G (4 ldk doubles), its
  *                 residues (4 nmod ldk bytes) and the GEMM residues (nmod x 4
- *                 n_pairs bytes) [one K pass: + the block-sparse orderings
- *                 and step lists, sparse_bytes]
+ *                 n_pairs bytes) + the block-sparse orderings and the
+ *                 step lists of one K pass (sparse_bytes)
  *   stage 2:      Hp and NHp for an r1 chunk (4 planes x roundup(n1c2,16) x
  *                 n_pairs doubles) [INT8: + both operands' packed residues,
  *                 2 x nmod x 3 x n_pairs bytes per r1 row, + the GEMM residues
@@ -43,11 +43,11 @@ static int64_t balanced(int64_t n, int64_t cmax, int64_t align)
  * wr2 and AO values, the sort keys and values, the natural-order pair
  * exponents, the occupancy flags, step lists and counts of every 256-pair
  * tile, each region rounded up to 256 bytes. */
-static int64_t sparse_bytes(int64_t n2, int n_ao, int64_t P, int64_t ldk, int kind)
+static int64_t sparse_bytes(int64_t n2, int n_ao, int64_t P, int64_t ldk, int64_t kc, int kind)
 {
     const int64_t ns = cutc_sort_len(n2);
     const int64_t tiles = CUTC_PK_ROWS_B(P) / CUTC_SP_ROWS, kt = ldk / CUTC_CRT_KSTEP(kind);
-    return 8 * ((int64_t)n_ao + 4) * n2 + 12 * ns + 4 * P + 5 * tiles * kt + 4 * tiles +
+    return 8 * ((int64_t)n_ao + 4) * n2 + 12 * ns + 4 * P + 5 * tiles * (kc / CUTC_CRT_KSTEP(kind)) + 4 * tiles +
            (int64_t)cutc_cluster_bytes(P, kt) + 11 * 256;
 }
 
@@ -77,8 +77,8 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
         for (int64_t nk = 1; nk <= 64; nk++) {
             kc = rup(cdiv(ldk, nk), CUTC_PK_KALIGN);
             fixed1 = 16 * P + rb * nmod * CUTC_PK_ROWS_B(P) * kc + rb * nmod * 511 * kc + 4096;
-            if (nk == 1) /* one K pass: the block-sparse contraction's orderings and lists */
-                fixed1 += sparse_bytes(n2, n_ao, P, ldk, precision == CUTC_PREC_INT8 ? CUTC_CRT_I8 : CUTC_CRT_BF16);
+            /* the block-sparse contraction's orderings, and the flags and lists of one pass */
+            fixed1 += sparse_bytes(n2, n_ao, P, ldk, kc, precision == CUTC_PREC_INT8 ? CUTC_CRT_I8 : CUTC_CRT_BF16);
             per_r1 = 32 * ldk + 32 + 4 * rb * nmod * kc + nmod * 4 * P;
             if (n_nuc > 32)
                 fixed1 += 8 * (int64_t)n_nuc * ldk;

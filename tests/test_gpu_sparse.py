@@ -3,9 +3,9 @@
 operand in stage 2 equals the build that runs every step (options
 dense=1), bit for bit, in int8, bf16 and fp16 modular arithmetic, while
 running only a fraction of the tiles (stats k_density, k_density_2e).  The
-orderings (r2 and each r1 chunk by nearest nucleus and distance, pairs by
-support) are applied in both builds, so X and int_2e_ao must agree exactly;
-each is also checked against the FP64 build (<= 1e-10).
+orderings (r2 and each r1 chunk along a Morton curve, pair rows clustered by
+their step supports) are applied in both builds, so X and int_2e_ao must
+agree exactly; each is also checked against the FP64 build (<= 1e-10).
 
 Systems: the 20-nucleus basis (n_ao = 200, 157 pair tiles) on 20000 r2
 points, several tiles with empty or short lists and padded ones; a

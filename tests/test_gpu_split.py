@@ -84,7 +84,8 @@ def k_pass_limit(s, precision):
 def test_modular_k_passes_match_one_pass(torch, precision):
     """Under a memory limit the modular stage 1 runs K in several passes
     whose residue products are added modulo each modulus: X and int_2e_ao
-    are bit-identical to the one-pass build (exact integer arithmetic)."""
+    are bit-identical to the one-pass build (exact integer arithmetic), and
+    the passes' list-driven contractions to the same passes run dense."""
     s = synth.make_system(6, 40, 300, 5000, 77, name="kpass")
     lim, p = k_pass_limit(s, precision)
     print(f"{precision} K passes: limit {lim / 1e6:.0f} MB, k_chunk {p.k_chunk} of ldk {p.ldk}", flush=True)
@@ -92,6 +93,10 @@ def test_modular_k_passes_match_one_pass(torch, precision):
     Xk, Ik, st = run(torch, s, precision=precision, mem_limit=lim)
     assert np.array_equal(Xk, X1) and np.array_equal(Ik, I1)
     assert st["workspace_bytes"] <= lim
+    Xd, Id, std = run(torch, s, precision=precision, mem_limit=lim, dense=1)
+    print(f"  k_density of the passes {st['k_density']:.3f} (dense {std['k_density']:.3f})", flush=True)
+    assert np.array_equal(Xd, X1) and np.array_equal(Id, I1)
+    assert 0.0 < st["k_density"] <= 1.0 and std["k_density"] == 1.0
 
 
 def test_int8_fewer_moduli_degrade_gracefully(torch):
