@@ -636,6 +636,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     double *G = NULL, *f2tab = NULL, *tab = NULL;
     int *rexpA = NULL, *rexpG = NULL, *rexpN = NULL, *kcount = NULL;
     int8_t *Apl = NULL, *Gpl = NULL, *Cpl = NULL;
+    double *gpart = NULL; /* the Jastrow kernel's partial row sums of G (modular) */
     uint8_t *occ = NULL;
     uint32_t *klist = NULL, *svals = NULL;
     uint64_t *skeys = NULL;
@@ -673,6 +674,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
                       ws_round(sizeof(double) * 4 * n1c * ldk) +
                       (n_nuc > 32 ? ws_round(sizeof(double) * n_nuc * ldk) : 0) +
                       ws_round(sizeof(double) * (prog->ts * n1c + 1));
+        if (!slice)
+            need += ws_round(sizeof(double) * 4 * n1c * cutc_jast_norm_blocks(ldk));
         if (sparse)
             need += ws_round(sizeof(double) * 3 * n2) + ws_round(sizeof(double) * n2) +
                     ws_round(sizeof(double) * n_ao * n2) + ws_round(sizeof(uint64_t) * nsort) +
@@ -693,6 +696,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     if (n_nuc > 32)
         WS_TAKE(f2tab, sizeof(double) * n_nuc * ldk);
     WS_TAKE(tab, sizeof(double) * (prog->ts * n1c + 1));
+    if (!slice)
+        WS_TAKE(gpart, sizeof(double) * 4 * n1c * cutc_jast_norm_blocks(ldk));
     if (sparse) {
         WS_TAKE(r2p, sizeof(double) * 3 * n2);
         WS_TAKE(wr2p, sizeof(double) * n2);
@@ -746,13 +751,24 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         nchunks++;
         ev = tr_begin(tr, CAT_JAST);
         rc = cutc_launch_jast_prep(r1, i0, n1e, rn, n_nuc, dmono, prog->n_mono, tab, prog->ts, st);
-        if (rc == CUTC_OK)
+        /* modular: the pair kernel also sums the squares of the G rows it
+         * writes (their 2-norm exponents), where its staged form runs */
+        int norms = 0;
+        if (rc == CUTC_OK && gpart) {
+            rc = cutc_launch_jast_pairs_norms(r1, r2, rn, n_nuc, tab, prog->ts, dcls, prog->n_cls, prog->kmax, i0,
+                                              n1e, n2, ldk, G, ldk, n1e * ldk, gpart, st);
+            norms = rc == CUTC_OK;
+            if (rc == CUTC_ERR_UNSUPPORTED)
+                rc = CUTC_OK;
+        }
+        if (rc == CUTC_OK && !norms)
             rc = cutc_launch_jast_pairs(r1, r2, rn, n_nuc, tab, prog->ts, dcls, prog->n_cls, prog->kmax, i0, n1e, 0,
                                         n2, ldk, G, ldk, n1e * ldk, f2tab, st);
         tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
-        rc = cutc_rowexp_k(G, ldk, 4 * n1e, n2, emode, rexpG, st);
+        rc = norms ? cutc_rowexp_finish(gpart, cutc_jast_norm_blocks(ldk), 4 * n1e, CUTC_ROWEXP_NORM, rexpG, st)
+                   : cutc_rowexp_k(G, ldk, 4 * n1e, n2, emode, rexpG, st);
         if (rc == CUTC_OK && kpass == 1)
             rc = slice ? cutc_oz_slice_k(nmod, f16, G, ldk, 4 * n1e, n2, ldk, CUTC_OZ_ROWS(4 * n1e), rexpG,
                                          (uint16_t *)Gpl, st)

@@ -193,6 +193,21 @@ __global__ __launch_bounds__(256) void rowexp_finish_kernel(const double *__rest
     rexp[r] = rs_exponent<MODE>(v);
 }
 
+extern "C" int cutc_rowexp_finish(const double *partial, int64_t ks, int64_t rows, int mode, int *rexp,
+                                  hipStream_t st)
+{
+    if (rows <= 0)
+        return 0;
+    if (ks < 1 || ks > 2147483647LL)
+        return CUTC_ERR_ARG;
+    const dim3 g2((unsigned)((rows + 255) / 256));
+    if (mode == CUTC_ROWEXP_MAX)
+        hipLaunchKernelGGL(rowexp_finish_kernel<1>, g2, dim3(256), 0, st, partial, (int)ks, rows, rexp);
+    else
+        hipLaunchKernelGGL(rowexp_finish_kernel<0>, g2, dim3(256), 0, st, partial, (int)ks, rows, rexp);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
 extern "C" int cutc_rowexp_colk(const cutc_colsrc *src, int64_t rows, int mode, double *partial, int *rexp,
                                 hipStream_t st)
 {

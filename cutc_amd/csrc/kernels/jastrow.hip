@@ -19,6 +19,10 @@
 
 #define JAST_THREADS 256
 
+// sum over the 64 lanes of a wave (the device library's DPP reduction, a
+// fixed order), returned to every lane
+extern "C" __device__ double __ockl_wfred_add_f64(double);
+
 // Row i1 of the per-r1 table: zero it, then add every monomial.
 __global__ __launch_bounds__(256) void jast_prep_kernel(const double *__restrict__ r1, int64_t ii0, int64_t n1,
                                                         const double *__restrict__ rn, int n_nuc,
@@ -205,14 +209,34 @@ __device__ static inline void rbar_grad_1div(double dx, double dy, double dz, do
 #define JAST_LDS_R1B 16
 #define JAST_ROWS_PER_BLOCK 128
 #define JAST_PPT 2 /* r2 points per thread */
+//
+// part (optional): the block's sums of squares of every G row it writes
+// (row m n1 + i1 of component m), over its 512 points, at
+// part[blockIdx.x 4 n1 + m n1 + i1], for the row exponents of the modular
+// quantisers (rowexp_finish_kernel sums them over the blocks in order).
+// Fixed-order reductions: the wave sum of the device library, then the 4
+// waves in order.
 template <int NMAX, bool ONE>
 __global__ __launch_bounds__(JAST_THREADS) void jast_pair_lds_kernel(
     const double *__restrict__ r1, const double *__restrict__ r2, const double *__restrict__ rn, int n_nuc,
     const double *__restrict__ tab, int64_t ts, const int *__restrict__ cls, int n_cls, int kmax,
     double *__restrict__ G, int64_t ii0, int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, int64_t ldg,
-    int64_t plane)
+    int64_t plane, double *__restrict__ part)
 {
     extern __shared__ double s_tab[]; // JAST_LDS_R1B rows x ts
+    __shared__ double s_part[JAST_LDS_R1B][4][JAST_THREADS / 64]; // [row][component][wave]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the previous batch's row sums: the 4 waves in order, one value per (row, component)
+    auto flush = [&](int64_t row0, int nrows) {
+        const int t = threadIdx.x;
+        if (part && t < 4 * nrows) {
+            const int r = t >> 2, m = t & 3;
+            const double v = ((s_part[r][m][0] + s_part[r][m][1]) + s_part[r][m][2]) + s_part[r][m][3];
+            part[(int64_t)blockIdx.x * 4 * n1 + (int64_t)m * n1 + row0 + r] = v;
+        }
+    };
+    int64_t prev_row0 = 0;
+    int prev_nr = 0;
     const int j1 = ONE ? cls[0] : 0, k1 = ONE ? cls[1] : 0;
     int64_t i2v[JAST_PPT];
     bool act[JAST_PPT], live[JAST_PPT];
@@ -243,6 +267,7 @@ __global__ __launch_bounds__(JAST_THREADS) void jast_pair_lds_kernel(
     for (int64_t i1_0 = (int64_t)blockIdx.y * JAST_LDS_R1B; i1_0 < n1; i1_0 += (int64_t)gridDim.y * JAST_LDS_R1B) {
         const int nr = (n1 - i1_0 < JAST_LDS_R1B) ? (int)(n1 - i1_0) : JAST_LDS_R1B;
         __syncthreads(); // the previous rows are no longer read
+        flush(prev_row0, prev_nr);
         for (int64_t t = threadIdx.x; t < nr * ts; t += JAST_THREADS)
             s_tab[t] = tab[i1_0 * ts + t];
         __syncthreads();
@@ -308,6 +333,7 @@ __global__ __launch_bounds__(JAST_THREADS) void jast_pair_lds_kernel(
                     }
                 }
             }
+            double sq[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int h = 0; h < JAST_PPT; h++) {
                 if (!act[h])
@@ -315,20 +341,38 @@ __global__ __launch_bounds__(JAST_THREADS) void jast_pair_lds_kernel(
                 double vx = ux[h] + s[h] * gx[h], vy = uy[h] + s[h] * gy[h], vz = uz[h] + s[h] * gz[h];
                 if (!live[h])
                     vx = vy = vz = 0.0;
+                const double v3 = -0.5 * (vx * vx + vy * vy + vz * vz);
                 double *o = G + (i1_0 + r) * ldg + i2v[h];
                 o[0] = vx;
                 o[plane] = vy;
                 o[2 * plane] = vz;
-                o[3 * plane] = -0.5 * (vx * vx + vy * vy + vz * vz);
+                o[3 * plane] = v3;
+                sq[0] = fma(vx, vx, sq[0]);
+                sq[1] = fma(vy, vy, sq[1]);
+                sq[2] = fma(vz, vz, sq[2]);
+                sq[3] = fma(v3, v3, sq[3]);
+            }
+            if (part) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const double v = __ockl_wfred_add_f64(sq[m]);
+                    if (lane == 0)
+                        s_part[r][m][wave] = v;
+                }
             }
         }
+        prev_row0 = i1_0;
+        prev_nr = nr;
     }
+    __syncthreads();
+    flush(prev_row0, prev_nr);
 }
 
 template <int NMAX, bool ONE>
 static int launch_pairs_lds(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab,
                             int64_t ts, const int *cls, int n_cls, int kmax, double *G, int64_t ii0, int64_t n1,
-                            int64_t jj0, int64_t n2, int64_t n2_fill, int64_t ldg, int64_t plane, hipStream_t st)
+                            int64_t jj0, int64_t n2, int64_t n2_fill, int64_t ldg, int64_t plane, double *part,
+                            hipStream_t st)
 {
     int64_t gx = (n2_fill + JAST_PPT * JAST_THREADS - 1) / (JAST_PPT * JAST_THREADS);
     int64_t gy = (n1 + JAST_ROWS_PER_BLOCK - 1) / JAST_ROWS_PER_BLOCK;
@@ -339,7 +383,7 @@ static int launch_pairs_lds(const double *r1, const double *r2, const double *rn
     const size_t lds = sizeof(double) * JAST_LDS_R1B * (size_t)ts;
     hipLaunchKernelGGL((jast_pair_lds_kernel<NMAX, ONE>), dim3((unsigned)gx, (unsigned)gy),
                        dim3(JAST_THREADS), lds, st, r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, G, ii0, n1, jj0, n2,
-                       n2_fill, ldg, plane);
+                       n2_fill, ldg, plane, part);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
@@ -361,22 +405,31 @@ static int launch_pairs(const double *r1, const double *r2, const double *rn, in
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-extern "C" int cutc_launch_jast_pairs(const double *r1, const double *r2, const double *rn, int n_nuc,
-                                      const double *tab, int64_t ts, const int *cls, int n_cls, int kmax, int64_t ii0,
-                                      int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, double *G, int64_t ldg,
-                                      int64_t plane, double *f2tab, hipStream_t st)
+// x blocks (512 points each) of the staged kernel over n2_fill points: the
+// number of partial row sums per G row it writes
+extern "C" int64_t cutc_jast_norm_blocks(int64_t n2_fill)
+{
+    return (n2_fill + JAST_PPT * JAST_THREADS - 1) / (JAST_PPT * JAST_THREADS);
+}
+
+static int jast_staged(int n_nuc, int64_t ts) { return n_nuc <= 32 && sizeof(double) * JAST_LDS_R1B * ts <= 48 * 1024; }
+
+static int jast_pairs(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab, int64_t ts,
+                      const int *cls, int n_cls, int kmax, int64_t ii0, int64_t n1, int64_t jj0, int64_t n2,
+                      int64_t n2_fill, double *G, int64_t ldg, int64_t plane, double *f2tab, double *part,
+                      hipStream_t st)
 {
     if (n1 <= 0 || n2_fill <= 0)
         return 0;
     /* up to 32 nuclei and a table block that fits 48 KB of LDS: the staged
      * kernel for the nucleus count rounded up to 4 */
-    if (n_nuc <= 32 && sizeof(double) * JAST_LDS_R1B * ts <= 48 * 1024) {
+    if (jast_staged(n_nuc, ts)) {
         const int one = n_cls == 1;
 #define LL(NM)                                                                                                       \
     return one ? launch_pairs_lds<NM, true>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, G, ii0, n1, jj0, n2,       \
-                                            n2_fill, ldg, plane, st)                                                  \
+                                            n2_fill, ldg, plane, part, st)                                            \
                : launch_pairs_lds<NM, false>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, G, ii0, n1, jj0, n2,      \
-                                             n2_fill, ldg, plane, st)
+                                             n2_fill, ldg, plane, part, st)
         switch ((n_nuc + 3) / 4) {
         case 0:
         case 1: LL(4);
@@ -409,4 +462,24 @@ extern "C" int cutc_launch_jast_pairs(const double *r1, const double *r2, const 
     }
     return launch_pairs<0, 32>(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, f2tab, n2_fill, G, ii0, n1, jj0, n2,
                                n2_fill, ldg, plane, st);
+}
+
+extern "C" int cutc_launch_jast_pairs(const double *r1, const double *r2, const double *rn, int n_nuc,
+                                      const double *tab, int64_t ts, const int *cls, int n_cls, int kmax, int64_t ii0,
+                                      int64_t n1, int64_t jj0, int64_t n2, int64_t n2_fill, double *G, int64_t ldg,
+                                      int64_t plane, double *f2tab, hipStream_t st)
+{
+    return jast_pairs(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, ii0, n1, jj0, n2, n2_fill, G, ldg, plane, f2tab,
+                      nullptr, st);
+}
+
+extern "C" int cutc_launch_jast_pairs_norms(const double *r1, const double *r2, const double *rn, int n_nuc,
+                                            const double *tab, int64_t ts, const int *cls, int n_cls, int kmax,
+                                            int64_t ii0, int64_t n1, int64_t n2, int64_t n2_fill, double *G,
+                                            int64_t ldg, int64_t plane, double *part, hipStream_t st)
+{
+    if (!jast_staged(n_nuc, ts) || !part)
+        return CUTC_ERR_UNSUPPORTED;
+    return jast_pairs(r1, r2, rn, n_nuc, tab, ts, cls, n_cls, kmax, ii0, n1, 0, n2, n2_fill, G, ldg, plane, nullptr,
+                      part, st);
 }

@@ -50,6 +50,18 @@ int cutc_launch_jast_pairs(const double *r1, const double *r2, const double *rn,
                            int64_t ts, const int *cls, int n_cls, int kmax, int64_t ii0, int64_t n1, int64_t jj0,
                            int64_t n2, int64_t n2_fill, double *G, int64_t ldg, int64_t plane, double *f2tab,
                            hipStream_t st);
+/* partial sums per G row written by cutc_launch_jast_pairs_norms: one per
+ * block of 512 of the n2_fill points */
+int64_t cutc_jast_norm_blocks(int64_t n2_fill);
+/* cutc_launch_jast_pairs (r2 points from 0) that also writes, for every G
+ * row m n1 + i1, its sums of squares over each block of 512 points into
+ * part[b 4 n1 + m n1 + i1] (b < cutc_jast_norm_blocks(n2_fill)), for
+ * cutc_rowexp_finish; CUTC_ERR_UNSUPPORTED (nothing launched) for the
+ * shapes of the unstaged kernel (more than 32 nuclei, large programs) */
+int cutc_launch_jast_pairs_norms(const double *r1, const double *r2, const double *rn, int n_nuc, const double *tab,
+                                 int64_t ts, const int *cls, int n_cls, int kmax, int64_t ii0, int64_t n1, int64_t n2,
+                                 int64_t n2_fill, double *G, int64_t ldg, int64_t plane, double *part,
+                                 hipStream_t st);
 
 /* Integrands (integrands.hip).  pairs: (k, l) per packed index, k <= l. */
 int cutc_launch_pair_table(int n_ao, int *pairs, hipStream_t st);
@@ -173,6 +185,9 @@ typedef struct cutc_colsrc {
 #define CUTC_ROWEXP_NORM 0
 #define CUTC_ROWEXP_MAX 1
 int cutc_rowexp_k(const double *X, int64_t ld, int64_t rows, int64_t K, int mode, int *rexp, hipStream_t st);
+/* row exponents from ks partial sums per row, partial[y rows + r] (sums of
+ * squares, CUTC_ROWEXP_NORM, or maxima), joined in order */
+int cutc_rowexp_finish(const double *partial, int64_t ks, int64_t rows, int mode, int *rexp, hipStream_t st);
 int cutc_rowexp_pairw(const double *wr2, const double *aos2, int64_t n2, int n_ao, int mode, int *rexp,
                       hipStream_t st);
 /* partial: CUTC_CRT_KSPLIT x rows doubles of scratch */

@@ -79,7 +79,7 @@ int cutc_make_plan(int64_t n1, int64_t n2, int n_ao, int n_nuc, int precision, i
             fixed1 = 16 * P + rb * nmod * CUTC_PK_ROWS_B(P) * kc + rb * nmod * 511 * kc + 4096;
             /* the block-sparse contraction's orderings, and the flags and lists of one pass */
             fixed1 += sparse_bytes(n2, n_ao, P, ldk, kc, precision == CUTC_PREC_INT8 ? CUTC_CRT_I8 : CUTC_CRT_BF16);
-            per_r1 = 32 * ldk + 32 + 4 * rb * nmod * kc + nmod * 4 * P;
+            per_r1 = 32 * ldk + 32 + 4 * rb * nmod * kc + nmod * 4 * P + 32 * cutc_jast_norm_blocks(ldk);
             if (n_nuc > 32)
                 fixed1 += 8 * (int64_t)n_nuc * ldk;
             if ((int64_t)budget >= fixed1 + per_r1 * want || kc == CUTC_PK_KALIGN)
