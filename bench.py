@@ -90,8 +90,8 @@ def parse():
                          "gradients on both grids of the cube20 workload (cutc_aos_device), 3 primitives per AO")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="batch64: grid sizes times this factor")
     ap.add_argument("--batch-streams", type=int, default=0,
-                    help="batch64: concurrent graph branches (0: the library default, 1); FP64 is also timed on one "
-                         "branch")
+                    help="batch64: concurrent graph branches (0: the library default, 4 for the modular forms and 1 for "
+                         "the others); FP64 is also timed on one branch")
     ap.add_argument("--variants", default="",
                     help="further split builds compared with the same FP64 build, e.g. bf16:3,bf16:4,fp16:4")
     ap.add_argument("--variant-warmup", type=int, default=1, help="untimed builds before each variant's timed one")

@@ -1723,12 +1723,14 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
                 need = b;
         }
         need = ws_round(need + need / 64 + ((size_t)16 << 20));
-        /* branches: opt->streams (default 1: on the 64-system benchmark batch
-         * two branches were slower, 40.2 against 39.7 s in FP64 and 16.8
-         * against 14.3 s in int8, as every system's contractions already fill
-         * the device and two compete for L2 and power), as many as the device
+        /* branches: opt->streams, by default 1 for FP64 and the slice forms
+         * (on the 64-system benchmark batch two branches were slower, 40.2
+         * against 39.7 s in FP64: every system's dense contractions fill the
+         * device and two compete for L2 and power) and 4 for the modular
+         * forms, whose list-driven contractions leave room (int8: 8.11 s on
+         * one branch, 7.82 on two, 7.53 on three); as many as the device
          * holds a workspace region and a chunk buffer for */
-        nb = opt->streams > 0 ? opt->streams : 1;
+        nb = opt->streams > 0 ? opt->streams : (is_crt(opt->precision) ? CUTC_BATCH_MAX_BRANCHES : 1);
         if (nb > nsys)
             nb = nsys;
         while (nb > 1 && (size_t)nb * (need + ws_round(xbytes)) > budget)

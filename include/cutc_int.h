@@ -125,7 +125,8 @@ typedef struct cutc_options {
                          copy stream, host copy threads); 1 = blocking copies
                          after the build, as the reference                      */
     int streams;      /* cutc_int_batch_device: concurrent branches the systems
-                         are recorded on (1..4, 0 = 1; fewer when the device
+                         are recorded on (1..4; 0 = 4 for the modular
+                         precisions, 1 for the others; fewer when the device
                          cannot hold a workspace and chunk buffer per branch) */
     int dense;        /* modular precisions: 1 = the stage-1 contraction runs
                          every K step; 0 = it skips the (256-pair, K-step)
