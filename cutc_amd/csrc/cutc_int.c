@@ -622,9 +622,42 @@ static int kcount_sum(const int *kcount, int64_t n, hipStream_t st, double *sum)
  * the pair rows sorted by the r2 segments where their quantised values are
  * non-zero, and every GEMM runs, per 256-pair tile, only the K steps where
  * that tile has a non-zero residue (opt->dense: all of them). */
+/* The r1-independent part of the block-sparse stage 1 (the permuted copies
+ * of r2, wr2 and the AO values, the clustered pair table and its exponents),
+ * kept across the r1 chunks of one streamed call: the first chunk computes
+ * it into these buffers, the others reuse it. */
+typedef struct s1pre {
+    double *r2p, *wr2p, *aos2p;
+    int *pairs, *rexpA;
+    int valid;
+} s1pre_t;
+
+static size_t s1pre_bytes(int64_t n2, int n_ao)
+{
+    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    return ws_round(sizeof(double) * 3 * n2) + ws_round(sizeof(double) * n2) +
+           ws_round(sizeof(double) * n_ao * n2) + ws_round(sizeof(int) * 2 * P) + ws_round(sizeof(int) * P);
+}
+
+/* carve the buffers of s1pre_bytes(n2, n_ao) bytes at base */
+static void s1pre_carve(s1pre_t *pre, char *base, int64_t n2, int n_ao)
+{
+    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    pre->r2p = (double *)base;
+    base += ws_round(sizeof(double) * 3 * n2);
+    pre->wr2p = (double *)base;
+    base += ws_round(sizeof(double) * n2);
+    pre->aos2p = (double *)base;
+    base += ws_round(sizeof(double) * n_ao * n2);
+    pre->pairs = (int *)base;
+    base += ws_round(sizeof(int) * 2 * P);
+    pre->rexpA = (int *)base;
+    pre->valid = 0;
+}
+
 static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                         const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
-                        const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats)
+                        const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats, s1pre_t *pre)
 {
     const cutc_jprog *prog = &jp->h;
     const cutc_jmono *dmono = jp->dmono;
@@ -713,17 +746,28 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     }
 
     ev = tr_begin(tr, CAT_SPLIT);
+    if (sparse && pre) {
+        /* the orderings of an earlier chunk of the call, or computed here for the later ones */
+        r2p = pre->r2p;
+        wr2p = pre->wr2p;
+        aos2p = pre->aos2p;
+        pairs = pre->pairs;
+        rexpA = pre->rexpA;
+    }
     if (sparse) {
         /* r2 order, then everything of stage 1 reads the permuted copies */
-        rc = cutc_order_r2(r2, wr2, aos2, n2, n_ao, box, skeys, svals, nsort, r2p, wr2p, aos2p, st);
+        if (!(pre && pre->valid))
+            rc = cutc_order_r2(r2, wr2, aos2, n2, n_ao, box, skeys, svals, nsort, r2p, wr2p, aos2p, st);
         r2 = r2p;
         wr2 = wr2p;
         aos2 = aos2p;
-        if (rc == CUTC_OK)
+        if (rc == CUTC_OK && !(pre && pre->valid))
             rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpN, st);
-        if (rc == CUTC_OK)
+        if (rc == CUTC_OK && !(pre && pre->valid))
             rc = cutc_order_pairs(wr2, aos2, n2, n_ao, crt.bits, rexpN, CUTC_CRT_KSTEP(kind), KT, cws, pairs, rexpA,
                                   st);
+        if (rc == CUTC_OK && pre)
+            pre->valid = 1;
         if (rc == CUTC_OK && kpass == 1)
             rc = cutc_zero_u8(occ, tiles * KT, st);
         if (rc == CUTC_OK && kpass == 1)
@@ -1353,7 +1397,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             rc = cutc_launch_fill(X, 4 * (int64_t)n1 * n_ao * n_ao, 0.0, st);
             tr_end(&tr, ev);
         } else if (opt->precision != CUTC_PREC_FP64) {
-            rc = stage1_split(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
+            rc = stage1_split(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats, NULL);
         } else {
             rc = stage1(opt, &pl, n1, n2, n_ao, n_nuc, r1, r2, wr2, rn, aos2, &prog, X, &tr, stats);
         }
@@ -1458,7 +1502,7 @@ int cutc_int_device_noalloc(const cutc_options *opt, int n_grid1, int n_grid2, i
 static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                            const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
                            const double *aos1, const double *aos2, const jprog_t *jp, double *out, double *Xc,
-                           int64_t nc, int nsplit2, tracer_t *tr, cutc_stats *stats)
+                           int64_t nc, int nsplit2, tracer_t *tr, cutc_stats *stats, s1pre_t *pre)
 {
     const int64_t nao2 = (int64_t)n_ao * n_ao;
     int rc = CUTC_OK;
@@ -1470,7 +1514,7 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
             rc = cutc_launch_fill(Xc, 4 * cnt * nao2, 0.0, tr->st);
             tr_end(tr, ev);
         } else if (opt->precision != CUTC_PREC_FP64) {
-            rc = stage1_split(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats);
+            rc = stage1_split(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats, pre);
         } else {
             rc = stage1(opt, pl, cnt, n2, n_ao, n_nuc, r1 + 3 * r0, r2, wr2, rn, aos2, jp, Xc, tr, stats);
         }
@@ -1527,10 +1571,16 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
         return rc;
     }
     double *Xc = NULL;
+    char *preb = NULL;
+    s1pre_t pre;
+    memset(&pre, 0, sizeof pre);
     tracer_t tr;
     int have_tr = 0;
-    /* chunk of the stored X */
-    const size_t budget = device_budget(opt);
+    /* chunk of the stored X; the modular precisions also keep the
+     * r1-independent orderings of stage 1 across the chunks */
+    const int keep = is_crt(opt->precision) && n2 > 0;
+    const size_t preby = keep ? s1pre_bytes(n2, n_ao) : 0;
+    const size_t budget0 = device_budget(opt), budget = budget0 > preby ? budget0 - preby : 0;
     cutc_plan p1;
     rc = cutc_make_plan(1, n2, n_ao, n_nuc, opt->precision, opt->slices, budget, 0, &p1);
     if (rc)
@@ -1552,13 +1602,18 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
             nc = n1;
         }
         TRYH(hipMalloc((void **)&Xc, sizeof(double) * 4 * (size_t)nc * nao2));
+        if (keep && nc < n1) {
+            TRYH(hipMalloc((void **)&preb, preby));
+            s1pre_carve(&pre, preb, n2, n_ao);
+        }
         cutc_plan pl;
         TRY(cutc_make_plan(nc, n2, n_ao, n_nuc, opt->precision, opt->slices, device_budget(opt), opt->n1_chunk, &pl));
         tr_init(&tr, st, stats != NULL || opt->verbose);
         have_tr = 1;
         int evt = tr_begin(&tr, CAT_TOTAL);
         rc = streamed_chunks(opt, &pl, n1, n2, n_ao, n_nuc, r1, wr1, r2, wr2, rn, aos1, aos2, &prog, out, Xc, nc,
-                             split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
+                             split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats,
+                             preb ? &pre : NULL);
         tr_end(&tr, evt);
         if (rc == CUTC_OK)
             rc = hip_rc(hipStreamSynchronize(st));
@@ -1577,6 +1632,7 @@ done:
     if (have_tr)
         tr_free(&tr);
     dfree(Xc);
+    dfree(preb);
     jprog_free(&prog);
     return rc;
 }
@@ -1785,7 +1841,7 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             else
                 rc = streamed_chunks(opt, &pl[i], y->n_grid1, y->n_grid2, y->n_ao, y->n_nuc, y->r1, y->wr1, y->r2,
                                      y->wr2, y->rn, y->aos_data1, y->aos_data2, &jp[i], y->int_2e_ao, Xc[b], nc[i],
-                                     split2[i], &tr[b], NULL);
+                                     split2[i], &tr[b], NULL, NULL);
             g_ws_fixed.base = NULL;
             g_ws_fixed.size = 0;
         }
