@@ -212,6 +212,9 @@ def batch_bench(args, torch, cutc, synth):
     for prec in BATCH_PRECISIONS:
         cutc.cutc_int_streamed_device(dsm, Ism, cutc.options(prec), stream)
         cutc.cutc_int_batch_device([dsm], [Ism], cutc.options(prec), stream)
+    # the system with the largest workspace once, so that the timed calls find the library's workspace grown
+    big = max(range(len(sys_)), key=lambda i: sys_[i].n_ao ** 2 * sys_[i].n2)
+    cutc.cutc_int_streamed_device(dss[big], Iseq[big], cutc.options("fp64"), stream)
     # sequential FP64 calls
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -232,6 +235,16 @@ def batch_bench(args, torch, cutc, synth):
     runs = [("fp64", 1)] + [(p, args.batch_streams) for p in BATCH_PRECISIONS]
     for prec, nstreams in runs:
         opt = cutc.options(prec, streams=nstreams)
+        first = None
+        if prec in ("int8", "bf16crt"):
+            # the modular batches spread over several graph branches, whose
+            # workspace the first call allocates: time the call after it
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st0 = cutc.cutc_int_batch_device(dss, Ib, opt, stream)
+            torch.cuda.synchronize()
+            first = {"call_wall_s": time.perf_counter() - t0, "setup_ms": st0["ms_setup"],
+                     "graph_launch_s": st0["ms_launch"] / 1e3}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         st = cutc.cutc_int_batch_device(dss, Ib, opt, stream)
@@ -247,6 +260,8 @@ def batch_bench(args, torch, cutc, synth):
                "branches": st["n_branches"],
                "eff_tflops": W / (st["ms_launch"] / 1e3) / 1e12, "speedup_vs_sequential_fp64": seq_s / wall,
                "max_rel_frob_vs_sequential_fp64": max(errs)}
+        if first is not None:
+            res["first_call"] = first
         log(args, f"batch {prec}: {res}")
         label = f"{prec}x{opt.slices}" if prec != "fp64" else prec
         out["batch"][label + ("_1branch" if nstreams == 1 and prec == "fp64" else "")] = res
