@@ -397,9 +397,13 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
 // (k, l) from the table ptab (2 ints per row) or, without one, the packed
 // pair index.  occ (optional): byte [tb][kt] set to 1 when a row of the
 // CUTC_SP_ROWS-row B tile tb has a non-zero quantised value in K step kt (64 K for
-// int8, 32 for the 16-bit kinds); the caller clears it first.  A wave's 64
-// threads cover 32 rows x 32 K inside one step of either size, so one
-// ballot per wave decides the flag.
+// int8, 32 for the 16-bit kinds); the caller clears it first.
+// grid (kpad / 128, ceil(P / 32)), as quant_k_pk_kernel: the 32 x 128 tile of
+// products is formed in LDS from coalesced reads (8 consecutive lanes read
+// 128 contiguous bytes of one AO row), then thread r = tid % 32 quantises
+// row 32 rb + r at the 16 points of j = tid / 32.  A wave's 64 threads then
+// cover 32 rows x 32 K inside one step of either size, so one ballot per
+// wave decides the flag.
 template <int KIND>
 __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const double *__restrict__ wr2,
                                                              const double *__restrict__ aos2, int64_t ld2,
@@ -408,31 +412,48 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
                                                              const int *__restrict__ ptab, int8_t *planes,
                                                              uint8_t *occ)
 {
-    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
-    const int64_t rb = blockIdx.y, p = rb * 32 + r;
-    const int64_t k0 = (int64_t)blockIdx.x * 128 + 16 * j;
+    __shared__ double tile[32][128 + 1]; // odd row pitch: the column reads below are conflict-free
+    const int64_t rb = blockIdx.y, kb = (int64_t)blockIdx.x * 128;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    {
+        const int rr = threadIdx.x >> 3, seg = threadIdx.x & 7; // 32 rows x 8 lanes
+        const int64_t p = rb * 32 + rr;
+        int k = 0, l = 0;
+        if (p < P) {
+            if (ptab) {
+                k = ptab[2 * p];
+                l = ptab[2 * p + 1];
+            } else {
+                pair_of(p, k, l);
+            }
+        }
+        const double *al = aos2 + ld2 * l, *ak = aos2 + ld2 * k;
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int col = 16 * it + 2 * seg;
+            const int64_t i = kb + col;
+            double v0 = 0.0, v1 = 0.0;
+            if (p < P) {
+                if (i < n2)
+                    v0 = wr2[i] * al[i] * ak[i];
+                if (i + 1 < n2)
+                    v1 = wr2[i + 1] * al[i + 1] * ak[i + 1];
+            }
+            tile[rr][col] = v0;
+            tile[rr][col + 1] = v1;
+        }
+    }
+    __syncthreads();
+    const int r = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const int64_t p = rb * 32 + r;
+    const int64_t k0 = kb + 16 * j;
     double q[16];
     bool nz = false;
-    if (p < P) {
-        int k, l;
-        if (ptab) {
-            k = ptab[2 * p];
-            l = ptab[2 * p + 1];
-        } else {
-            pair_of(p, k, l);
-        }
-        const int sh = c.bits - rexp[p];
+    const int sh = (p < P) ? c.bits - rexp[p] : 0;
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int64_t i = k0 + e;
-            q[e] = (i < n2) ? rint(ldexp(wr2[i] * aos2[i + ld2 * l] * aos2[i + ld2 * k], sh)) : 0.0;
-            nz |= q[e] != 0.0;
-        }
-    } else {
-#pragma unroll
-        for (int e = 0; e < 16; e++)
-            q[e] = 0.0;
+    for (int e = 0; e < 16; e++) {
+        q[e] = rint(ldexp(tile[r][16 * j + e], sh));
+        nz |= q[e] != 0.0;
     }
     store_residues<KIND>(c, q, planes, rb, r, k0, KT, plane);
     if (occ) {
