@@ -124,3 +124,21 @@ def test_cube40_streamed_split(torch, cube40, precision):
     del I, M
     assert e_full <= TOL_SPLIT and errs["dd"] <= TOL_SPLIT
     assert max(scaled.values()) <= TOL_SPLIT, scaled
+
+
+def test_cube40_streamed_int8_two_chunks(torch, cube40):
+    """Two r1 chunks of 8 points: the second reuses the first's r2 order and
+    pair clustering (kept across the chunks of a streamed call); whole
+    tensor within the metric's bar of the FP64 build."""
+    s, ds, ref = cube40
+    I = torch.full_like(ref["fp64"], float("nan"))
+    st = cutc.cutc_int_streamed_device(ds, I, cutc.options("int8", n1_chunk=8))
+    torch.cuda.synchronize()
+    cutc.release_workspace()
+    assert st["n1_chunks"] == 2
+    assert bool(torch.isfinite(I).all())
+    e_full = rel_dev(torch, I, ref["fp64"])
+    print(f"  cube40 streamed int8 in 2 chunks: whole tensor vs FP64 {e_full:.2e} ({st['ms_total'] / 1e3:.2f} s)",
+          flush=True)
+    del I
+    assert e_full <= TOL_SPLIT
