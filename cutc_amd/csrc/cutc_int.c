@@ -1226,10 +1226,12 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 }
                 ev = tr_begin(tr, CAT_SPLIT);
                 const double alpha = part == 0 ? 1.0 : -0.5;
-                const int mode = part == 0 ? (first ? 0 : 1) : 2;
-                rc = slice ? cutc_scatter_m((const double *)Cpl, 1, P, alpha, pairs, n_ao, out, mode, st)
+                /* slices: the full layout (store, add, add antisymmetric); modular:
+                 * the halves of M (store or add), completed by cutc_crt_finalize_m */
+                rc = slice ? cutc_scatter_m((const double *)Cpl, 1, P, alpha, pairs, n_ao, out,
+                                            part == 0 ? (first ? 0 : 1) : 2, st)
                            : cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, alpha, pairs, sparse ? pairsH : NULL,
-                                                n_ao, out, mode, st);
+                                                n_ao, out, part, first ? 0 : 1, st);
                 tr_end(tr, ev);
                 TRY(rc);
             }
@@ -1239,7 +1241,10 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     }
     if (symm) {
         ev = tr_begin(tr, CAT_SYMM);
-        rc = cutc_launch_trans_inplace(out, nao2, st);
+        if (split && !slice) /* the modular halves of M into the full layout */
+            rc = cutc_crt_finalize_m(out, n_ao, st);
+        if (rc == CUTC_OK)
+            rc = cutc_launch_trans_inplace(out, nao2, st);
         tr_end(tr, ev);
         TRY(rc);
     }

@@ -1173,13 +1173,23 @@ extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P,
 
 // thread per (p, q), p fastest; modes as the FP64 step-2 epilogue; the
 // rows q of the A side have their own pair table pairsB
+// The stage-2 results of the modular contraction go into M in halves, so
+// that every store is coalesced: thread p (X-side pairs in packed order,
+// (k, l) = pairs[p], k <= l, so that consecutive p are consecutive rows
+// k + n_ao l) and row q of C (H-side pair (a, b) = pairsB[q], a <= b).
+// part 0 (X^3 H, symmetric in (a, b)): the S slot, column a + n_ao b;
+// part 1 (X^{0..2} NH, antisymmetric in (a, b)): the A slot, column
+// b + n_ao a for a < b (for a = b it is zero: NH has a zero diagonal).  Only
+// the rows k <= l are written; mode 0 stores, mode 1 adds.
+// finalize_m_kernel then forms M[:, ab] = S + A, M[:, ba] = S - A and the
+// rows k > l from the rows l < k, which is the full layout the reference's
+// GEMMs write (before the symmetrisation M + M^T).
 __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t *__restrict__ C,
-                                                        int64_t P,
-                                                        const int *__restrict__ rexpA,
+                                                        int64_t P, const int *__restrict__ rexpA,
                                                         const int *__restrict__ rexpB, double alpha,
                                                         const int *__restrict__ pairs,
-                                                        const int *__restrict__ pairsB, int n_ao, double *M, int mode,
-                                                        int npb)
+                                                        const int *__restrict__ pairsB, int n_ao, double *M, int part,
+                                                        int mode, int npb)
 {
     int64_t q0, p;
     rebuild_rowcol(npb, q0, p);
@@ -1187,43 +1197,94 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
         return;
     const int k = pairs[2 * p], l = pairs[2 * p + 1];
     const int64_t n2 = (int64_t)n_ao * n_ao;
-    const int64_t kl = k + (int64_t)n_ao * l, lk = l + (int64_t)n_ao * k;
+    const int64_t kl = k + (int64_t)n_ao * l;
     const int ea = rexpA[p] - 2 * c.bits;
     for (int64_t q = q0; q < P; q += gridDim.x / npb) {
-    const double V = crt_value(c, C + q * P + p, P * P);
-    const double v = alpha * ldexp(V, ea + rexpB[q]);
-    const int a = pairsB[2 * q], b = pairsB[2 * q + 1];
-    double *o1 = M + n2 * (a + (int64_t)n_ao * b);
-    double *o2 = M + n2 * (b + (int64_t)n_ao * a);
-    if (mode == 0) {
-        o1[kl] = v;
-        o1[lk] = v;
-        o2[kl] = v;
-        o2[lk] = v;
-    } else {
-        const double s = (mode == 2) ? -v : v;
-        o1[kl] += v;
-        if (k != l)
-            o1[lk] += v;
-        if (a != b) {
-            o2[kl] += s;
-            if (k != l)
-                o2[lk] += s;
+        const int a = pairsB[2 * q], b = pairsB[2 * q + 1];
+        if (part == 1 && a == b)
+            continue;
+        const double v = alpha * ldexp(crt_value(c, C + q * P + p, P * P), ea + rexpB[q]);
+        double *o = M + n2 * (part == 0 ? a + (int64_t)n_ao * b : b + (int64_t)n_ao * a) + kl;
+        if (mode == 0)
+            *o = v;
+        else
+            *o += v;
+    }
+}
+
+// one block per column pair (a, b), a <= b (blockIdx.x = the packed pair
+// index): the n_ao x n_ao matrices Mc[k][l] = M[k + n_ao l][col] of the
+// columns ab and ba in 32 x 32 tiles (i, j), i <= j; the rows k <= l of S
+// (column ab) and A (column ba, a < b) give
+//   M_ab[k][l] = M_ab[l][k] = S[k][l] + A[k][l],
+//   M_ba[k][l] = M_ba[l][k] = S[k][l] - A[k][l],
+// the lower tiles written through LDS so that both stores are coalesced.
+__global__ __launch_bounds__(256) void finalize_m_kernel(double *M, int n_ao)
+{
+    __shared__ double ts[32][33], ta[32][33];
+    int a, b;
+    pair_of(blockIdx.x, a, b);
+    const int64_t n2 = (int64_t)n_ao * n_ao;
+    double *Mab = M + n2 * (a + (int64_t)n_ao * b), *Mba = M + n2 * (b + (int64_t)n_ao * a);
+    const bool anti = a < b;
+    const int nt = (n_ao + 31) / 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+    for (int ti = 0; ti < nt; ti++)
+        for (int tj = ti; tj < nt; tj++) {
+            // upper tile rows k in tile ti, columns l in tile tj (k <= l wherever ti < tj)
+            __syncthreads();
+            for (int yy = ty; yy < 32; yy += 8) {
+                const int k = 32 * ti + tx, l = 32 * tj + yy;
+                double sv = 0.0, av = 0.0;
+                if (k < n_ao && l < n_ao && k <= l) {
+                    const int64_t o = k + (int64_t)n_ao * l;
+                    sv = Mab[o];
+                    av = anti ? Mba[o] : 0.0;
+                    Mab[o] = sv + av;
+                    if (anti)
+                        Mba[o] = sv - av;
+                }
+                ts[yy][tx] = sv;
+                ta[yy][tx] = av;
+            }
+            __syncthreads();
+            // lower tile: rows l (tile tj), columns k (tile ti), k < l
+            for (int yy = ty; yy < 32; yy += 8) {
+                const int l = 32 * tj + tx, k = 32 * ti + yy;
+                if (k < n_ao && l < n_ao && k < l) {
+                    const int64_t o = l + (int64_t)n_ao * k;
+                    const double sv = ts[tx][yy], av = ta[tx][yy];
+                    Mab[o] = sv + av;
+                    if (anti)
+                        Mba[o] = sv - av;
+                }
+            }
         }
-    }
-    }
 }
 
 extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
                                   const int *rexpB, double alpha, const int *pairs, const int *pairsB, int n_ao,
-                                  double *M, int mode, hipStream_t st)
+                                  double *M, int part, int mode, hipStream_t st)
 {
     if (P <= 0)
         return 0;
+    if ((part != 0 && part != 1) || (mode != 0 && mode != 1))
+        return CUTC_ERR_ARG;
     const int64_t npb = (P + 255) / 256, rows = P < 1048576 / npb + 1 ? P : 1048576 / npb + 1;
     if (npb > 1048576)
         return CUTC_ERR_ARG;
     hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, rexpA, rexpB,
-                       alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, mode, (int)npb);
+                       alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, part, mode, (int)npb);
+    return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+}
+
+extern "C" int cutc_crt_finalize_m(double *M, int n_ao, hipStream_t st)
+{
+    const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
+    if (P <= 0)
+        return 0;
+    if (P > 2147483647LL)
+        return CUTC_ERR_ARG;
+    hipLaunchKernelGGL(finalize_m_kernel, dim3((unsigned)P), dim3(256), 0, st, M, n_ao);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }

@@ -317,12 +317,18 @@ int cutc_scatter_m(const double *C, int nsplit, int64_t P, double alpha, const i
 int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N, const int *rexpA,
                        const int *rexpB, const int *pairs, int n_ao, double *X, int64_t n1, int64_t n1c,
                        int64_t i0, hipStream_t st);
-/* Rebuild alpha C into the n_ao^2 x n_ao^2 matrix M; mode as cutc_gemm2_f64. */
-/* pairsB (optional): the pair table of the rows of the GEMM's A side (rexpB),
- * when it differs from pairs (the B side's, rexpA) */
+/* Rebuild alpha C into halves of the n_ao^2 x n_ao^2 matrix M: part 0 (the
+ * symmetric X^3 H) into the columns a + n_ao b, part 1 (the antisymmetric
+ * X^{0..2} NH) into the columns b + n_ao a, a < b, rows k + n_ao l, k <= l
+ * only; mode 0 stores, 1 adds.  pairs: the packed-order table of the B side
+ * (rexpA); pairsB (optional): the table of the rows of the GEMM's A side
+ * (rexpB), when it differs.  cutc_crt_finalize_m then forms the full M. */
 int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
                        const int *rexpB, double alpha, const int *pairs, const int *pairsB, int n_ao, double *M,
-                       int mode, hipStream_t st);
+                       int part, int mode, hipStream_t st);
+/* M[:, ab] = S + A, M[:, ba] = S - A (a < b) and the rows k > l from k < l,
+ * from the halves cutc_crt_rebuild_m writes */
+int cutc_crt_finalize_m(double *M, int n_ao, hipStream_t st);
 
 /* Dense TN GEMM (gemm_f64.hip): C = alpha A^T B + beta C, K-contiguous rows
  * (lda, ldb >= K).  K a multiple of 16 with even leading dimensions and
