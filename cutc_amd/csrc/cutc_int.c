@@ -611,17 +611,6 @@ static int kcount_sum(const int *kcount, int64_t n, hipStream_t st, double *sum)
     return rc;
 }
 
-/* Stage 1 with a split-precision contraction.  INT8 / BF16_CRT / FP16_CRT:
- * the modular (CRT) scheme, pair residues quantised on the fly from w2 and
- * chi, each G chunk quantised after the Jastrow block, one int8 (or bf16 /
- * fp16) GEMM per modulus, CRT rebuild scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
- * pairs and of each G chunk, one slice GEMM into FP64, scattered into X.
- * Modular: block-sparse.  The r2 points are reordered
- * (nearest nucleus, then distance; the Jastrow, pair weights and AO values
- * all use the permuted copies, so the contraction over r2 is unchanged),
- * the pair rows sorted by the r2 segments where their quantised values are
- * non-zero, and every GEMM runs, per 256-pair tile, only the K steps where
- * that tile has a non-zero residue (opt->dense: all of them). */
 /* The r1-independent part of the block-sparse stage 1 (the permuted copies
  * of r2, wr2 and the AO values, the clustered pair table and its exponents),
  * kept across the r1 chunks of one streamed call: the first chunk computes
@@ -655,6 +644,18 @@ static void s1pre_carve(s1pre_t *pre, char *base, int64_t n2, int n_ao)
     pre->valid = 0;
 }
 
+/* Stage 1 with a split-precision contraction.  INT8 / BF16_CRT / FP16_CRT:
+ * the modular (CRT) scheme, pair residues quantised on the fly from w2 and
+ * chi, each G chunk quantised after the Jastrow block, one int8 (or bf16 /
+ * fp16) GEMM per modulus, CRT rebuild scattered into X.  BF16 / FP16: the slice scheme, digit planes of the
+ * pairs and of each G chunk, one slice GEMM into FP64, scattered into X.
+ * Modular: block-sparse.  The r2 points are reordered along a Morton
+ * curve (the Jastrow, pair weights and AO values all use the permuted
+ * copies, so the contraction over r2 is unchanged), the pair rows clustered
+ * into 256-pair tiles by the K steps where their quantised values are
+ * non-zero, and every GEMM runs, per tile, only the K steps where that tile
+ * has a non-zero residue (opt->dense: all of them); pre (optional) keeps the
+ * orderings across the r1 chunks of a streamed call. */
 static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                         const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
                         const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats, s1pre_t *pre)
@@ -1007,12 +1008,13 @@ static int tune_split2(int64_t P, int n_ao, int64_t K, size_t budget, hipStream_
  * at the end.  nsplit2 > 1: the FP64 contractions run split-K (for small
  * n_pairs, whose P x P tile grid leaves most of the GPU idle).
  * Modular precisions with r1 given: block-sparse, as stage 1 with the roles
- * of the operands swapped.  Each r1 chunk's points run in (nearest nucleus,
- * distance) order (the integrands are formed at the permuted points and X is
- * read through the same permutation), the H-side pairs are sorted by where
- * their quantised H values are non-zero, and the contraction runs, per
- * 256-pair H tile (the GEMM's A side), only the K steps where that tile has a
- * non-zero residue (opt->dense: all of them). */
+ * of the operands swapped.  Each r1 chunk's points run in Morton order (the
+ * integrands are formed at the permuted points and X is read through the
+ * same permutation), the H-side pairs are clustered by where their quantised
+ * H values are non-zero, and the contraction runs, per 256-pair H tile (the
+ * GEMM's A side), only the K steps where that tile has a non-zero residue
+ * (opt->dense: all of them).  Its results go into halves of M (see
+ * cutc_crt_rebuild_m), completed with the symmetrisation. */
 static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t r0, int64_t cnt, int n_ao,
                   const double *r1, const double *rn, int n_nuc, const double *wr1, const double *aos1,
                   const double *X, double *out, int accumulate, int symm, int nsplit2, tracer_t *tr,
