@@ -68,7 +68,8 @@ static void tr_init(tracer_t *t, hipStream_t st, int timing)
     t->timing = timing;
 }
 
-static int tr_begin(tracer_t *t, int cat)
+/* events of category cat around work queued on stream s (tr_begin: on t->st) */
+static int tr_begin_on(tracer_t *t, int cat, hipStream_t s)
 {
     t->launches[cat]++;
     if (!t->timing)
@@ -97,16 +98,19 @@ static int tr_begin(tracer_t *t, int cat)
         return -1;
     }
     t->cat[i] = cat;
-    hipEventRecord(t->ev[2 * i], t->st);
+    hipEventRecord(t->ev[2 * i], s);
     t->nev++;
     return i;
 }
 
-static void tr_end(tracer_t *t, int i)
+static void tr_end_on(tracer_t *t, int i, hipStream_t s)
 {
     if (i >= 0)
-        hipEventRecord(t->ev[2 * i + 1], t->st);
+        hipEventRecord(t->ev[2 * i + 1], s);
 }
+
+static int tr_begin(tracer_t *t, int cat) { return tr_begin_on(t, cat, t->st); }
+static void tr_end(tracer_t *t, int i) { tr_end_on(t, i, t->st); }
 
 /* After the stream is synchronised: per-category milliseconds. */
 static void tr_collect(tracer_t *t, double *ms)
@@ -520,6 +524,48 @@ void cutc_release_workspace(void)
         }                                                                                                            \
     } while (0)
 
+/* A second stream per device (non-blocking, default priority) for the
+ * Jastrow of the next r1 chunk of a split stage 1, and the two events that
+ * order it against the build stream; made on first use, kept until the
+ * process ends. */
+static struct {
+    hipStream_t st;
+    hipEvent_t eq, ej;
+} g_side[WS_MAX_DEV];
+
+static int side_stream(hipStream_t *s, hipEvent_t *eq, hipEvent_t *ej)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
+        return CUTC_ERR_HIP;
+    if (!g_side[dev].st) {
+        hipStream_t x = NULL;
+        hipEvent_t a = NULL, b = NULL;
+        int rc = hip_rc(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        if (rc == CUTC_OK)
+            rc = hip_rc(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+        if (rc == CUTC_OK)
+            rc = hip_rc(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+        if (rc != CUTC_OK) {
+            if (a)
+                hipEventDestroy(a);
+            if (b)
+                hipEventDestroy(b);
+            if (x)
+                hipStreamDestroy(x);
+            hipGetLastError(); /* the build goes on in order on its own stream */
+            return rc;
+        }
+        g_side[dev].st = x;
+        g_side[dev].eq = a;
+        g_side[dev].ej = b;
+    }
+    *s = g_side[dev].st;
+    *eq = g_side[dev].eq;
+    *ej = g_side[dev].ej;
+    return CUTC_OK;
+}
+
 /* ------------------------------------------------------------ stage 1 */
 
 static int stage1(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
@@ -644,6 +690,56 @@ static void s1pre_carve(s1pre_t *pre, char *base, int64_t n2, int n_ao)
     pre->valid = 0;
 }
 
+/* What the Jastrow of one r1 chunk of stage 1 reads and writes. */
+typedef struct {
+    const jprog_t *jp;
+    const double *r1, *r2, *rn;
+    int n_nuc;
+    int64_t n2, ldk;
+    double *tab, *G, *gpart, *f2tab;
+} jchunk_t;
+
+/* The Jastrow of r1 points [i0, i0 + n1e) on stream s: the per-point table,
+ * then the 4 G rows of every point (row stride ldk, component stride
+ * n1e ldk).  With gpart (modular) the staged pair kernel also writes the
+ * partial row sums of squares and *norms is set; otherwise the caller takes
+ * the row exponents from G. */
+static int jast_chunk(const jchunk_t *a, int64_t i0, int64_t n1e, hipStream_t s, int *norms)
+{
+    const cutc_jprog *prog = &a->jp->h;
+    int rc = cutc_launch_jast_prep(a->r1, i0, n1e, a->rn, a->n_nuc, a->jp->dmono, prog->n_mono, a->tab, prog->ts, s);
+    *norms = 0;
+    if (rc == CUTC_OK && a->gpart) {
+        rc = cutc_launch_jast_pairs_norms(a->r1, a->r2, a->rn, a->n_nuc, a->tab, prog->ts, a->jp->dcls, prog->n_cls,
+                                          prog->kmax, i0, n1e, a->n2, a->ldk, a->G, a->ldk, n1e * a->ldk, a->gpart, s);
+        *norms = rc == CUTC_OK;
+        if (rc == CUTC_ERR_UNSUPPORTED)
+            rc = CUTC_OK;
+    }
+    if (rc == CUTC_OK && !*norms)
+        rc = cutc_launch_jast_pairs(a->r1, a->r2, a->rn, a->n_nuc, a->tab, prog->ts, a->jp->dcls, prog->n_cls,
+                                    prog->kmax, i0, n1e, 0, a->n2, a->ldk, a->G, a->ldk, n1e * a->ldk, a->f2tab, s);
+    return rc;
+}
+
+/* jast_chunk on the side stream js, after the work queued so far on the
+ * build stream; ej marks its end */
+static int jast_queue(tracer_t *tr, const jchunk_t *a, int64_t i0, int64_t n1e, hipStream_t js, hipEvent_t eq,
+                      hipEvent_t ej, int *norms)
+{
+    int rc = hip_rc(hipEventRecord(eq, tr->st));
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipStreamWaitEvent(js, eq, 0));
+    if (rc == CUTC_OK) {
+        const int ev = tr_begin_on(tr, CAT_JAST, js);
+        rc = jast_chunk(a, i0, n1e, js, norms);
+        tr_end_on(tr, ev, js);
+    }
+    if (rc == CUTC_OK)
+        rc = hip_rc(hipEventRecord(ej, js));
+    return rc;
+}
+
 /* Stage 1 with a split-precision contraction.  INT8 / BF16_CRT / FP16_CRT:
  * the modular (CRT) scheme, pair residues quantised on the fly from w2 and
  * chi, each G chunk quantised after the Jastrow block, one int8 (or bf16 /
@@ -655,15 +751,21 @@ static void s1pre_carve(s1pre_t *pre, char *base, int64_t n2, int n_ao)
  * into 256-pair tiles by the K steps where their quantised values are
  * non-zero, and every GEMM runs, per tile, only the K steps where that tile
  * has a non-zero residue (opt->dense: all of them); pre (optional) keeps the
- * orderings across the r1 chunks of a streamed call. */
+ * orderings across the r1 chunks of a streamed call.
+ * With one K pass the Jastrow runs on a second stream, one chunk ahead: that
+ * of the first chunk beside the pair orderings (it reads only the permuted
+ * r2), that of chunk c + 1 once chunk c's G has been quantised, beside chunk
+ * c's contraction (the FP VALU work of the one against the memory-bound
+ * GEMMs of the other); chunk c + 1's quantiser waits for it.  Not under
+ * graph capture or in the allocation-free entry point. */
 static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t n2, int n_ao, int n_nuc,
                         const double *r1, const double *r2, const double *wr2, const double *rn, const double *aos2,
                         const jprog_t *jp, double *X, tracer_t *tr, cutc_stats *stats, s1pre_t *pre)
 {
     const cutc_jprog *prog = &jp->h;
-    const cutc_jmono *dmono = jp->dmono;
-    const int *dcls = jp->dcls;
     const hipStream_t st = tr->st;
+    hipStream_t js = st; /* the stream of the Jastrow */
+    hipEvent_t eq = NULL, ej = NULL;
     const int64_t P = pl->n_pairs, ldk = pl->ldk, n1c = pl->n1_chunk;
     const int nmod = opt->slices;
     int *pairs = NULL;
@@ -745,6 +847,12 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         WS_TAKE(box, sizeof(double) * 6);
         WS_TAKE(cws, cutc_cluster_bytes(P, KT));
     }
+    /* the Jastrow one chunk ahead on the side stream js (js == st: in order) */
+    if (kpass == 1 && n1 > 0 && (sparse || n1 > n1c) && !tr->capture && !g_ws_fixed.base &&
+        side_stream(&js, &eq, &ej) != CUTC_OK)
+        js = st;
+    jchunk_t jc = {jp, r1, r2, rn, n_nuc, n2, ldk, tab, G, gpart, f2tab}; /* r2: the permuted copy once made */
+    int queued = 0, norms = 0; /* the next chunk's Jastrow is on js; its G row sums are written */
 
     ev = tr_begin(tr, CAT_SPLIT);
     if (sparse && pre) {
@@ -762,6 +870,11 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         r2 = r2p;
         wr2 = wr2p;
         aos2 = aos2p;
+        jc.r2 = r2;
+        if (rc == CUTC_OK && js != st) {
+            rc = jast_queue(tr, &jc, 0, n1 < n1c ? n1 : n1c, js, eq, ej, &norms);
+            queued = rc == CUTC_OK;
+        }
         if (rc == CUTC_OK && !(pre && pre->valid))
             rc = cutc_rowexp_pairw(wr2, aos2, n2, n_ao, emode, rexpN, st);
         if (rc == CUTC_OK && !(pre && pre->valid))
@@ -794,22 +907,16 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
     for (int64_t i0 = 0; i0 < n1; i0 += n1c) {
         const int64_t n1e = (n1 - i0 < n1c) ? n1 - i0 : n1c;
         nchunks++;
-        ev = tr_begin(tr, CAT_JAST);
-        rc = cutc_launch_jast_prep(r1, i0, n1e, rn, n_nuc, dmono, prog->n_mono, tab, prog->ts, st);
         /* modular: the pair kernel also sums the squares of the G rows it
          * writes (their 2-norm exponents), where its staged form runs */
-        int norms = 0;
-        if (rc == CUTC_OK && gpart) {
-            rc = cutc_launch_jast_pairs_norms(r1, r2, rn, n_nuc, tab, prog->ts, dcls, prog->n_cls, prog->kmax, i0,
-                                              n1e, n2, ldk, G, ldk, n1e * ldk, gpart, st);
-            norms = rc == CUTC_OK;
-            if (rc == CUTC_ERR_UNSUPPORTED)
-                rc = CUTC_OK;
+        if (queued) {
+            rc = hip_rc(hipStreamWaitEvent(st, ej, 0));
+            queued = 0;
+        } else {
+            ev = tr_begin(tr, CAT_JAST);
+            rc = jast_chunk(&jc, i0, n1e, st, &norms);
+            tr_end(tr, ev);
         }
-        if (rc == CUTC_OK && !norms)
-            rc = cutc_launch_jast_pairs(r1, r2, rn, n_nuc, tab, prog->ts, dcls, prog->n_cls, prog->kmax, i0, n1e, 0,
-                                        n2, ldk, G, ldk, n1e * ldk, f2tab, st);
-        tr_end(tr, ev);
         TRY(rc);
         ev = tr_begin(tr, CAT_SPLIT);
         rc = norms ? cutc_rowexp_finish(gpart, cutc_jast_norm_blocks(ldk), 4 * n1e, CUTC_ROWEXP_NORM, rexpG, st)
@@ -820,6 +927,13 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
                        : cutc_crt_quant_k_pk(&crt, G, ldk, 4 * n1e, n2, ldk, rexpG, Gpl, st);
         tr_end(tr, ev);
         TRY(rc);
+        /* G, its row sums and the point table are free: the next chunk's
+         * Jastrow goes on js, beside this chunk's contraction */
+        if (js != st && i0 + n1c < n1) {
+            const int64_t i1 = i0 + n1c;
+            TRY(jast_queue(tr, &jc, i1, n1 - i1 < n1c ? n1 - i1 : n1c, js, eq, ej, &norms));
+            queued = 1;
+        }
         if (kpass == 1) {
             ev = tr_begin(tr, CAT_GEMM1);
             rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
@@ -888,8 +1002,11 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
         stats->k_density = d;
     }
 done:
-    if (rc != CUTC_OK && !tr->capture)
+    if (rc != CUTC_OK && !tr->capture) {
         hipStreamSynchronize(st);
+        if (js != st)
+            hipStreamSynchronize(js);
+    }
     return rc;
 }
 
