@@ -1225,9 +1225,11 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
             /* the chunk's r1 order, then the integrands at the permuted points */
             rc = cutc_order_points(r1 + 3 * (r0 + i0), n1e, box, skeys, perm1, cutc_sort_len(n1e), st);
         }
+        /* (block-sparse: H alone, which orders the H pairs; H and NH follow
+         * in that order below) */
         if (rc == CUTC_OK)
-            rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, NHp, P, plane,
-                                           sparse ? perm1 : NULL, st);
+            rc = cutc_launch_r1_integrands(wr1, aos1, n1, r0 + i0, n1e, n_ao, pairs, Hp, sparse ? NULL : NHp, P,
+                                           plane, sparse ? perm1 : NULL, st);
         for (int m = 0; m < 4 && rc == CUTC_OK && padn > 0; m++)
             rc = cutc_launch_fill((m == 0 ? Hp : NHp + (m - 1) * plane) + pad0, padn, 0.0, st);
         tr_end(tr, ev);

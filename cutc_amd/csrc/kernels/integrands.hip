@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void r1_integrands_kernel(const double *__rest
             const double va = s[a * R + r], vb = s[b * R + r];
             const int64_t row = r0 + r;
             Hp[row * ldq + q] = w * vb * va;
-            for (int m = 0; m < 3; m++) {
+            for (int m = 0; m < 3 && NHp; m++) {
                 const double da = s[(m + 1) * cs + a * R + r], db = s[(m + 1) * cs + b * R + r];
                 NHp[m * nh_plane + row * ldq + q] = w * (va * db - vb * da);
             }

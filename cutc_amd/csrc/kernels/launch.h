@@ -67,7 +67,8 @@ int cutc_launch_jast_pairs_norms(const double *r1, const double *r2, const doubl
 int cutc_launch_pair_table(int n_ao, int *pairs, hipStream_t st);
 int cutc_launch_pair_weights(const double *wr2, const double *aos2, int64_t n2_ld, int64_t jj0, int64_t n2,
                              int64_t n2_fill, int n_ao, double *Apk, int64_t lda, hipStream_t st);
-/* perm (optional): row r of Hp / NHp is point ii0 + perm[r] instead of ii0 + r */
+/* perm (optional): row r of Hp / NHp is point ii0 + perm[r] instead of ii0 + r;
+ * NHp NULL: Hp only */
 int cutc_launch_r1_integrands(const double *wr1, const double *aos1, int64_t n1_ld, int64_t ii0, int64_t n1,
                               int n_ao, const int *pairs, double *Hp, double *NHp, int64_t ldq, int64_t nh_plane,
                               const uint32_t *perm, hipStream_t st);
