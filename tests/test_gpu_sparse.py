@@ -77,3 +77,19 @@ def test_sparse_equals_dense(torch, system, precision):
     print(f"  vs FP64 build: X {ex:.2e} int_2e_ao {ei:.2e}", flush=True)
     assert ex <= TOL and ei <= TOL
     assert np.isfinite(float(torch.linalg.vector_norm(Is)))
+
+
+@pytest.mark.parametrize("precision", ["int8", "bf16crt"])
+def test_chunked_equals_one_chunk(torch, system, precision):
+    """r1 chunks of 128 points (the Jastrow of each next chunk on the side
+    stream, beside the current chunk's contraction) against the whole r1 set
+    in one chunk: every G row's residues, exponents and rebuild are the
+    same arithmetic, so X and int_2e_ao agree bit for bit."""
+    name, s, ds, X64, I64 = system
+    print(f"chunked {name} {s.shape_str()} {precision}", flush=True)
+    Xc, Ic, stc = build(torch, ds, precision, 0, n1_chunk=128)
+    X1, I1, st1 = build(torch, ds, precision, 0, n1_chunk=s.n1)
+    print(f"  chunks {stc['n1_chunks']} / {st1['n1_chunks']}", flush=True)
+    assert stc["n1_chunks"] == (s.n1 + 127) // 128 and st1["n1_chunks"] == 1
+    assert torch.equal(Xc, X1), "X differs between chunked and one-chunk builds"
+    assert torch.equal(Ic, I1), "int_2e_ao differs between chunked and one-chunk builds"
