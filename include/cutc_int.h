@@ -167,7 +167,9 @@ void cutc_default_options(cutc_options *opt);
 /* Whole path on device pointers (inputs already in HBM), on `stream`
  * (a hipStream_t, NULL = default stream); returns when the work is
  * complete.  X: (n_ao,n_ao,n_grid1,4) device buffer.  int_2e_ao: n_ao^4.
- * stats may be NULL. */
+ * stats may be NULL.  The split precisions run the Jastrow of the next r1
+ * chunk on a second stream of the library's (one per device, non-blocking,
+ * default priority), ordered against `stream` by events. */
 int cutc_int_device(const cutc_options *opt, int n_grid1, int n_grid2, int n_ao, int n_nuc, int size_bh,
                     const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
                     const double *aos_data1, const double *aos_data2, const double *c_bh, const int *m_bh,
