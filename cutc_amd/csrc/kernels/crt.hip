@@ -1157,12 +1157,87 @@ __global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t
     }
 }
 
+// crt_value of the 4 consecutive elements C[0..3] (4-byte aligned, planes
+// `stride` apart): one dword load per modulus, the same arithmetic per
+// element as crt_value
+__device__ static inline void crt_value4(const cutc_crt &c, const int8_t *C, int64_t stride, double (&v)[4])
+{
+    double sh[4] = {0.0, 0.0, 0.0, 0.0}, sm[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < c.n; i++) {
+        const uint32_t w = *(const uint32_t *)(C + (int64_t)i * stride);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int x = (int)(int8_t)(w >> (8 * q)) * c.y[i];
+            int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
+            if (u >= c.m[i])
+                u -= c.m[i];
+            if (u < 0)
+                u += c.m[i];
+            const double du = (double)u;
+            sh[q] = fma(du, c.Mi[i][0], sh[q]);
+            sm[q] = fma(du, c.Mi[i][1], sm[q]);
+            sl[q] = fma(du, c.Mi[i][2], sl[q]);
+        }
+    }
+    const double two40 = 1099511627776.0, two80 = 1208925819614629174706176.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double s = (sh[q] * two80 + sm[q] * two40) + sl[q];
+        const double r = rint(s * c.Minv);
+        const double vh = sh[q] - r * c.M[0], vm = sm[q] - r * c.M[1], vl = sl[q] - r * c.M[2];
+        v[q] = (vh * two80 + vm * two40) + vl;
+    }
+}
+
+// rebuild_x_kernel with 4 consecutive p per thread (P a multiple of 4):
+// thread per (p..p+3, j), a block 1024 p of one row j
+__global__ __launch_bounds__(256) void rebuild_x4_kernel(cutc_crt c, const int8_t *__restrict__ C, int64_t P,
+                                                         int64_t N, const int *__restrict__ rexpA,
+                                                         const int *__restrict__ rexpB,
+                                                         const int *__restrict__ pairs, int n_ao, double *X,
+                                                         int64_t n1, int64_t n1c, int64_t i0, int npb)
+{
+    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t j0 = t / npb, p = (int64_t)(t % npb) * 1024 + 4 * threadIdx.x;
+    if (p >= P)
+        return;
+    int k[4], l[4], ea[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        k[q] = pairs[2 * (p + q)];
+        l[q] = pairs[2 * (p + q) + 1];
+        ea[q] = rexpA[p + q] - 2 * c.bits;
+    }
+    const int64_t n2 = (int64_t)n_ao * n_ao;
+    for (int64_t j = j0; j < N; j += gridDim.x / npb) {
+        double V[4];
+        crt_value4(c, C + j * P + p, P * N, V);
+        const int eb = rexpB[j];
+        const int64_t m = j / n1c, i1 = j - m * n1c;
+        double *o = X + (m * n1 + i0 + i1) * n2;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double v = ldexp(V[q], ea[q] + eb);
+            o[k[q] + (int64_t)n_ao * l[q]] = v;
+            o[l[q] + (int64_t)n_ao * k[q]] = v;
+        }
+    }
+}
+
 extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N,
                                   const int *rexpA, const int *rexpB, const int *pairs, int n_ao, double *X, int64_t n1,
                                   int64_t n1c, int64_t i0, hipStream_t st)
 {
     if (P <= 0 || N <= 0)
         return 0;
+    if (P % 4 == 0 && (uintptr_t)C % 4 == 0) {
+        const int64_t npb = (P + 1023) / 1024, rows = N < 1048576 / npb + 1 ? N : 1048576 / npb + 1;
+        if (npb > 1048576)
+            return CUTC_ERR_ARG;
+        hipLaunchKernelGGL(rebuild_x4_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, N, rexpA,
+                           rexpB, pairs, n_ao, X, n1, n1c, i0, (int)npb);
+        return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
+    }
     const int64_t npb = (P + 255) / 256, rows = N < 1048576 / npb + 1 ? N : 1048576 / npb + 1;
     if (npb > 1048576)
         return CUTC_ERR_ARG;
