@@ -938,7 +938,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             ev = tr_begin(tr, CAT_GEMM1);
             rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Gpl, 4 * n1e, (const uint16_t *)Apl, P, ldk, rexpG,
                                       rexpA, (double *)Cpl, st)
-                       : cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, ldk, kl, kc_, 0, Cpl, 0, st);
+                       : cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, ldk, kl, kc_, 0, 1, Cpl, 0, st);
             tr_end(tr, ev);
             TRY(rc);
         }
@@ -963,7 +963,7 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             tr_end(tr, ev);
             TRY(rc);
             ev = tr_begin(tr, CAT_GEMM1);
-            rc = cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, kc, kl, kc_, 0, Cpl, kp > 0, st);
+            rc = cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, kc, kl, kc_, 0, 1, Cpl, kp > 0, st);
             tr_end(tr, ev);
             TRY(rc);
             if (kl && i0 == 0 && stats && !tr->capture) {
@@ -1025,6 +1025,32 @@ static int split2_heuristic(int64_t P, int64_t K, size_t budget)
     while (s < 16 && tiles * s < 512 && K / (2 * s) >= 1024 && (size_t)(2 * s) * P * P * 8 <= budget / 16)
         s *= 2;
     return s;
+}
+
+/* K splits of the block-sparse modular step-2 contraction (256 x 256 blocks,
+ * one per pair of 256-pair tiles, lists per H tile): split when the tiles
+ * give fewer than 512 blocks, into at most 16 parts whose int8 residues
+ * (nmod P^2 bytes each) stay within a sixteenth of the budget; stage 2 runs
+ * fewer parts where a contraction has too few steps for them. */
+static int split2_crt_heuristic(int64_t P, int nmod, size_t budget)
+{
+    const int64_t t = (P + 255) / 256, tiles = t * t;
+    int s = 1;
+    while (s < 16 && tiles * s < 512 && (size_t)(2 * s) * nmod * P * P <= budget / 16)
+        s *= 2;
+    return s;
+}
+
+/* the step-2 K split for the precision of opt (resolved options): FP64
+ * and the block-sparse modular forms split small contractions, the slice
+ * forms and the dense modular ones do not */
+static int split2_for(const cutc_options *opt, int64_t P, int64_t K, size_t budget)
+{
+    if (opt->precision == CUTC_PREC_FP64)
+        return split2_heuristic(P, K, budget);
+    if (is_crt(opt->precision) && !opt->dense)
+        return split2_crt_heuristic(P, opt->slices, budget);
+    return 1;
 }
 
 /* Autotuner of the split: times the FP64 step-2 contraction and its reduce
@@ -1164,10 +1190,12 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
     /* operand planes of both sides and the contraction's results */
     const size_t bpl = slice ? (size_t)nmod * 2 * CUTC_OZ_ROWS(P) * 3 * kp_max
                              : rb * nmod * CUTC_PK_ROWS_A(P) * 3 * kp_max;
-    const size_t bcp = slice ? sizeof(double) * P * P : (size_t)nmod * P * P;
     /* block-sparse state: r1 order of a chunk, sort keys, the H-side pair
      * table, occupancy flags, step lists and counts per 256-pair H tile */
     const int sparse = split && !slice && r1 != NULL && P > 0;
+    /* modular, list-driven: up to nsc parts of a K split, each its own residues */
+    const int nsc = (sparse && !opt->dense && nsplit2 > 1) ? nsplit2 : 1;
+    const size_t bcp = slice ? sizeof(double) * P * P : (size_t)nsc * nmod * P * P;
     const int kind = slice ? 0 : crt_kind(opt->precision);
     const int64_t tilesH = CUTC_PK_ROWS_A(P) / CUTC_SP_ROWS;
     const int64_t KT2 = sparse ? 3 * kp_max / CUTC_CRT_KSTEP(kind) : 0;
@@ -1323,11 +1351,17 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 tr_end(tr, ev);
                 TRY(rc);
                 const int lists = sparse && !opt->dense;
+                /* K split parts: at least 8 padding units of the lists each */
+                int ns = 1;
+                if (lists && nsc > 1) {
+                    const int64_t np = nseg * kp / CUTC_CRT_KSTEP(kind) / (8 * CUTC_CRT_LIST_ALIGN(kind));
+                    ns = np >= nsc ? nsc : (np > 1 ? (int)np : 1);
+                }
                 ev = tr_begin(tr, CAT_GEMM2);
                 rc = slice ? cutc_oz_gemm(nmod, f16, (const uint16_t *)Bpl, P, (const uint16_t *)Apl, P, nseg * kp, rexpB,
                                           rexpA, (double *)Cpl, st)
                            : cutc_crt_gemm_pk_list(&crt, Bpl, P, Apl, P, nseg * kp, lists ? klist : NULL,
-                                                   lists ? kcount : NULL, 1, Cpl, 0, st);
+                                                   lists ? kcount : NULL, 1, ns, Cpl, 0, st);
                 tr_end(tr, ev);
                 TRY(rc);
                 if (lists && stats && !tr->capture) {
@@ -1352,7 +1386,7 @@ static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int6
                 rc = slice ? cutc_scatter_m((const double *)Cpl, 1, P, alpha, pairs, n_ao, out,
                                             part == 0 ? (first ? 0 : 1) : 2, st)
                            : cutc_crt_rebuild_m(&crt, Cpl, P, rexpA, rexpB, alpha, pairs, sparse ? pairsH : NULL,
-                                                n_ao, out, part, first ? 0 : 1, st);
+                                                n_ao, out, part, first ? 0 : 1, ns, st);
                 tr_end(tr, ev);
                 TRY(rc);
             }
@@ -1537,7 +1571,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             tr_end(&tr, ev);
         } else {
             rc = stage2(opt, &pl, n1, 0, n1, n_ao, r1, rn, n_nuc, wr1, aos1, X, out, 0, 1,
-                        split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
+                        split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
         }
     }
     tr_end(&tr, evs);
@@ -1738,7 +1772,7 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
         have_tr = 1;
         int evt = tr_begin(&tr, CAT_TOTAL);
         rc = streamed_chunks(opt, &pl, n1, n2, n_ao, n_nuc, r1, wr1, r2, wr2, rn, aos1, aos2, &prog, out, Xc, nc,
-                             split2_heuristic(pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats,
+                             split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats,
                              preb ? &pre : NULL);
         tr_end(&tr, evt);
         if (rc == CUTC_OK)
@@ -1891,15 +1925,21 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             TRY(cutc_make_plan(nc[i], y->n_grid2, y->n_ao, y->n_nuc, opt->precision, opt->slices, budget,
                                nc[i] < 8192 ? nc[i] : 8192, &pl[i]));
             TRY(jprog_load(y->n_nuc, y->size_bh, y->c_bh, y->m_bh, y->n_bh, y->o_bh, y->n_grid2 > 0, &jp[i]));
-            /* the autotuner picks the K split of the FP64 step-2 contraction */
+            /* the autotuner picks the K split of the FP64 step-2 contraction;
+             * the modular ones split as in a single call */
             split2[i] = opt->precision == CUTC_PREC_FP64
                             ? tune_split2(pl[i].n_pairs, y->n_ao, pl[i].n1_chunk_2e, budget, st,
                                           bst ? &bst->ms_tune : NULL)
-                            : 1;
-            /* stage 1 also holds the program's per-r1 table, stage 2 the split partial sums */
+                            : split2_for(opt, pl[i].n_pairs, pl[i].n1_chunk_2e, budget);
+            /* stage 1 also holds the program's per-r1 table, stage 2 the split
+             * partial sums (FP64) or the further parts' residues (modular) */
+            const int64_t P2 = pl[i].n_pairs * pl[i].n_pairs;
             const size_t b1 = pl[i].bytes_stage1 + sizeof(double) * ((size_t)jp[i].h.ts * pl[i].n1_chunk + 1) + 4096;
             const size_t b2 = pl[i].bytes_stage2 +
-                              (split2[i] > 1 ? sizeof(double) * split2[i] * pl[i].n_pairs * pl[i].n_pairs + 256 : 0);
+                              (split2[i] <= 1 ? 0
+                               : opt->precision == CUTC_PREC_FP64
+                                   ? sizeof(double) * split2[i] * P2 + 256
+                                   : (size_t)(split2[i] - 1) * opt->slices * P2 + 256);
             const size_t b = b1 > b2 ? b1 : b2;
             if (b > need)
                 need = b;

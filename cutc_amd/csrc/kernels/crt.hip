@@ -627,7 +627,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
                                                              int tiles_b, int64_t M, int64_t N, int mod,
                                                              double inv_mod, int8_t *__restrict__ C, int accum,
                                                              const uint32_t *__restrict__ klist,
-                                                             const int *__restrict__ kcount)
+                                                             const int *__restrict__ kcount, int64_t sstride)
 {
     static_assert(SEG % D == 0 && D % BAR == 0 && (int64_t)SEG * 64 * 16384 < 2147483648LL, "segment");
     static_assert(D >= L + BAR && L >= BAR + 1 && (8 + 2 * WB) * L <= 63 && (WB == 1 || WB == 2), "pipeline shape");
@@ -644,8 +644,20 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     else
         cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
     const int lt = LA ? ta : tb;                       // the tile whose list is run
-    const int64_t NS = SP ? (int64_t)kcount[lt] : KT;  // steps run
+    int64_t NS = SP ? (int64_t)kcount[lt] : KT;        // steps run
     const uint32_t *kl = SP ? klist + (int64_t)lt * KT : nullptr;
+    if constexpr (SP) {
+        // K split over gridDim.y: part y runs its share of the list, in
+        // whole groups of D steps (the list's padding unit), into its own
+        // residues sstride bytes on
+        if (gridDim.y > 1) {
+            const int64_t g = NS / D, y = blockIdx.y, ny = gridDim.y;
+            const int64_t s0 = g * y / ny * D, s1 = g * (y + 1) / ny * D;
+            kl += s0;
+            NS = s1 - s0;
+            C += y * sstride;
+        }
+    }
     int64_t kbase = 0;                                 // first step of the current segment
     uint32_t so_nx = 0;                                // SP: offset of the step L + 1 ahead
     auto off = [&](int64_t s) -> uint32_t {
@@ -854,7 +866,7 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
                                                             int tiles_b, int64_t M, int64_t N, int mod,
                                                             double inv_mod, int8_t *__restrict__ C, int accum,
                                                             const uint32_t *__restrict__ klist,
-                                                            const int *__restrict__ kcount)
+                                                            const int *__restrict__ kcount, int64_t sstride)
 {
     constexpr int SEG = 85 * W;
     static_assert(W * 32 * 127 * 127 + 383 < (1 << 24), "exact FP32 sums between reductions");
@@ -875,8 +887,20 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     else
         cutc_group_tile<4>(t, tiles_a, tiles_b, ta, tb);
     const int lt = LA ? ta : tb;                       // the tile whose list is run
-    const int64_t NS = SP ? (int64_t)kcount[lt] : KT;  // steps run
+    int64_t NS = SP ? (int64_t)kcount[lt] : KT;        // steps run
     const uint32_t *kl = SP ? klist + (int64_t)lt * KT : nullptr;
+    if constexpr (SP) {
+        // K split over gridDim.y: part y runs its share of the list, in
+        // whole windows of W steps (the list's padding unit), into its own
+        // residues sstride bytes on
+        if (gridDim.y > 1) {
+            const int64_t g = NS / W, y = blockIdx.y, ny = gridDim.y;
+            const int64_t s0 = g * y / ny * W, s1 = g * (y + 1) / ny * W;
+            kl += s0;
+            NS = s1 - s0;
+            C += y * sstride;
+        }
+    }
     int64_t kbase = 0;                                 // first step of the current segment
     uint32_t so_nx = 0;                                // SP: offset of the step L + 1 ahead
     auto off = [&](int64_t s) -> uint32_t {
@@ -1041,17 +1065,17 @@ static_assert(CUTC_PK_KALIGN / 32 % 24 == 0, "16-bit K steps per window");
 extern "C" int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
                                 int64_t kpad, int8_t *C, int accumulate, hipStream_t st)
 {
-    return cutc_crt_gemm_pk_list(c, A, M, B, N, kpad, NULL, NULL, 0, C, accumulate, st);
+    return cutc_crt_gemm_pk_list(c, A, M, B, N, kpad, NULL, NULL, 0, 1, C, accumulate, st);
 }
 
 extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N,
-                                     int64_t kpad, const uint32_t *klist, const int *kcount, int list_a, int8_t *C,
-                                     int accumulate, hipStream_t st)
+                                     int64_t kpad, const uint32_t *klist, const int *kcount, int list_a,
+                                     int nsplit, int8_t *C, int accumulate, hipStream_t st)
 {
     if (M <= 0 || N <= 0)
         return 0;
     if (kpad <= 0 || kpad % CUTC_PK_KALIGN || c->kind < CUTC_CRT_I8 || c->kind > CUTC_CRT_FP16 ||
-        (!klist) != (!kcount))
+        (!klist) != (!kcount) || nsplit < 1 || nsplit > 65535 || (nsplit > 1 && !klist))
         return CUTC_ERR_ARG;
     const int64_t KT = pk_steps(c->kind, kpad), rb = CUTC_CRT_RBYTES(c->kind);
     if (4 * KT * PK_TILE > 4294967295LL) /* 32-bit offsets inside a wave's four row blocks */
@@ -1063,14 +1087,15 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
     if (ta * tb > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
-    const dim3 grid((unsigned)(ta * tb));
+    const dim3 grid((unsigned)(ta * tb), (unsigned)nsplit);
+    const int64_t sstride = (int64_t)c->n * M * N; /* bytes between the parts of a K split */
     for (int i = 0; i < c->n; i++) {
         /* int8: 6-stage rings, loads 4 steps ahead, a barrier every second
          * step; 16-bit: CRT16_SHAPE */
         int8_t *Ci = C + (int64_t)i * M * N;
 #define GEMM_ARGS                                                                                                      \
     grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate, \
-        klist, kcount
+        klist, kcount, sstride
 #define GEMM_SP(K8, K16)                                                                                               \
     do {                                                                                                               \
         if (c->kind == CUTC_CRT_I8)                                                                                    \
@@ -1104,6 +1129,36 @@ __device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, in
     double sh = 0.0, sm = 0.0, sl = 0.0;
     for (int i = 0; i < c.n; i++) {
         const int x = (int)C[(int64_t)i * stride] * c.y[i];
+        int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
+        if (u >= c.m[i])
+            u -= c.m[i];
+        if (u < 0)
+            u += c.m[i];
+        const double du = (double)u;
+        sh = fma(du, c.Mi[i][0], sh);
+        sm = fma(du, c.Mi[i][1], sm);
+        sl = fma(du, c.Mi[i][2], sl);
+    }
+    const double two40 = 1099511627776.0, two80 = 1208925819614629174706176.0;
+    const double s = (sh * two80 + sm * two40) + sl;
+    const double q = rint(s * c.Minv);
+    const double vh = sh - q * c.M[0], vm = sm - q * c.M[1], vl = sl - q * c.M[2];
+    return (vh * two80 + vm * two40) + vl;
+}
+
+// crt_value of the sum of the residues of nsplit parts sstride bytes apart
+// (the parts of a K-split contraction): the sum of residues is a residue of
+// the whole sum, |sum y_i| < 2^15 nsplit, and for nsplit < 512 the float
+// quotient is still within one of the exact one, which the corrections undo
+__device__ static inline double crt_value_parts(const cutc_crt &c, const int8_t *C, int64_t stride, int nsplit,
+                                                int64_t sstride)
+{
+    double sh = 0.0, sm = 0.0, sl = 0.0;
+    for (int i = 0; i < c.n; i++) {
+        int r = 0;
+        for (int s = 0; s < nsplit; s++)
+            r += (int)C[(int64_t)s * sstride + (int64_t)i * stride];
+        const int x = r * c.y[i];
         int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
         if (u >= c.m[i])
             u -= c.m[i];
@@ -1264,7 +1319,7 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
                                                         const int *__restrict__ rexpB, double alpha,
                                                         const int *__restrict__ pairs,
                                                         const int *__restrict__ pairsB, int n_ao, double *M, int part,
-                                                        int mode, int npb)
+                                                        int mode, int npb, int nsplit, int64_t sstride)
 {
     int64_t q0, p;
     rebuild_rowcol(npb, q0, p);
@@ -1278,7 +1333,7 @@ __global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t
         const int a = pairsB[2 * q], b = pairsB[2 * q + 1];
         if (part == 1 && a == b)
             continue;
-        const double v = alpha * ldexp(crt_value(c, C + q * P + p, P * P), ea + rexpB[q]);
+        const double v = alpha * ldexp(crt_value_parts(c, C + q * P + p, P * P, nsplit, sstride), ea + rexpB[q]);
         double *o = M + n2 * (part == 0 ? a + (int64_t)n_ao * b : b + (int64_t)n_ao * a) + kl;
         if (mode == 0)
             *o = v;
@@ -1339,17 +1394,18 @@ __global__ __launch_bounds__(256) void finalize_m_kernel(double *M, int n_ao)
 
 extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
                                   const int *rexpB, double alpha, const int *pairs, const int *pairsB, int n_ao,
-                                  double *M, int part, int mode, hipStream_t st)
+                                  double *M, int part, int mode, int nsplit, hipStream_t st)
 {
     if (P <= 0)
         return 0;
-    if ((part != 0 && part != 1) || (mode != 0 && mode != 1))
+    if ((part != 0 && part != 1) || (mode != 0 && mode != 1) || nsplit < 1 || nsplit > 256)
         return CUTC_ERR_ARG;
     const int64_t npb = (P + 255) / 256, rows = P < 1048576 / npb + 1 ? P : 1048576 / npb + 1;
     if (npb > 1048576)
         return CUTC_ERR_ARG;
     hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, rexpA, rexpB,
-                       alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, part, mode, (int)npb);
+                       alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, part, mode, (int)npb, nsplit,
+                       (int64_t)c->n * P * P);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 

@@ -236,10 +236,10 @@ int cutc_crt_quant_colk_pk_occ(const cutc_crt *c, const cutc_colsrc *src, int64_
  * accumulate: add the residues already in C (one K pass of several). */
 int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
                      int8_t *C, int accumulate, hipStream_t st);
-/* Block-sparse stage 1 (order.hip).  The r2 points are reordered (nearest
- * nucleus, then distance to it) and the pair rows sorted by where their
- * quantised values are non-zero, so that whole (CUTC_SP_ROWS-pair B tile,
- * K step) tiles of the pair planes are zero; the quantiser flags the others
+/* Block-sparse stage 1 (order.hip).  The r2 points are reordered along a
+ * Morton curve and the pair rows clustered by where their quantised values
+ * are non-zero, so that whole (CUTC_SP_ROWS-pair B tile, K step) tiles of
+ * the pair planes are zero; the quantiser flags the others
  * (occ[tb][kt], one byte each, cleared first) and the GEMM runs, per B tile,
  * only the listed steps (blocks of 256 A x CUTC_SP_ROWS B rows).  A zero
  * tile adds 0 to every modular sum: the results are those of the dense
@@ -251,10 +251,13 @@ int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t
 #define CUTC_SP_ROWS 256  /* pair rows per B tile of the step lists */
 /* the same GEMM over the steps klist[t KT + s], s < kcount[t], of tile t
  * (KT = kpad / KSTEP) of the B side (list_a = 0) or of the A side
- * (list_a = 1); klist = kcount = NULL: every step */
+ * (list_a = 1); klist = kcount = NULL: every step.  nsplit > 1 (lists
+ * only): each tile's list runs in nsplit consecutive parts (whole padding
+ * units), part s into its own residues at C + s n M N, whose sum modulo
+ * m_i is the whole result (cutc_crt_rebuild_m adds them) */
 int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
-                          const uint32_t *klist, const int *kcount, int list_a, int8_t *C, int accumulate,
-                          hipStream_t st);
+                          const uint32_t *klist, const int *kcount, int list_a, int nsplit, int8_t *C,
+                          int accumulate, hipStream_t st);
 /* pair planes with (k, l) of row p from ptab (NULL: the packed pair index)
  * and, with occ, the non-zero flags of every (B tile, step) */
 int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2, const double *aos2, int64_t ld2, int64_t n2,
@@ -323,10 +326,12 @@ int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N,
  * X^{0..2} NH) into the columns b + n_ao a, a < b, rows k + n_ao l, k <= l
  * only; mode 0 stores, 1 adds.  pairs: the packed-order table of the B side
  * (rexpA); pairsB (optional): the table of the rows of the GEMM's A side
- * (rexpB), when it differs.  cutc_crt_finalize_m then forms the full M. */
+ * (rexpB), when it differs.  cutc_crt_finalize_m then forms the full M.
+ * nsplit: the parts of a K-split GEMM (C + s n P^2, s < nsplit <= 256),
+ * whose residues are added before the rebuild. */
 int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P, const int *rexpA,
                        const int *rexpB, double alpha, const int *pairs, const int *pairsB, int n_ao, double *M,
-                       int part, int mode, hipStream_t st);
+                       int part, int mode, int nsplit, hipStream_t st);
 /* M[:, ab] = S + A, M[:, ba] = S - A (a < b) and the rows k > l from k < l,
  * from the halves cutc_crt_rebuild_m writes */
 int cutc_crt_finalize_m(double *M, int n_ao, hipStream_t st);

@@ -238,7 +238,8 @@ typedef struct cutc_batch_stats {
     double ms_launch; /* the one graph launch, stream-ordered hipEvents */
     int64_t n_nodes;  /* nodes recorded in the graph */
     int n_systems;
-    int n_split;      /* systems whose FP64 step-2 contraction the autotuner split along K */
+    int n_split;      /* systems whose step-2 contraction is split along K (FP64: by
+                         the autotuner; block-sparse modular: by tile count) */
     int n_branches;   /* concurrent branches the systems were recorded on */
 } cutc_batch_stats;
 

@@ -10,7 +10,10 @@ agree exactly; each is also checked against the FP64 build (<= 1e-10).
 Systems: the 20-nucleus basis (n_ao = 200, 157 pair tiles) on 20000 r2
 points, several tiles with empty or short lists and padded ones; a
 3-nucleus system with one pair tile whose every step is occupied (the list
-is the whole K); a system whose r2 count is not a multiple of the step."""
+is the whole K); a system whose r2 count is not a multiple of the step; a
+small basis (n_ao = 30, 2 x 2 tiles of 256 pairs) with 12000 r1 points, whose
+stage-2 contraction runs its lists in K-split parts (4 for X^3 H, 12 for
+X^{0..2} NH), each into its own residues, summed before the rebuild."""
 import numpy as np
 import pytest
 
@@ -44,6 +47,7 @@ SYSTEMS = {
     "cube20": lambda: synth.cube20(n1=1024, n2=20000),
     "triatomic": lambda: synth.triatomic(n1=700, n2=3001),
     "odd": lambda: synth.make_system(7, 23, 515, 9973, 91, name="odd"),
+    "splitk": lambda: synth.make_system(4, 30, 12000, 1500, 1201, name="splitk"),
 }
 
 
