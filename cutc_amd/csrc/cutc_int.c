@@ -1015,14 +1015,14 @@ done:
 /* K splits of the FP64 step-2 contraction for n_pairs P and K = n1 chunk:
  * split only when the P x P grid of 128 x 128 tiles is small (< 256 blocks),
  * into at most 16 slices of at least 1024 K rows, keeping the partial sums
- * within a sixteenth of the budget. */
-static int split2_heuristic(int64_t P, int64_t K, size_t budget)
+ * within `room` bytes (split2_room). */
+static int split2_heuristic(int64_t P, int64_t K, size_t room)
 {
     const int64_t tiles = ((P + 127) / 128) * ((P + 127) / 128);
     if (tiles >= 256 || K < 2048)
         return 1;
     int s = 1;
-    while (s < 16 && tiles * s < 512 && K / (2 * s) >= 1024 && (size_t)(2 * s) * P * P * 8 <= budget / 16)
+    while (s < 16 && tiles * s < 512 && K / (2 * s) >= 1024 && (size_t)(2 * s) * P * P * 8 <= room)
         s *= 2;
     return s;
 }
@@ -1030,13 +1030,13 @@ static int split2_heuristic(int64_t P, int64_t K, size_t budget)
 /* K splits of the block-sparse modular step-2 contraction (256 x 256 blocks,
  * one per pair of 256-pair tiles, lists per H tile): split when the tiles
  * give fewer than 512 blocks, into at most 16 parts whose int8 residues
- * (nmod P^2 bytes each) stay within a sixteenth of the budget; stage 2 runs
- * fewer parts where a contraction has too few steps for them. */
-static int split2_crt_heuristic(int64_t P, int nmod, size_t budget)
+ * beyond the first part's (nmod P^2 bytes each) fit in `room` bytes; stage 2
+ * runs fewer parts where a contraction has too few steps for them. */
+static int split2_crt_heuristic(int64_t P, int nmod, size_t room)
 {
     const int64_t t = (P + 255) / 256, tiles = t * t;
     int s = 1;
-    while (s < 16 && tiles * s < 512 && (size_t)(2 * s) * nmod * P * P <= budget / 16)
+    while (s < 16 && tiles * s < 512 && (size_t)(2 * s - 1) * nmod * P * P <= room)
         s *= 2;
     return s;
 }
@@ -1044,13 +1044,21 @@ static int split2_crt_heuristic(int64_t P, int nmod, size_t budget)
 /* the step-2 K split for the precision of opt (resolved options): FP64
  * and the block-sparse modular forms split small contractions, the slice
  * forms and the dense modular ones do not */
-static int split2_for(const cutc_options *opt, int64_t P, int64_t K, size_t budget)
+static int split2_for(const cutc_options *opt, int64_t P, int64_t K, size_t room)
 {
     if (opt->precision == CUTC_PREC_FP64)
-        return split2_heuristic(P, K, budget);
+        return split2_heuristic(P, K, room);
     if (is_crt(opt->precision) && !opt->dense)
-        return split2_crt_heuristic(P, opt->slices, budget);
+        return split2_crt_heuristic(P, opt->slices, room);
     return 1;
+}
+
+/* device bytes a step-2 K split may take beside the plan's stage 2: a
+ * sixteenth of the budget, within what the plan leaves free */
+static size_t split2_room(const cutc_plan *pl, size_t budget)
+{
+    const size_t left = budget > pl->bytes_stage2 ? budget - pl->bytes_stage2 : 0;
+    return left < budget / 16 ? left : budget / 16;
 }
 
 /* Autotuner of the split: times the FP64 step-2 contraction and its reduce
@@ -1069,7 +1077,7 @@ static int tune_split2(int64_t P, int n_ao, int64_t K, size_t budget, hipStream_
     for (int i = 0; i < g_ntune; i++)
         if (g_tune[i].P == P && g_tune[i].K == K)
             return g_tune[i].split;
-    int smax = split2_heuristic(P, K, budget);
+    int smax = split2_heuristic(P, K, budget / 16);
     if (smax <= 1)
         return 1;
     while (smax < 16 && K / (2 * smax) >= 1024 && (size_t)(2 * smax) * P * P * 8 <= budget / 16)
@@ -1134,7 +1142,7 @@ static int tune_split2(int64_t P, int n_ao, int64_t K, size_t budget, hipStream_
     if (ms_tune)
         *ms_tune += now_ms() - t0;
     if (rc != CUTC_OK)
-        return split2_heuristic(P, K, budget);
+        return split2_heuristic(P, K, budget / 16);
     if (g_ntune < TUNE_CACHE) {
         g_tune[g_ntune].P = P;
         g_tune[g_ntune].K = K;
@@ -1571,7 +1579,8 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             tr_end(&tr, ev);
         } else {
             rc = stage2(opt, &pl, n1, 0, n1, n_ao, r1, rn, n_nuc, wr1, aos1, X, out, 0, 1,
-                        split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats);
+                        split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, split2_room(&pl, device_budget(opt))), &tr,
+                        stats);
         }
     }
     tr_end(&tr, evs);
@@ -1772,8 +1781,8 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
         have_tr = 1;
         int evt = tr_begin(&tr, CAT_TOTAL);
         rc = streamed_chunks(opt, &pl, n1, n2, n_ao, n_nuc, r1, wr1, r2, wr2, rn, aos1, aos2, &prog, out, Xc, nc,
-                             split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, device_budget(opt)), &tr, stats,
-                             preb ? &pre : NULL);
+                             split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, split2_room(&pl, device_budget(opt))),
+                             &tr, stats, preb ? &pre : NULL);
         tr_end(&tr, evt);
         if (rc == CUTC_OK)
             rc = hip_rc(hipStreamSynchronize(st));
@@ -1930,7 +1939,7 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             split2[i] = opt->precision == CUTC_PREC_FP64
                             ? tune_split2(pl[i].n_pairs, y->n_ao, pl[i].n1_chunk_2e, budget, st,
                                           bst ? &bst->ms_tune : NULL)
-                            : split2_for(opt, pl[i].n_pairs, pl[i].n1_chunk_2e, budget);
+                            : split2_for(opt, pl[i].n_pairs, pl[i].n1_chunk_2e, budget / 16);
             /* stage 1 also holds the program's per-r1 table, stage 2 the split
              * partial sums (FP64) or the further parts' residues (modular) */
             const int64_t P2 = pl[i].n_pairs * pl[i].n_pairs;
