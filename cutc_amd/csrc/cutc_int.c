@@ -334,20 +334,25 @@ static struct {
 } g_ws_fixed;
 
 /* free device memory plus this device's cached workspace, minus a margin */
-static size_t device_budget(const cutc_options *o)
+/* device bytes a call may plan with: free memory plus the workspace arena
+ * (and `held`, memory of the library's the call will reuse), less a 1 GiB
+ * margin, at most mem_limit */
+static size_t device_budget_plus(const cutc_options *o, size_t held)
 {
     if (g_ws_fixed.base)
         return (o && o->mem_limit && o->mem_limit < g_ws_fixed.size) ? o->mem_limit : g_ws_fixed.size;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess)
         return 0;
-    fr += arena_size_current();
+    fr += arena_size_current() + held;
     const size_t margin = (size_t)1 << 30;
     size_t b = fr > margin ? fr - margin : 0;
     if (o && o->mem_limit && o->mem_limit < b)
         b = o->mem_limit;
     return b;
 }
+
+static size_t device_budget(const cutc_options *o) { return device_budget_plus(o, 0); }
 
 static void dfree(void *p)
 {
@@ -452,11 +457,14 @@ int cutc_internal_workspace(size_t need, char **base)
 /* Device copies of the host entry points' arrays (inputs, X, int_2e_ao): a
  * second grow-only allocation per device, kept and released with the
  * workspace, so that repeated cutc_int_ex calls do not allocate (a fresh
- * 90 GB hipMalloc beside a full workspace took 2.5 s). */
-static struct {
+ * 90 GB hipMalloc beside a full workspace took 2.5 s).  The streamed build's
+ * X chunk and kept stage-1 orderings: a third (42 GB on the 40-nucleus
+ * workload, ~1 s to allocate and free per call). */
+typedef struct {
     char *base;
     size_t size;
-} g_hbuf[WS_MAX_DEV];
+} devbuf_t;
+static devbuf_t g_hbuf[WS_MAX_DEV], g_sbuf[WS_MAX_DEV];
 
 static void free_on_device(int d, char **base, size_t *size)
 {
@@ -477,19 +485,21 @@ static void release_arenas(void)
     for (int d = 0; d < WS_MAX_DEV; d++) {
         free_on_device(d, &g_arena[d].base, &g_arena[d].size);
         free_on_device(d, &g_hbuf[d].base, &g_hbuf[d].size);
+        free_on_device(d, &g_sbuf[d].base, &g_sbuf[d].size);
     }
 }
 
-/* the host-array buffer of the current device, at least need bytes; when it
- * has to grow and the device is full, the workspace is given back first */
-static int hbuf_get(size_t need, char **base)
+/* buffer bufs[current device], at least need bytes (shrink: and at most
+ * 2 need, reallocated otherwise); when it has to be allocated and the
+ * device is full, the workspace is given back first */
+static int buf_get(devbuf_t *bufs, size_t need, int shrink, char **base)
 {
     int dev = 0;
     *base = NULL;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
         return CUTC_ERR_HIP;
-    if (g_hbuf[dev].size < need) {
-        free_on_device(dev, &g_hbuf[dev].base, &g_hbuf[dev].size);
+    if (bufs[dev].size < need || (shrink && bufs[dev].size / 2 > need)) {
+        free_on_device(dev, &bufs[dev].base, &bufs[dev].size);
         void *p = NULL;
         hipError_t e = hipMalloc(&p, need);
         if (e == hipErrorOutOfMemory) {
@@ -499,11 +509,22 @@ static int hbuf_get(size_t need, char **base)
         }
         if (e != hipSuccess)
             return hip_rc(e);
-        g_hbuf[dev].base = (char *)p;
-        g_hbuf[dev].size = need;
+        bufs[dev].base = (char *)p;
+        bufs[dev].size = need;
     }
-    *base = g_hbuf[dev].base;
+    *base = bufs[dev].base;
     return CUTC_OK;
+}
+
+static int hbuf_get(size_t need, char **base) { return buf_get(g_hbuf, need, 0, base); }
+
+/* bytes of bufs[current device] */
+static size_t buf_size_current(const devbuf_t *bufs)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WS_MAX_DEV)
+        return 0;
+    return bufs[dev].size;
 }
 
 void cutc_internal_release_workspace(void) { release_arenas(); }
@@ -1701,8 +1722,10 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
 /* Streamed build: X is never stored whole.  The r1 points run in chunks of
  * nc; each chunk's X (4 x nc x n_ao^2, one device allocation for the call)
  * goes straight from stage 1 into stage 2, which adds it into out; out is
- * symmetrised after the last chunk.  nc takes a quarter of the memory left
- * over by the stages' fixed buffers (or opt->n1_chunk when smaller). */
+ * symmetrised after the last chunk.  nc takes half the memory left over by
+ * the stages' fixed buffers (or opt->n1_chunk when smaller): every X chunk
+ * repeats the pair quantisation of stage 1 and the orderings, quantisation
+ * and rebuild of stage 2, while smaller G chunks of stage 1 cost little. */
 static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int n_nuc, int size_bh,
                          const double *r1, const double *wr1, const double *r2, const double *wr2, const double *rn,
                          const double *aos1, const double *aos2, const double *c_bh, const int *m_bh,
@@ -1749,7 +1772,9 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
      * r1-independent orderings of stage 1 across the chunks */
     const int keep = is_crt(opt->precision) && n2 > 0;
     const size_t preby = keep ? s1pre_bytes(n2, n_ao) : 0;
-    const size_t budget0 = device_budget(opt), budget = budget0 > preby ? budget0 - preby : 0;
+    /* the buffer of an earlier call (Xc and the orderings) counts as free */
+    const size_t budget0 = device_budget_plus(opt, buf_size_current(g_sbuf)),
+                 budget = budget0 > preby ? budget0 - preby : 0;
     cutc_plan p1;
     rc = cutc_make_plan(1, n2, n_ao, n_nuc, opt->precision, opt->slices, budget, 0, &p1);
     if (rc)
@@ -1757,7 +1782,7 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
     {
         const size_t fixed = p1.bytes_stage1 > p1.bytes_stage2 ? p1.bytes_stage1 : p1.bytes_stage2;
         const size_t xper = 32 * (size_t)nao2;
-        int64_t nc = budget > fixed ? (int64_t)((budget - fixed) / 4 / xper) : 0;
+        int64_t nc = budget > fixed ? (int64_t)((budget - fixed) / 2 / xper) : 0;
         if (opt->n1_chunk > 0 && opt->n1_chunk < nc)
             nc = opt->n1_chunk;
         if (nc < 1) {
@@ -1770,9 +1795,15 @@ static int streamed_path(const cutc_options *opt, int n1, int n2, int n_ao, int 
         } else {
             nc = n1;
         }
-        TRYH(hipMalloc((void **)&Xc, sizeof(double) * 4 * (size_t)nc * nao2));
+        /* X chunk and kept orderings in the grow-only buffer of the streamed
+         * calls (at most twice this call's need, so that the stages keep
+         * the memory the chunk size assumed) */
+        const size_t xb = ws_round(sizeof(double) * 4 * (size_t)nc * nao2);
+        char *sb = NULL;
+        TRY(buf_get(g_sbuf, xb + (keep && nc < n1 ? preby : 0), 1, &sb));
+        Xc = (double *)sb;
         if (keep && nc < n1) {
-            TRYH(hipMalloc((void **)&preb, preby));
+            preb = sb + xb;
             s1pre_carve(&pre, preb, n2, n_ao);
         }
         cutc_plan pl;
@@ -1800,8 +1831,6 @@ done:
     hipStreamSynchronize(st);
     if (have_tr)
         tr_free(&tr);
-    dfree(Xc);
-    dfree(preb);
     jprog_free(&prog);
     return rc;
 }
@@ -1898,7 +1927,7 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
          * usually: fewer chunks mean fewer step-2 passes) */
         int64_t cmax = opt->n1_chunk;
         if (cmax <= 0) {
-            const size_t quarter = device_budget(opt) / 4;
+            const size_t quarter = device_budget_plus(opt, buf_size_current(g_sbuf)) / 4;
             int64_t hi = 1;
             for (int i = 0; i < nsys; i++)
                 if (sys[i].n_grid1 > hi)
@@ -1923,7 +1952,8 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
         const double t0 = now_ms();
         /* plans against the device budget; the workspace of a branch is the
          * largest stage of the batch */
-        const size_t budget = device_budget(opt);
+        /* (the chunk buffers of an earlier batch or streamed call count as free) */
+        const size_t budget = device_budget_plus(opt, buf_size_current(g_sbuf));
         size_t need = 0;
         for (int i = 0; i < nsys; i++) {
             const cutc_system *y = &sys[i];
@@ -1976,8 +2006,14 @@ static int batch_path(const cutc_options *opt, int nsys, const cutc_system *sys,
             branch[i] = b;
             load[b] += system_work(&sys[i]);
         }
-        for (int r = 0; r < nb && xbytes; r++)
-            TRYH(hipMalloc((void **)&Xc[r], xbytes));
+        /* the branches' chunk buffers in the grow-only buffer of the
+         * streamed calls (at most twice what they need) */
+        if (xbytes) {
+            char *sb = NULL;
+            TRY(buf_get(g_sbuf, (size_t)nb * ws_round(xbytes), 1, &sb));
+            for (int r = 0; r < nb; r++)
+                Xc[r] = (double *)(sb + (size_t)r * ws_round(xbytes));
+        }
         ws_t ws;
         TRY(ws_begin(&ws, (size_t)nb * need));
         /* record on private streams (the legacy default stream cannot be
@@ -2069,7 +2105,6 @@ done:
         if (ej[r])
             hipEventDestroy(ej[r]);
         tr_free(&tr[r]);
-        dfree(Xc[r]);
     }
     if (exec)
         hipGraphExecDestroy(exec);
