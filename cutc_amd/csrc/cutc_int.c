@@ -1188,7 +1188,7 @@ static int tune_split2(int64_t P, int n_ao, int64_t K, size_t budget, hipStream_
  * (opt->dense: all of them).  Its results go into halves of M (see
  * cutc_crt_rebuild_m), completed with the symmetrisation. */
 static int stage2(const cutc_options *opt, const cutc_plan *pl, int64_t n1, int64_t r0, int64_t cnt, int n_ao,
-                  const double *r1, const double *rn, int n_nuc, const double *wr1, const double *aos1,
+                  const double *r1, const double *wr1, const double *aos1,
                   const double *X, double *out, int accumulate, int symm, int nsplit2, tracer_t *tr,
                   cutc_stats *stats)
 {
@@ -1599,7 +1599,7 @@ static int device_path(const cutc_options *opt, int stages, int n1, int n2, int 
             rc = cutc_launch_fill(out, (int64_t)n_ao * n_ao * n_ao * n_ao, 0.0, st);
             tr_end(&tr, ev);
         } else {
-            rc = stage2(opt, &pl, n1, 0, n1, n_ao, r1, rn, n_nuc, wr1, aos1, X, out, 0, 1,
+            rc = stage2(opt, &pl, n1, 0, n1, n_ao, r1, wr1, aos1, X, out, 0, 1,
                         split2_for(opt, pl.n_pairs, pl.n1_chunk_2e, split2_room(&pl, device_budget(opt))), &tr,
                         stats);
         }
@@ -1711,8 +1711,8 @@ static int streamed_chunks(const cutc_options *opt, const cutc_plan *pl, int64_t
         tr_end(tr, evs);
         if (rc == CUTC_OK) {
             evs = tr_begin(tr, CAT_STAGE2);
-            rc = stage2(opt, pl, n1, r0, cnt, n_ao, r1, rn, n_nuc, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, nsplit2,
-                        tr, stats);
+            rc = stage2(opt, pl, n1, r0, cnt, n_ao, r1, wr1, aos1, Xc, out, r0 > 0, r0 + cnt >= n1, nsplit2, tr,
+                        stats);
             tr_end(tr, evs);
         }
     }
