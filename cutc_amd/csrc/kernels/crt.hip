@@ -1119,110 +1119,41 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
 
 // ---------------------------------------------------------------- rebuild
 
-// V from the residues c_i of one element (see crt_host.c), as a double.
-// u_i = (c_i y_i) mod m_i in [0, m_i): |c_i y_i| < 2^15, so the float
-// quotient is within 2^-7 of the exact one and its floor is off by at most
-// one, which the two corrections undo (checked against the integer
-// remainder for every modulus, y and residue).
-__device__ static inline double crt_value(const cutc_crt &c, const int8_t *C, int64_t stride)
-{
-    double sh = 0.0, sm = 0.0, sl = 0.0;
-    for (int i = 0; i < c.n; i++) {
-        const int x = (int)C[(int64_t)i * stride] * c.y[i];
-        int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
-        if (u >= c.m[i])
-            u -= c.m[i];
-        if (u < 0)
-            u += c.m[i];
-        const double du = (double)u;
-        sh = fma(du, c.Mi[i][0], sh);
-        sm = fma(du, c.Mi[i][1], sm);
-        sl = fma(du, c.Mi[i][2], sl);
-    }
-    const double two40 = 1099511627776.0, two80 = 1208925819614629174706176.0;
-    const double s = (sh * two80 + sm * two40) + sl;
-    const double q = rint(s * c.Minv);
-    const double vh = sh - q * c.M[0], vm = sm - q * c.M[1], vl = sl - q * c.M[2];
-    return (vh * two80 + vm * two40) + vl;
-}
+// The rebuilt values V of W consecutive elements C[0 .. W) (W-byte aligned;
+// the residue planes of the moduli `stride` bytes apart, the parts of a
+// K-split contraction `sstride` bytes apart, their residues summed first),
+// as doubles (see crt_host.c).  u_i = (c_i y_i) mod m_i in [0, m_i) for the
+// summed residue c_i: |c_i y_i| < 2^15 nsplit, so for nsplit < 512 the float
+// quotient is within one of the exact one, and the two corrections undo
+// that (checked against the integer remainder for every modulus, y and
+// residue of one part).  One load of W bytes per modulus and part.
+template <int W> struct wbytes;
+template <> struct wbytes<1> { typedef uint8_t T; };
+template <> struct wbytes<2> { typedef uint16_t T; };
+template <> struct wbytes<4> { typedef uint32_t T; };
 
-// crt_value of the sum of the residues of nsplit parts sstride bytes apart
-// (the parts of a K-split contraction): the sum of residues is a residue of
-// the whole sum, |sum y_i| < 2^15 nsplit, and for nsplit < 512 the float
-// quotient is still within one of the exact one, which the corrections undo
-__device__ static inline double crt_value_parts(const cutc_crt &c, const int8_t *C, int64_t stride, int nsplit,
-                                                int64_t sstride)
+template <int W>
+__device__ static inline void crt_values(const cutc_crt &c, const int8_t *C, int64_t stride, int nsplit,
+                                         int64_t sstride, double (&v)[W])
 {
-    double sh = 0.0, sm = 0.0, sl = 0.0;
-    for (int i = 0; i < c.n; i++) {
-        int r = 0;
-        for (int s = 0; s < nsplit; s++)
-            r += (int)C[(int64_t)s * sstride + (int64_t)i * stride];
-        const int x = r * c.y[i];
-        int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
-        if (u >= c.m[i])
-            u -= c.m[i];
-        if (u < 0)
-            u += c.m[i];
-        const double du = (double)u;
-        sh = fma(du, c.Mi[i][0], sh);
-        sm = fma(du, c.Mi[i][1], sm);
-        sl = fma(du, c.Mi[i][2], sl);
-    }
-    const double two40 = 1099511627776.0, two80 = 1208925819614629174706176.0;
-    const double s = (sh * two80 + sm * two40) + sl;
-    const double q = rint(s * c.Minv);
-    const double vh = sh - q * c.M[0], vm = sm - q * c.M[1], vl = sl - q * c.M[2];
-    return (vh * two80 + vm * two40) + vl;
-}
-
-// Block -> (row of C, block of 256 columns) of the rebuild kernels: one
-// row's npb blocks are consecutive in each XCD's contiguous range of the
-// grid (cutc_xcd_remap), so the scattered stores of a row (its whole output
-// line of n_ao^2 doubles) meet in one L2 and leave it as full lines; rows
-// past the grid repeat with stride gridDim.x / npb.
-__device__ static inline void rebuild_rowcol(int npb, int64_t &row, int64_t &p)
-{
-    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
-    row = t / npb;
-    p = (int64_t)(t % npb) * 256 + threadIdx.x;
-}
-
-// thread per (p, j), p fastest
-__global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t *__restrict__ C,
-                                                        int64_t P, int64_t N, const int *__restrict__ rexpA,
-                                                        const int *__restrict__ rexpB,
-                                                        const int *__restrict__ pairs, int n_ao, double *X,
-                                                        int64_t n1, int64_t n1c, int64_t i0, int npb)
-{
-    int64_t j0, p;
-    rebuild_rowcol(npb, j0, p);
-    if (p >= P)
-        return;
-    const int k = pairs[2 * p], l = pairs[2 * p + 1];
-    const int64_t n2 = (int64_t)n_ao * n_ao;
-    const int ea = rexpA[p] - 2 * c.bits;
-    for (int64_t j = j0; j < N; j += gridDim.x / npb) {
-        const double V = crt_value(c, C + j * P + p, P * N);
-        const double v = ldexp(V, ea + rexpB[j]);
-        const int64_t m = j / n1c, i1 = j - m * n1c;
-        double *o = X + (m * n1 + i0 + i1) * n2;
-        o[k + (int64_t)n_ao * l] = v;
-        o[l + (int64_t)n_ao * k] = v;
-    }
-}
-
-// crt_value of the 4 consecutive elements C[0..3] (4-byte aligned, planes
-// `stride` apart): one dword load per modulus, the same arithmetic per
-// element as crt_value
-__device__ static inline void crt_value4(const cutc_crt &c, const int8_t *C, int64_t stride, double (&v)[4])
-{
-    double sh[4] = {0.0, 0.0, 0.0, 0.0}, sm[4] = {0.0, 0.0, 0.0, 0.0}, sl[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = 0; i < c.n; i++) {
-        const uint32_t w = *(const uint32_t *)(C + (int64_t)i * stride);
+    double sh[W], sm[W], sl[W];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int x = (int)(int8_t)(w >> (8 * q)) * c.y[i];
+    for (int q = 0; q < W; q++)
+        sh[q] = sm[q] = sl[q] = 0.0;
+    for (int i = 0; i < c.n; i++) {
+        int r[W];
+#pragma unroll
+        for (int q = 0; q < W; q++)
+            r[q] = 0;
+        for (int s = 0; s < nsplit; s++) {
+            const uint32_t w = *(const typename wbytes<W>::T *)(C + (int64_t)s * sstride + (int64_t)i * stride);
+#pragma unroll
+            for (int q = 0; q < W; q++)
+                r[q] += (int)(int8_t)(w >> (8 * q));
+        }
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+            const int x = r[q] * c.y[i];
             int u = x - c.m[i] * (int)__builtin_floorf((float)x * (float)c.inv_m[i]);
             if (u >= c.m[i])
                 u -= c.m[i];
@@ -1236,47 +1167,70 @@ __device__ static inline void crt_value4(const cutc_crt &c, const int8_t *C, int
     }
     const double two40 = 1099511627776.0, two80 = 1208925819614629174706176.0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < W; q++) {
         const double s = (sh[q] * two80 + sm[q] * two40) + sl[q];
-        const double r = rint(s * c.Minv);
-        const double vh = sh[q] - r * c.M[0], vm = sm[q] - r * c.M[1], vl = sl[q] - r * c.M[2];
+        const double t = rint(s * c.Minv);
+        const double vh = sh[q] - t * c.M[0], vm = sm[q] - t * c.M[1], vl = sl[q] - t * c.M[2];
         v[q] = (vh * two80 + vm * two40) + vl;
     }
 }
 
-// rebuild_x_kernel with 4 consecutive p per thread (P a multiple of 4):
-// thread per (p..p+3, j), a block 1024 p of one row j
-__global__ __launch_bounds__(256) void rebuild_x4_kernel(cutc_crt c, const int8_t *__restrict__ C, int64_t P,
-                                                         int64_t N, const int *__restrict__ rexpA,
-                                                         const int *__restrict__ rexpB,
-                                                         const int *__restrict__ pairs, int n_ao, double *X,
-                                                         int64_t n1, int64_t n1c, int64_t i0, int npb)
+// Block -> (row of C, block of 256 W columns) of the rebuild kernels: one
+// row's npb blocks are consecutive in each XCD's contiguous range of the
+// grid (cutc_xcd_remap), so the scattered stores of a row (its whole output
+// line of n_ao^2 doubles) meet in one L2 and leave it as full lines; rows
+// past the grid repeat with stride gridDim.x / npb.  Thread: columns
+// p .. p + W - 1.
+template <int W> __device__ static inline void rebuild_rowcol(int npb, int64_t &row, int64_t &p)
 {
     const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t j0 = t / npb, p = (int64_t)(t % npb) * 1024 + 4 * threadIdx.x;
+    row = t / npb;
+    p = (int64_t)(t % npb) * 256 * W + W * threadIdx.x;
+}
+
+// thread per (p .. p + W - 1, j), P a multiple of W
+template <int W>
+__global__ __launch_bounds__(256) void rebuild_x_kernel(cutc_crt c, const int8_t *__restrict__ C, int64_t P,
+                                                        int64_t N, const int *__restrict__ rexpA,
+                                                        const int *__restrict__ rexpB,
+                                                        const int *__restrict__ pairs, int n_ao, double *X,
+                                                        int64_t n1, int64_t n1c, int64_t i0, int npb)
+{
+    int64_t j0, p;
+    rebuild_rowcol<W>(npb, j0, p);
     if (p >= P)
         return;
-    int k[4], l[4], ea[4];
+    int k[W], l[W], ea[W];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < W; q++) {
         k[q] = pairs[2 * (p + q)];
         l[q] = pairs[2 * (p + q) + 1];
         ea[q] = rexpA[p + q] - 2 * c.bits;
     }
     const int64_t n2 = (int64_t)n_ao * n_ao;
     for (int64_t j = j0; j < N; j += gridDim.x / npb) {
-        double V[4];
-        crt_value4(c, C + j * P + p, P * N, V);
+        double V[W];
+        crt_values<W>(c, C + j * P + p, P * N, 1, 0, V);
         const int eb = rexpB[j];
         const int64_t m = j / n1c, i1 = j - m * n1c;
         double *o = X + (m * n1 + i0 + i1) * n2;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < W; q++) {
             const double v = ldexp(V[q], ea[q] + eb);
             o[k[q] + (int64_t)n_ao * l[q]] = v;
             o[l[q] + (int64_t)n_ao * k[q]] = v;
         }
     }
+}
+
+// the widest W in {4, 2, 1} that divides P with C aligned to it
+static int rebuild_width(int64_t P, const int8_t *C)
+{
+    if (P % 4 == 0 && (uintptr_t)C % 4 == 0)
+        return 4;
+    if (P % 2 == 0 && (uintptr_t)C % 2 == 0)
+        return 2;
+    return 1;
 }
 
 extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P, int64_t N,
@@ -1285,24 +1239,25 @@ extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P,
 {
     if (P <= 0 || N <= 0)
         return 0;
-    if (P % 4 == 0 && (uintptr_t)C % 4 == 0) {
-        const int64_t npb = (P + 1023) / 1024, rows = N < 1048576 / npb + 1 ? N : 1048576 / npb + 1;
-        if (npb > 1048576)
-            return CUTC_ERR_ARG;
-        hipLaunchKernelGGL(rebuild_x4_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, N, rexpA,
-                           rexpB, pairs, n_ao, X, n1, n1c, i0, (int)npb);
-        return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
-    }
-    const int64_t npb = (P + 255) / 256, rows = N < 1048576 / npb + 1 ? N : 1048576 / npb + 1;
+    const int W = rebuild_width(P, C);
+    const int64_t npb = (P + 256 * W - 1) / (256 * W), rows = N < 1048576 / npb + 1 ? N : 1048576 / npb + 1;
     if (npb > 1048576)
         return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(rebuild_x_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, N, rexpA, rexpB,
-                       pairs, n_ao, X, n1, n1c, i0, (int)npb);
+    const dim3 grid((unsigned)(npb * rows));
+    if (W == 4)
+        hipLaunchKernelGGL(rebuild_x_kernel<4>, grid, dim3(256), 0, st, *c, C, P, N, rexpA, rexpB, pairs, n_ao, X,
+                           n1, n1c, i0, (int)npb);
+    else if (W == 2)
+        hipLaunchKernelGGL(rebuild_x_kernel<2>, grid, dim3(256), 0, st, *c, C, P, N, rexpA, rexpB, pairs, n_ao, X,
+                           n1, n1c, i0, (int)npb);
+    else
+        hipLaunchKernelGGL(rebuild_x_kernel<1>, grid, dim3(256), 0, st, *c, C, P, N, rexpA, rexpB, pairs, n_ao, X,
+                           n1, n1c, i0, (int)npb);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
-// thread per (p, q), p fastest; modes as the FP64 step-2 epilogue; the
-// rows q of the A side have their own pair table pairsB
+// thread per (p .. p + W - 1, q), p fastest, P a multiple of W; the rows q
+// of the A side have their own pair table pairsB.
 // The stage-2 results of the modular contraction go into M in halves, so
 // that every store is coalesced: thread p (X-side pairs in packed order,
 // (k, l) = pairs[p], k <= l, so that consecutive p are consecutive rows
@@ -1314,31 +1269,41 @@ extern "C" int cutc_crt_rebuild_x(const cutc_crt *c, const int8_t *C, int64_t P,
 // finalize_m_kernel then forms M[:, ab] = S + A, M[:, ba] = S - A and the
 // rows k > l from the rows l < k, which is the full layout the reference's
 // GEMMs write (before the symmetrisation M + M^T).
-__global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t *__restrict__ C,
-                                                        int64_t P, const int *__restrict__ rexpA,
-                                                        const int *__restrict__ rexpB, double alpha,
-                                                        const int *__restrict__ pairs,
+template <int W>
+__global__ __launch_bounds__(256) void rebuild_m_kernel(cutc_crt c, const int8_t *__restrict__ C, int64_t P,
+                                                        const int *__restrict__ rexpA, const int *__restrict__ rexpB,
+                                                        double alpha, const int *__restrict__ pairs,
                                                         const int *__restrict__ pairsB, int n_ao, double *M, int part,
                                                         int mode, int npb, int nsplit, int64_t sstride)
 {
     int64_t q0, p;
-    rebuild_rowcol(npb, q0, p);
+    rebuild_rowcol<W>(npb, q0, p);
     if (p >= P)
         return;
-    const int k = pairs[2 * p], l = pairs[2 * p + 1];
+    int64_t kl[W];
+    int ea[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+        kl[e] = pairs[2 * (p + e)] + (int64_t)n_ao * pairs[2 * (p + e) + 1];
+        ea[e] = rexpA[p + e] - 2 * c.bits;
+    }
     const int64_t n2 = (int64_t)n_ao * n_ao;
-    const int64_t kl = k + (int64_t)n_ao * l;
-    const int ea = rexpA[p] - 2 * c.bits;
     for (int64_t q = q0; q < P; q += gridDim.x / npb) {
         const int a = pairsB[2 * q], b = pairsB[2 * q + 1];
         if (part == 1 && a == b)
             continue;
-        const double v = alpha * ldexp(crt_value_parts(c, C + q * P + p, P * P, nsplit, sstride), ea + rexpB[q]);
-        double *o = M + n2 * (part == 0 ? a + (int64_t)n_ao * b : b + (int64_t)n_ao * a) + kl;
-        if (mode == 0)
-            *o = v;
-        else
-            *o += v;
+        double V[W];
+        crt_values<W>(c, C + q * P + p, P * P, nsplit, sstride, V);
+        const int eb = rexpB[q];
+        double *o = M + n2 * (part == 0 ? a + (int64_t)n_ao * b : b + (int64_t)n_ao * a);
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            const double v = alpha * ldexp(V[e], ea[e] + eb);
+            if (mode == 0)
+                o[kl[e]] = v;
+            else
+                o[kl[e]] += v;
+        }
     }
 }
 
@@ -1400,12 +1365,22 @@ extern "C" int cutc_crt_rebuild_m(const cutc_crt *c, const int8_t *C, int64_t P,
         return 0;
     if ((part != 0 && part != 1) || (mode != 0 && mode != 1) || nsplit < 1 || nsplit > 256)
         return CUTC_ERR_ARG;
-    const int64_t npb = (P + 255) / 256, rows = P < 1048576 / npb + 1 ? P : 1048576 / npb + 1;
+    const int W = rebuild_width(P, C);
+    const int64_t npb = (P + 256 * W - 1) / (256 * W), rows = P < 1048576 / npb + 1 ? P : 1048576 / npb + 1;
     if (npb > 1048576)
         return CUTC_ERR_ARG;
-    hipLaunchKernelGGL(rebuild_m_kernel, dim3((unsigned)(npb * rows)), dim3(256), 0, st, *c, C, P, rexpA, rexpB,
-                       alpha, pairs, pairsB ? pairsB : pairs, n_ao, M, part, mode, (int)npb, nsplit,
-                       (int64_t)c->n * P * P);
+    const dim3 grid((unsigned)(npb * rows));
+    const int64_t sstride = (int64_t)c->n * P * P;
+    const int *pb = pairsB ? pairsB : pairs;
+    if (W == 4)
+        hipLaunchKernelGGL(rebuild_m_kernel<4>, grid, dim3(256), 0, st, *c, C, P, rexpA, rexpB, alpha, pairs, pb, n_ao,
+                           M, part, mode, (int)npb, nsplit, sstride);
+    else if (W == 2)
+        hipLaunchKernelGGL(rebuild_m_kernel<2>, grid, dim3(256), 0, st, *c, C, P, rexpA, rexpB, alpha, pairs, pb, n_ao,
+                           M, part, mode, (int)npb, nsplit, sstride);
+    else
+        hipLaunchKernelGGL(rebuild_m_kernel<1>, grid, dim3(256), 0, st, *c, C, P, rexpA, rexpB, alpha, pairs, pb, n_ao,
+                           M, part, mode, (int)npb, nsplit, sstride);
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
 }
 
