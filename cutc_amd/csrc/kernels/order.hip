@@ -462,23 +462,36 @@ __global__ __launch_bounds__(CL_THREADS) void cl_select_kernel(const int *__rest
     __syncthreads();
     const uint32_t thr = (uint32_t)s_thr;
     const int quota = s_quota;
-    int eq_before = 0, sel_before = 0;
-    for (int64_t c0 = 0; c0 < P; c0 += CL_THREADS) {
-        const int64_t p = c0 + tid;
-        const bool live = p < P && !taken[p];
-        const uint32_t d = live ? dist[p] : 0u;
-        const int feq = live && d == thr;
-        int teq, tsel;
-        const int eqr = eq_before + cl_exscan(feq, tmp, &teq);
-        const int sel = live && (d < thr || (feq && eqr < quota));
-        const int selr = cl_exscan(sel, tmp, &tsel);
-        if (sel) {
-            order[base + sel_before + selr] = (uint32_t)p;
-            taken[p] = 1;
+    // ordered compaction: thread tid owns the contiguous rows [r0, r1); it
+    // counts its rows below the threshold and at it, one scan of each gives
+    // its place in the tile and the rank of its first row at the threshold
+    // (the first `quota` of those, in row order, are taken)
+    const int64_t run = (P + CL_THREADS - 1) / CL_THREADS;
+    const int64_t r0 = (int64_t)tid * run < P ? (int64_t)tid * run : P, r1 = r0 + run < P ? r0 + run : P;
+    int nlt = 0, neq = 0;
+    for (int64_t p = r0; p < r1; p++)
+        if (!taken[p]) {
+            const uint32_t d = dist[p];
+            nlt += d < thr;
+            neq += d == thr;
         }
-        eq_before += teq;
-        sel_before += tsel;
-    }
+    int teq, tlt;
+    int eqr = cl_exscan(neq, tmp, &teq);
+    const int ltr = cl_exscan(nlt, tmp, &tlt);
+    // rows placed before this thread's: all below the threshold in earlier
+    // runs and those at it among the first quota
+    int pos = ltr + (eqr < quota ? eqr : quota);
+    for (int64_t p = r0; p < r1; p++)
+        if (!taken[p]) {
+            const uint32_t d = dist[p];
+            const int sel = d < thr || (d == thr && eqr < quota);
+            eqr += d == thr;
+            if (sel) {
+                order[base + pos++] = (uint32_t)p;
+                taken[p] = 1;
+            }
+        }
+    __syncthreads();
     if (base + need < P)
         cl_pick_seed(cnt, P, taken, sbest, state);
 }
