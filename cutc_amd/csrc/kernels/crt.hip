@@ -577,6 +577,13 @@ extern "C" int cutc_crt_quant_colk_pk_occ(const cutc_crt *c, const cutc_colsrc *
 }
 
 typedef __attribute__((address_space(3))) int8_t lds_i8;
+// the moduli of one launch of the GEMM kernels (block z of the grid's
+// moduli uses m[z], inv_m[z])
+struct gemm_moduli {
+    int m[CUTC_CRT_MAX];
+    double inv_m[CUTC_CRT_MAX];
+};
+
 #define PK_TILE 2048 // bytes of one (32-row block, 64-byte K step) tile
 
 // s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt in bits 3:0 and
@@ -624,8 +631,9 @@ __device__ static inline void pk_dma16(const int8_t *g, int8_t *l)
 template <int D, int L, int BAR, int SEG, bool SP, int WB, bool LA>
 __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__restrict__ A,
                                                              const int8_t *__restrict__ B, int64_t KT, int tiles_a,
-                                                             int tiles_b, int64_t M, int64_t N, int mod,
-                                                             double inv_mod, int8_t *__restrict__ C, int accum,
+                                                             int tiles_b, int64_t M, int64_t N, gemm_moduli mods,
+                                                             int nsplit, int64_t pa, int64_t pb,
+                                                             int8_t *__restrict__ C, int accum,
                                                              const uint32_t *__restrict__ klist,
                                                              const int *__restrict__ kcount, int64_t sstride)
 {
@@ -634,10 +642,19 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     __shared__ __attribute__((aligned(16))) int8_t sB[D][4 * WB * PK_TILE];
     __shared__ uint32_t sK[SP ? SEG : 1];
 
-    // consecutive blocks share the listed tile (4 tiles of the other side
-    // each): they run the same list in step and are replaced together, so
-    // the listed tile is read once per group through L2
-    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    // one launch runs every modulus z (and every part y of a K split): the
+    // grid is z-major, then y, then the tiles; consecutive blocks share the
+    // listed tile (4 tiles of the other side each): they run the same list
+    // in step and are replaced together, so the listed tile is read once per
+    // group through L2
+    int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    const int zy = t / (tiles_a * tiles_b), z = zy / nsplit, y = zy - z * nsplit;
+    t -= zy * (tiles_a * tiles_b);
+    A += (int64_t)z * pa;
+    B += (int64_t)z * pb;
+    C += (int64_t)z * M * N;
+    const int mod = mods.m[z];
+    const double inv_mod = mods.inv_m[z];
     int ta, tb;
     if constexpr (LA)
         cutc_group_tile<4>(t, tiles_b, tiles_a, tb, ta);
@@ -647,15 +664,15 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_pk_kernel(const int8_t *__res
     int64_t NS = SP ? (int64_t)kcount[lt] : KT;        // steps run
     const uint32_t *kl = SP ? klist + (int64_t)lt * KT : nullptr;
     if constexpr (SP) {
-        // K split over gridDim.y: part y runs its share of the list, in
+        // K split in nsplit parts: part y runs its share of the list, in
         // whole groups of D steps (the list's padding unit), into its own
         // residues sstride bytes on
-        if (gridDim.y > 1) {
-            const int64_t g = NS / D, y = blockIdx.y, ny = gridDim.y;
-            const int64_t s0 = g * y / ny * D, s1 = g * (y + 1) / ny * D;
+        if (nsplit > 1) {
+            const int64_t g = NS / D;
+            const int64_t s0 = g * y / nsplit * D, s1 = g * (y + 1) / nsplit * D;
             kl += s0;
             NS = s1 - s0;
-            C += y * sstride;
+            C += (int64_t)y * sstride;
         }
     }
     int64_t kbase = 0;                                 // first step of the current segment
@@ -863,8 +880,9 @@ __host__ __device__ constexpr int crt16_red_elem(int pos) { return 2 * ((pos - 1
 template <bool F16, int NA, int DB, int L, int BAR, int W, bool SP, int WB, bool LA>
 __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__restrict__ A,
                                                             const int8_t *__restrict__ B, int64_t KT, int tiles_a,
-                                                            int tiles_b, int64_t M, int64_t N, int mod,
-                                                            double inv_mod, int8_t *__restrict__ C, int accum,
+                                                            int tiles_b, int64_t M, int64_t N, gemm_moduli mods,
+                                                            int nsplit, int64_t pa, int64_t pb,
+                                                            int8_t *__restrict__ C, int accum,
                                                             const uint32_t *__restrict__ klist,
                                                             const int *__restrict__ kcount, int64_t sstride)
 {
@@ -877,10 +895,19 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     __shared__ __attribute__((aligned(16))) int8_t sB[DB][4 * WB * PK_TILE];
     __shared__ uint32_t sK[SP ? SEG : 1];
 
-    // consecutive blocks share the listed tile (4 tiles of the other side
-    // each): they run the same list in step and are replaced together, so
-    // the listed tile is read once per group through L2
-    const int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    // one launch runs every modulus z (and every part y of a K split): the
+    // grid is z-major, then y, then the tiles; consecutive blocks share the
+    // listed tile (4 tiles of the other side each): they run the same list
+    // in step and are replaced together, so the listed tile is read once per
+    // group through L2
+    int t = cutc_xcd_remap(blockIdx.x, gridDim.x);
+    const int zy = t / (tiles_a * tiles_b), z = zy / nsplit, y = zy - z * nsplit;
+    t -= zy * (tiles_a * tiles_b);
+    A += (int64_t)z * pa;
+    B += (int64_t)z * pb;
+    C += (int64_t)z * M * N;
+    const int mod = mods.m[z];
+    const double inv_mod = mods.inv_m[z];
     int ta, tb;
     if constexpr (LA)
         cutc_group_tile<4>(t, tiles_b, tiles_a, tb, ta);
@@ -890,15 +917,15 @@ __global__ __launch_bounds__(256, 1) void crt16_gemm_kernel(const int8_t *__rest
     int64_t NS = SP ? (int64_t)kcount[lt] : KT;        // steps run
     const uint32_t *kl = SP ? klist + (int64_t)lt * KT : nullptr;
     if constexpr (SP) {
-        // K split over gridDim.y: part y runs its share of the list, in
+        // K split in nsplit parts: part y runs its share of the list, in
         // whole windows of W steps (the list's padding unit), into its own
         // residues sstride bytes on
-        if (gridDim.y > 1) {
-            const int64_t g = NS / W, y = blockIdx.y, ny = gridDim.y;
-            const int64_t s0 = g * y / ny * W, s1 = g * (y + 1) / ny * W;
+        if (nsplit > 1) {
+            const int64_t g = NS / W;
+            const int64_t s0 = g * y / nsplit * W, s1 = g * (y + 1) / nsplit * W;
             kl += s0;
             NS = s1 - s0;
-            C += y * sstride;
+            C += (int64_t)y * sstride;
         }
     }
     int64_t kbase = 0;                                 // first step of the current segment
@@ -1087,15 +1114,30 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
     if (ta * tb > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
-    const dim3 grid((unsigned)(ta * tb), (unsigned)nsplit);
+    /* One launch covers every part of a K split and every tile of `per`
+     * moduli: all of them, except for the int8 contraction with B-side
+     * lists over at least 1024 tiles, where a launch per modulus keeps one
+     * modulus' residues in L2 at a time (1% faster there; the 16-bit
+     * kernels and the small grids gain 4-5% from the single launch). */
+    const int per = (c->kind == CUTC_CRT_I8 && sp && !la && ta * tb * nsplit >= 1024) ? 1 : c->n;
+    if (ta * tb * nsplit * per > 2147483647)
+        return CUTC_ERR_ARG;
     const int64_t sstride = (int64_t)c->n * M * N; /* bytes between the parts of a K split */
-    for (int i = 0; i < c->n; i++) {
+    for (int z0 = 0; z0 < c->n; z0 += per) {
+        const int nz = c->n - z0 < per ? c->n - z0 : per;
+        const dim3 grid((unsigned)(ta * tb * nsplit * nz));
+        gemm_moduli mods;
+        for (int i = 0; i < CUTC_CRT_MAX; i++) {
+            mods.m[i] = i < nz ? c->m[z0 + i] : 1;
+            mods.inv_m[i] = i < nz ? c->inv_m[z0 + i] : 1.0;
+        }
+        const int8_t *Az = A + z0 * pa, *Bz = B + z0 * pb;
+        int8_t *Cz = C + (int64_t)z0 * M * N;
         /* int8: 6-stage rings, loads 4 steps ahead, a barrier every second
          * step; 16-bit: CRT16_SHAPE */
-        int8_t *Ci = C + (int64_t)i * M * N;
 #define GEMM_ARGS                                                                                                      \
-    grid, dim3(256), 0, st, A + i * pa, B + i * pb, KT, (int)ta, (int)tb, M, N, c->m[i], c->inv_m[i], Ci, accumulate, \
-        klist, kcount, sstride
+    grid, dim3(256), 0, st, Az, Bz, KT, (int)ta, (int)tb, M, N, mods, nsplit, pa, pb, Cz, accumulate, klist, kcount,   \
+        sstride
 #define GEMM_SP(K8, K16)                                                                                               \
     do {                                                                                                               \
         if (c->kind == CUTC_CRT_I8)                                                                                    \
