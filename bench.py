@@ -706,15 +706,17 @@ def main():
                 "work": f"2 x {npairs} slice products x {P} pairs x 4 n1 x n2 {args.precision} flops",
                 "fp64_equiv_tflops": exec1 / (last["ms_gemm1"] / 1e3) / 1e12}
     else:
-        launches = last["n_gemm1"] * opt.slices  # one launch per modulus and r1 chunk
+        # int8: one launch per modulus and r1 chunk (chunks of >= 1024 tiles,
+        # as on this workload); 16-bit: one launch per r1 chunk, all moduli
+        i8 = args.precision == "int8"
+        launches = last["n_gemm1"] * (opt.slices if i8 else 1)
         # the list-driven contraction runs k_density of the (256-pair tile,
         # K step) tiles: the executed work is the dense work times that
         dens = last["k_density"] if last["k_density"] > 0 else 1.0
         ops = 2.0 * opt.slices * P * 4 * s.n1 * s.n2 * dens
         achieved = ops / (last["ms_gemm1"] / 1e3) / 1e12
-        i8 = args.precision == "int8"
-        roof = {"kernel": ("crt_gemm_pk_kernel" if i8 else "crt16_gemm_kernel") +
-                          " (step-1 contraction, one launch per modulus and r1 chunk)",
+        roof = {"kernel": ("crt_gemm_pk_kernel (step-1 contraction, one launch per modulus and r1 chunk)" if i8
+                           else "crt16_gemm_kernel (step-1 contraction, one launch per r1 chunk, all moduli)"),
                 "bound": "mfma", "achieved": achieved, "peak": PEAK[args.precision],
                 "unit": "TOPS (int8)" if i8 else "TFLOP/s", "frac": achieved / PEAK[args.precision],
                 "traffic": traffic, "launches": launches, "avg_launch_ms": last["ms_gemm1"] / max(1, launches),
@@ -775,7 +777,7 @@ def main():
                 berr = errors_vs_fp64()
                 bdens = b["k_density"] if b["k_density"] > 0 else 1.0
                 bops = 2.0 * bopt.slices * P * 4 * s.n1 * s.n2 * bdens
-                blaunch = b["n_gemm1"] * bopt.slices
+                blaunch = b["n_gemm1"]  # one launch per r1 chunk, all moduli
                 out["bf16_split"] = {
                     "precision": f"bf16crt x{bopt.slices} moduli (bf16 MFMA, FP32 accumulation kept exact, CRT "
                                  "rebuild in FP64)",
@@ -784,8 +786,8 @@ def main():
                     "meets_1e-10": berr["max_rel_frob"] <= 1e-10,
                     "stage_ms": {k: b[k] for k in ("ms_jastrow", "ms_integrands", "ms_split", "ms_gemm1",
                                                    "ms_gemm2", "ms_symm")},
-                    "roofline": {"kernel": "crt16_gemm_kernel (step-1 contraction, one launch per modulus and r1 "
-                                           "chunk)", "bound": "mfma",
+                    "roofline": {"kernel": "crt16_gemm_kernel (step-1 contraction, one launch per r1 chunk, "
+                                           "all moduli)", "bound": "mfma",
                                  "achieved": bops / (b["ms_gemm1"] / 1e3) / 1e12, "peak": PEAK["bf16"],
                                  "unit": "TFLOP/s", "frac": bops / (b["ms_gemm1"] / 1e3) / 1e12 / PEAK["bf16"],
                                  "launches": blaunch, "avg_launch_ms": b["ms_gemm1"] / max(1, blaunch),

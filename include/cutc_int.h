@@ -142,7 +142,10 @@ typedef struct cutc_stats {
     double ms_gemm2;       /* step-2 contraction kernels                */
     double ms_symm;        /* M + M^T                                   */
     double ms_split;       /* slicing kernels of the split modes        */
-    int n_gemm1;           /* step-1 contraction launches               */
+    int n_gemm1;           /* step-1 contraction calls (one per r1 chunk
+                              and K pass; modular: every modulus, or for
+                              int8 chunks of >= 1024 tiles one launch per
+                              modulus)                                   */
     int n_gemm2;           /* step-2 contraction launches               */
     int n_jastrow;         /* Jastrow launches                          */
     int64_t n1_chunk;      /* r1 rows per chunk used                    */
