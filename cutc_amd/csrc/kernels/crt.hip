@@ -398,7 +398,8 @@ extern "C" int cutc_crt_quant_k_pk(const cutc_crt *c, const double *X, int64_t l
 // pair index.  occ (optional): byte [tb][kt] set to 1 when a row of the
 // CUTC_SP_ROWS-row B tile tb has a non-zero quantised value in K step kt (64 K for
 // int8, 32 for the 16-bit kinds); the caller clears it first.
-// grid (kpad / 128, ceil(P / 32)), as quant_k_pk_kernel: the 32 x 128 tile of
+// one block per (32-row block rb, 128-point range), nrb = ceil(P / 32) row
+// blocks fastest; as in quant_k_pk_kernel the 32 x 128 tile of
 // products is formed in LDS from coalesced reads (8 consecutive lanes read
 // 128 contiguous bytes of one AO row), then thread r = tid % 32 quantises
 // row 32 rb + r at the 16 points of j = tid / 32.  A wave's 64 threads then
@@ -410,10 +411,12 @@ __global__ __launch_bounds__(256) void quant_pairw_pk_kernel(cutc_crt c, const d
                                                              int64_t n2, int n_ao, int64_t KT, int64_t plane,
                                                              const int *__restrict__ rexp,
                                                              const int *__restrict__ ptab, int8_t *planes,
-                                                             uint8_t *occ)
+                                                             uint8_t *occ, int nrb)
 {
     __shared__ double tile[32][128 + 1]; // odd row pitch: the column reads below are conflict-free
-    const int64_t rb = blockIdx.y, kb = (int64_t)blockIdx.x * 128;
+    // row blocks fastest: the blocks in flight share one 128-point range of
+    // r2, whose AO values (n_ao x 1 KB) stay in L2 for all of them
+    const int64_t rb = blockIdx.x % nrb, kb = (int64_t)(blockIdx.x / nrb) * 128;
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     {
         const int rr = threadIdx.x >> 3, seg = threadIdx.x & 7; // 32 rows x 8 lanes
@@ -485,13 +488,14 @@ extern "C" int cutc_crt_quant_pairw_pk_occ(const cutc_crt *c, const double *wr2,
     const int64_t P = (int64_t)n_ao * (n_ao + 1) / 2;
     if (P <= 0 || kpad <= 0)
         return 0;
-    if (kpad % CUTC_PK_KALIGN || (P + 31) / 32 > 65535)
+    const int64_t nrb = (P + 31) / 32;
+    if (kpad % CUTC_PK_KALIGN || nrb * (kpad / 128) > 2147483647LL)
         return CUTC_ERR_ARG;
     const int64_t plane = CUTC_PK_ROWS_B(P) * kpad * CUTC_CRT_RBYTES(c->kind);
-    const dim3 grid((unsigned)(kpad / 128), (unsigned)((P + 31) / 32));
+    const dim3 grid((unsigned)(nrb * (kpad / 128)));
 #define QP(K_)                                                                                                         \
     hipLaunchKernelGGL(quant_pairw_pk_kernel<K_>, grid, dim3(256), 0, st, *c, wr2, aos2, ld2, n2, n_ao,                \
-                       pk_steps(K_, kpad), plane, rexp, ptab, planes, occ)
+                       pk_steps(K_, kpad), plane, rexp, ptab, planes, occ, (int)nrb)
     CRT_KIND_DISPATCH(c->kind, QP);
 #undef QP
     return hipGetLastError() == hipSuccess ? 0 : CUTC_ERR_HIP;
