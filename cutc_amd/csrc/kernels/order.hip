@@ -378,12 +378,13 @@ __device__ static inline void cl_pick_seed(const int *__restrict__ cnt, int64_t 
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned long long best = 0;
-    for (int64_t p = tid; p < P; p += CL_THREADS)
-        if (!taken[p]) {
-            const unsigned long long key =
-                ((unsigned long long)(unsigned)cnt[p] << 32) | (unsigned long long)(0xffffffffu - (uint32_t)p);
-            best = key > best ? key : best;
-        }
+#pragma unroll 8
+    for (int64_t p = tid; p < P; p += CL_THREADS) {
+        const int c = cnt[p];
+        const unsigned long long key =
+            ((unsigned long long)(unsigned)c << 32) | (unsigned long long)(0xffffffffu - (uint32_t)p);
+        best = (!taken[p] && key > best) ? key : best;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(best, off, 64);
         best = o > best ? o : best;
@@ -436,9 +437,12 @@ __global__ __launch_bounds__(CL_THREADS) void cl_select_kernel(const int *__rest
     for (int i = tid; i < CL_BINS; i += CL_THREADS)
         hist[i] = 0;
     __syncthreads();
-    for (int64_t p = tid; p < P; p += CL_THREADS)
+#pragma unroll 8
+    for (int64_t p = tid; p < P; p += CL_THREADS) {
+        const uint32_t d = dist[p];
         if (!taken[p])
-            atomicAdd(&hist[dist[p]], 1);
+            atomicAdd(&hist[d < CL_BINS ? d : CL_BINS - 1], 1);
+    }
     __syncthreads();
     // the smallest d* with need rows at distance <= d*
     constexpr int BPT = CL_BINS / CL_THREADS;
@@ -469,21 +473,23 @@ __global__ __launch_bounds__(CL_THREADS) void cl_select_kernel(const int *__rest
     const int64_t run = (P + CL_THREADS - 1) / CL_THREADS;
     const int64_t r0 = (int64_t)tid * run < P ? (int64_t)tid * run : P, r1 = r0 + run < P ? r0 + run : P;
     int nlt = 0, neq = 0;
-    for (int64_t p = r0; p < r1; p++)
-        if (!taken[p]) {
-            const uint32_t d = dist[p];
-            nlt += d < thr;
-            neq += d == thr;
-        }
+#pragma unroll 8
+    for (int64_t p = r0; p < r1; p++) {
+        const uint32_t d = dist[p];
+        const bool rem = !taken[p];
+        nlt += rem && d < thr;
+        neq += rem && d == thr;
+    }
     int teq, tlt;
     int eqr = cl_exscan(neq, tmp, &teq);
     const int ltr = cl_exscan(nlt, tmp, &tlt);
     // rows placed before this thread's: all below the threshold in earlier
     // runs and those at it among the first quota
     int pos = ltr + (eqr < quota ? eqr : quota);
-    for (int64_t p = r0; p < r1; p++)
+#pragma unroll 8
+    for (int64_t p = r0; p < r1; p++) {
+        const uint32_t d = dist[p];
         if (!taken[p]) {
-            const uint32_t d = dist[p];
             const int sel = d < thr || (d == thr && eqr < quota);
             eqr += d == thr;
             if (sel) {
@@ -491,6 +497,7 @@ __global__ __launch_bounds__(CL_THREADS) void cl_select_kernel(const int *__rest
                 taken[p] = 1;
             }
         }
+    }
     __syncthreads();
     if (base + need < P)
         cl_pick_seed(cnt, P, taken, sbest, state);
