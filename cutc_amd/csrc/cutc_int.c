@@ -984,7 +984,8 @@ static int stage1_split(const cutc_options *opt, const cutc_plan *pl, int64_t n1
             tr_end(tr, ev);
             TRY(rc);
             ev = tr_begin(tr, CAT_GEMM1);
-            rc = cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, kc, kl, kc_, 0, 1, Cpl, kp > 0, st);
+            rc = cutc_crt_gemm_pk_list(&crt, Gpl, 4 * n1e, Apl, P, kc, kl, kc_, 0, 1, Cpl,
+                                       (kp > 0) | CUTC_GEMM_KPASS, st);
             tr_end(tr, ev);
             TRY(rc);
             if (kl && i0 == 0 && stats && !tr->capture) {

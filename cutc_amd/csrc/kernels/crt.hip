@@ -1119,11 +1119,15 @@ extern "C" int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t
         return CUTC_ERR_ARG;
     const int64_t pa = CUTC_PK_ROWS_A(M) * kpad * rb, pb = CUTC_PK_ROWS_B(N) * kpad * rb;
     /* One launch covers every part of a K split and every tile of `per`
-     * moduli: all of them, except for the int8 contraction with B-side
-     * lists over at least 1024 tiles, where a launch per modulus keeps one
-     * modulus' residues in L2 at a time (1% faster there; the 16-bit
-     * kernels and the small grids gain 4-5% from the single launch). */
-    const int per = (c->kind == CUTC_CRT_I8 && sp && !la && ta * tb * nsplit >= 1024) ? 1 : c->n;
+     * moduli: all of them, except for the contraction with B-side lists over
+     * at least 1024 tiles in int8, or in 16-bit arithmetic as one of several
+     * K passes, where a launch per modulus keeps one modulus' residues in L2
+     * at a time (int8: 1% faster; the 16-bit K passes of the 40-nucleus
+     * build: 2-3% faster).  The 16-bit single pass and the small grids gain
+     * 4-5% from the single launch. */
+    const int kpass = (accumulate & CUTC_GEMM_KPASS) != 0;
+    accumulate &= 1;
+    const int per = (sp && !la && ta * tb * nsplit >= 1024 && (c->kind == CUTC_CRT_I8 || kpass)) ? 1 : c->n;
     if (ta * tb * nsplit * per > 2147483647)
         return CUTC_ERR_ARG;
     const int64_t sstride = (int64_t)c->n * M * N; /* bytes between the parts of a K split */

@@ -254,9 +254,12 @@ int cutc_crt_gemm_pk(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t
  * (list_a = 1); klist = kcount = NULL: every step.  nsplit > 1 (lists
  * only): each tile's list runs in nsplit consecutive parts (whole padding
  * units), part s into its own residues at C + s n M N, whose sum modulo
- * m_i is the whole result (cutc_crt_rebuild_m adds them).  The moduli and
- * parts share one launch (a launch per modulus for the int8 B-list
- * contraction over 1024 tiles or more) */
+ * m_i is the whole result (cutc_crt_rebuild_m adds them).  accumulate:
+ * bit 0 adds the products into C (modulo m_i); CUTC_GEMM_KPASS marks one of
+ * several K passes.  The moduli and parts share one launch (a launch per
+ * modulus for the B-list contraction over 1024 tiles or more in int8, or
+ * in a K pass) */
+#define CUTC_GEMM_KPASS 2
 int cutc_crt_gemm_pk_list(const cutc_crt *c, const int8_t *A, int64_t M, const int8_t *B, int64_t N, int64_t kpad,
                           const uint32_t *klist, const int *kcount, int list_a, int nsplit, int8_t *C,
                           int accumulate, hipStream_t st);
